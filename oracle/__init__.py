@@ -1,0 +1,249 @@
+"""ctypes front end of the CPU oracle (oracle/sdbf_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, by __graft_entry__.smoke() as the
+checker, and by bench.py's cpu_baseline leg.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_SO):
+            build()
+        L = C.CDLL(_SO)
+        P = C.c_void_p
+        L.orc_entropy_table.argtypes = [P]
+        L.orc_gen_ranks.argtypes = [P, C.c_uint64, P]
+        L.orc_gen_scores.argtypes = [P, C.c_uint64, P, P]
+        L.orc_bf_sha1_insert.argtypes = [P, P]
+        L.orc_bf_sha1_insert.restype = C.c_uint32
+        L.orc_sha1_words.argtypes = [P, P]
+        L.orc_digest_stream.argtypes = [P, C.c_uint64, C.POINTER(P), C.POINTER(C.c_uint32),
+                                        C.POINTER(C.c_uint32)]
+        L.orc_block_count.argtypes = [C.c_uint64, C.c_uint32]
+        L.orc_block_count.restype = C.c_uint64
+        L.orc_digest_block.argtypes = [P, C.c_uint64, C.c_uint32, P, P, C.c_int]
+        L.orc_block_threshold.argtypes = [P, C.POINTER(C.c_uint32), C.POINTER(C.c_int32)]
+        L.orc_hamming.argtypes = [P, C.c_uint32, P]
+        L.orc_bitcount_cut_256.argtypes = [P, P, C.c_uint32, C.c_int32]
+        L.orc_bitcount_cut_256.restype = C.c_uint32
+        L.orc_max_score.argtypes = [P, C.c_uint32, P]
+        L.orc_max_score.restype = C.c_double
+        L.orc_score.argtypes = [P, P]
+        L.orc_score.restype = C.c_int
+        L.orc_compare_all.argtypes = [P, C.c_uint32, P, C.c_int]
+        L.orc_compare_to.argtypes = [P, C.c_uint32, P, C.c_uint32, P, C.c_int]
+        L.orc_digest_stream_many.argtypes = [C.c_uint32, P, P, P, P, P, P, C.c_int]
+        L.orc_free.argtypes = [P]
+        L.orc_b64encode.argtypes = [P, C.c_uint64, P]
+        L.orc_b64encode.restype = C.c_uint64
+        L.orc_to_string.argtypes = [P, C.c_uint64, C.c_uint32, P]
+        L.orc_to_string.restype = C.c_uint64
+        L.orc_elem_count.argtypes = [P, C.c_uint64]
+        L.orc_elem_count.restype = C.c_int32
+        L.orc_threads_available.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _u8(buf) -> np.ndarray:
+    if isinstance(buf, np.ndarray):
+        return np.ascontiguousarray(buf, dtype=np.uint8)
+    return np.frombuffer(bytes(buf), dtype=np.uint8)
+
+
+def entropy_table() -> np.ndarray:
+    out = np.zeros(65, dtype=np.uint64)
+    lib().orc_entropy_table(_ptr(out))
+    return out
+
+
+def ranks(buf) -> np.ndarray:
+    b = _u8(buf)
+    r = np.zeros(b.size, dtype=np.uint16)
+    lib().orc_gen_ranks(_ptr(b), b.size, _ptr(r))
+    return r
+
+
+def scores(rk: np.ndarray, with_histo: bool = False):
+    rk = np.ascontiguousarray(rk, dtype=np.uint16)
+    s = np.zeros(rk.size, dtype=np.uint16)
+    h = np.zeros(66, dtype=np.int32)
+    lib().orc_gen_scores(_ptr(rk), rk.size, _ptr(s), _ptr(h) if with_histo else None)
+    return (s, h) if with_histo else s
+
+
+def sha1_words(block64) -> np.ndarray:
+    b = _u8(block64)
+    assert b.size >= 64
+    h = np.zeros(5, dtype=np.uint32)
+    lib().orc_sha1_words(_ptr(b), _ptr(h))
+    return h
+
+
+def bf_sha1_insert(bf: np.ndarray, h: np.ndarray) -> int:
+    return int(lib().orc_bf_sha1_insert(_ptr(bf), _ptr(np.ascontiguousarray(h, np.uint32))))
+
+
+def block_threshold(histo) -> tuple[int, int]:
+    h = np.ascontiguousarray(histo, dtype=np.int32)
+    k = C.c_uint32()
+    a = C.c_int32()
+    lib().orc_block_threshold(_ptr(h), C.byref(k), C.byref(a))
+    return k.value, a.value
+
+
+def bitcount_cut(a, b, cut_off: int, slack: int) -> int:
+    return int(lib().orc_bitcount_cut_256(_ptr(_u8(a)), _ptr(_u8(b)), cut_off, slack))
+
+
+class _CDigest(C.Structure):
+    _fields_ = [("bf", C.c_void_p), ("elem_counts", C.c_void_p), ("hamming", C.c_void_p),
+                ("name", C.c_char_p), ("bf_count", C.c_uint32), ("last_count", C.c_uint32),
+                ("max_elem", C.c_uint32)]
+
+
+@dataclass
+class Digest:
+    """One sdbf digest as the oracle computes it."""
+    name: bytes
+    filters: np.ndarray            # uint8 [bf_count * 256]
+    bf_count: int
+    last_count: int = 0            # stream mode
+    elem_counts: np.ndarray | None = None   # dd mode
+    orig_size: int = 0
+    dd_block_size: int = 0
+    hamming: np.ndarray = field(default=None)
+    _c: _CDigest | None = field(default=None, repr=False)
+
+    @property
+    def max_elem(self) -> int:
+        return 192 if self.elem_counts is not None else 160
+
+    def cstruct(self) -> _CDigest:
+        if self._c is None:
+            self.filters = np.ascontiguousarray(self.filters, dtype=np.uint8)
+            self.hamming = np.zeros(max(self.bf_count, 1), dtype=np.uint16)
+            lib().orc_hamming(_ptr(self.filters), self.bf_count, _ptr(self.hamming))
+            ec = None
+            if self.elem_counts is not None:
+                self.elem_counts = np.ascontiguousarray(self.elem_counts, dtype=np.uint16)
+                ec = self.elem_counts.ctypes.data
+            self._c = _CDigest(self.filters.ctypes.data, ec, self.hamming.ctypes.data,
+                               self.name, self.bf_count, self.last_count, self.max_elem)
+        return self._c
+
+    def elem_count(self, i: int) -> int:
+        return int(lib().orc_elem_count(C.byref(self.cstruct()), i))
+
+    def to_string(self) -> str:
+        c = self.cstruct()
+        n = lib().orc_to_string(C.byref(c), self.orig_size, self.dd_block_size, None)
+        buf = C.create_string_buffer(int(n) + 1)
+        m = lib().orc_to_string(C.byref(c), self.orig_size, self.dd_block_size, buf)
+        return buf.raw[:m].decode("ascii")
+
+
+def digest_stream(buf, name: bytes = b"x") -> Digest:
+    b = _u8(buf)
+    p = C.c_void_p()
+    n = C.c_uint32()
+    last = C.c_uint32()
+    rc = lib().orc_digest_stream(_ptr(b), b.size, C.byref(p), C.byref(n), C.byref(last))
+    if rc != 0:
+        raise ValueError(rc)
+    f = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint8)), shape=(n.value * 256,)).copy()
+    lib().orc_free(p)
+    return Digest(name, f, n.value, last.value, None, b.size, 0)
+
+
+def digest_block(buf, dd: int = 16384, name: bytes = b"x", threads: int = 1) -> Digest:
+    b = _u8(buf)
+    if b.size < 512:
+        raise ValueError(-3)
+    cnt = int(lib().orc_block_count(b.size, dd))
+    f = np.zeros(cnt * 256, dtype=np.uint8)
+    ec = np.zeros(max(cnt, 1), dtype=np.uint16)
+    rc = lib().orc_digest_block(_ptr(b), b.size, dd, _ptr(f), _ptr(ec), threads)
+    if rc != 0:
+        raise ValueError(rc)
+    return Digest(name, f, cnt, 0, ec[:cnt], b.size, dd)
+
+
+def digest_stream_many(bufs, names=None, threads: int = 1) -> list[Digest]:
+    arrs = [_u8(b) for b in bufs]
+    n = len(arrs)
+    ptrs = (C.c_void_p * n)(*[a.ctypes.data for a in arrs])
+    lens = np.array([a.size for a in arrs], dtype=np.uint64)
+    fptr = (C.c_void_p * n)()
+    cnt = np.zeros(n, dtype=np.uint32)
+    last = np.zeros(n, dtype=np.uint32)
+    st = np.zeros(n, dtype=np.int32)
+    lib().orc_digest_stream_many(n, ptrs, _ptr(lens), fptr, _ptr(cnt), _ptr(last), _ptr(st),
+                                 threads)
+    out = []
+    for i in range(n):
+        if st[i] != 0:
+            out.append(None)
+            continue
+        f = np.ctypeslib.as_array(C.cast(fptr[i], C.POINTER(C.c_uint8)),
+                                  shape=(int(cnt[i]) * 256,)).copy()
+        lib().orc_free(fptr[i])
+        nm = names[i] if names is not None else b"f%05d" % i
+        out.append(Digest(nm, f, int(cnt[i]), int(last[i]), None, arrs[i].size, 0))
+    return out
+
+
+def max_score(ref: Digest, i: int, tgt: Digest) -> float:
+    return float(lib().orc_max_score(C.byref(ref.cstruct()), i, C.byref(tgt.cstruct())))
+
+
+def score(a: Digest, b: Digest) -> int:
+    return int(lib().orc_score(C.byref(a.cstruct()), C.byref(b.cstruct())))
+
+
+def _array(ds):
+    arr = (_CDigest * len(ds))()
+    for i, d in enumerate(ds):
+        arr[i] = d.cstruct()
+    return arr
+
+
+def compare_all(ds: list[Digest], threads: int = 1) -> np.ndarray:
+    """Scores of all pairs i<j in row-major order."""
+    n = len(ds)
+    out = np.zeros(n * (n - 1) // 2, dtype=np.int32)
+    if n > 1:
+        lib().orc_compare_all(_array(ds), n, _ptr(out), threads)
+    return out
+
+
+def compare_to(q: list[Digest], t: list[Digest], threads: int = 1) -> np.ndarray:
+    out = np.zeros((len(q), len(t)), dtype=np.int32)
+    if len(q) and len(t):
+        lib().orc_compare_to(_array(q), len(q), _array(t), len(t), _ptr(out), threads)
+    return out
+
+
+def threads_available() -> int:
+    return int(lib().orc_threads_available())

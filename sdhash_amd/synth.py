@@ -1,0 +1,124 @@
+"""Deterministic synthetic inputs of the "segment mix" used by the tests and
+the benchmark.
+
+Every input is a concatenation of 4 KiB segments, drawn
+  40 % uniform random bytes,
+  40 % Zipf(s=1.1) over a 64-symbol alphabet (integer CDF table),
+  10 % a random 16-byte motif repeated,
+  10 % zeros,
+and sets of files get chunks of 4-64 KiB copied between them ("planted"),
+so that pairwise similarity scores span 0-100.  All draws come from numpy's
+PCG64 integer streams, so a seed gives the same bytes on every host.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+SEG = 4096
+KIB = 1024
+MIB = 1 << 20
+
+# Zipf(s=1.1) over 64 ranks as an integer CDF with total 2**24.
+_W = np.arange(1, 65, dtype=np.float64) ** -1.1
+_CDF = np.round(np.cumsum(_W) / _W.sum() * (1 << 24)).astype(np.int64)
+_CDF[-1] = 1 << 24
+
+
+def _rng(seed: int) -> np.random.Generator:
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def segment_mix(rng: np.random.Generator, nbytes: int) -> np.ndarray:
+    """nbytes of segment-mix data (the last segment is cut to size)."""
+    nseg = (nbytes + SEG - 1) // SEG
+    out = np.zeros((nseg, SEG), dtype=np.uint8)
+    kind = rng.integers(0, 10, nseg)
+    uni = np.flatnonzero(kind < 4)
+    zpf = np.flatnonzero((kind >= 4) & (kind < 8))
+    mot = np.flatnonzero(kind == 8)
+    if uni.size:
+        out[uni] = rng.integers(0, 256, (uni.size, SEG), dtype=np.uint8)
+    if zpf.size:
+        keys = rng.integers(0, 1 << 32, (zpf.size, 256), dtype=np.uint64)
+        alpha = np.argsort(keys, axis=1, kind="stable")[:, :64].astype(np.uint8)
+        draw = rng.integers(0, 1 << 24, (zpf.size, SEG), dtype=np.int64)
+        sym = np.searchsorted(_CDF, draw, side="right")
+        out[zpf] = np.take_along_axis(alpha, sym, axis=1)
+    if mot.size:
+        motif = rng.integers(0, 256, (mot.size, 16), dtype=np.uint8)
+        out[mot] = np.tile(motif, (1, SEG // 16))
+    return out.reshape(-1)[:nbytes].copy()
+
+
+def plant(rng: np.random.Generator, files: list[np.ndarray], max_frac: float,
+          lo: int = 4 * KIB, hi: int = 64 * KIB) -> list[np.ndarray]:
+    """Copy 4-64 KiB chunks of one partner file's original content into each
+    file until a fraction drawn from [0, max_frac] of it is covered."""
+    src = [f.copy() for f in files]
+    out = [f.copy() for f in files]
+    n = len(files)
+    if n < 2:
+        return out
+    for i in range(n):
+        size = out[i].size
+        frac = int(rng.integers(0, int(round(max_frac * 100)) + 1)) / 100.0
+        target = int(frac * size)
+        # one partner file per receiver, so that planted pairs share a lot
+        s = int(rng.integers(0, n - 1))
+        s = s + 1 if s >= i else s
+        done = 0
+        while done < target:
+            c = int(rng.integers(lo, hi + 1))
+            if src[s].size < c or size < c:
+                break
+            so = int(rng.integers(0, src[s].size - c + 1))
+            do = int(rng.integers(0, size - c + 1))
+            out[i][do:do + c] = src[s][so:so + c]
+            done += c
+    return out
+
+
+def file_set(seed: int, n: int, size: int, max_frac: float = 0.5) -> list[np.ndarray]:
+    """n files of `size` bytes with planted cross-file chunks (0..max_frac)."""
+    rng = _rng(seed)
+    files = [segment_mix(rng, size) for _ in range(n)]
+    return plant(rng, files, max_frac)
+
+
+def ragged_set(seed: int, sizes, max_frac: float = 0.05) -> list[np.ndarray]:
+    """Files of the given sizes with planted chunks."""
+    rng = _rng(seed)
+    files = [segment_mix(rng, int(s)) for s in sizes]
+    return plant(rng, files, max_frac)
+
+
+def lognormal_sizes(seed: int, n: int, median: int = 256 * KIB, sigma: float = 1.5,
+                    lo: int = 4 * KIB, hi: int = 64 * MIB) -> np.ndarray:
+    rng = _rng(seed)
+    z = rng.standard_normal(n)
+    s = np.exp(np.log(median) + sigma * z)
+    return np.clip(s, lo, hi).astype(np.int64)
+
+
+def disk_image(seed: int, nbytes: int, block: int = 64 * MIB) -> np.ndarray:
+    """A disk image of segment-mix data, generated in pieces."""
+    rng = _rng(seed)
+    out = np.empty(nbytes, dtype=np.uint8)
+    pos = 0
+    while pos < nbytes:
+        n = min(block, nbytes - pos)
+        out[pos:pos + n] = segment_mix(rng, n)
+        pos += n
+    return out
+
+
+def fragments(seed: int, image: np.ndarray, n: int, size: int = 4 * KIB) -> list[np.ndarray]:
+    """n fragments: the first half cut from the image at random offsets,
+    the second half fresh uniform random bytes."""
+    rng = _rng(seed)
+    half = n // 2
+    offs = rng.integers(0, image.size - size + 1, half)
+    out = [image[o:o + size].copy() for o in offs]
+    fresh = rng.integers(0, 256, (n - half, size), dtype=np.uint8)
+    out.extend(fresh[i] for i in range(n - half))
+    return out
