@@ -1,0 +1,142 @@
+/*
+ * sdbfx.h -- C ABI of the MI355X (gfx950) sdbf engine.
+ *
+ * Plain pointers and sizes only.  Every call returns 0 on success or a
+ * negative status; sdbfx_last_error() describes the last failure of a
+ * context.  A context owns one HIP stream on one device and is not
+ * thread-safe: use one context per host thread.
+ *
+ * Status codes follow the reference's exception codes
+ * (sdbf/sdbf_class.cc:37,81,117,146,153,192,209):
+ *    0  ok
+ *   -1  cannot process (bad arguments, device failure, out of memory)
+ *   -2  invalid digest format
+ *   -3  input too small (< 512 bytes)
+ *
+ * The reference binds this path through C++ classes (sdbf, sdbf_set); each
+ * entry point below names the reference function it replaces.
+ */
+#ifndef SDBFX_H
+#define SDBFX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sdbfx_ctx sdbfx_ctx;   /* device + stream + scratch buffers */
+typedef struct sdbfx_set sdbfx_set;   /* device-resident collection of digests */
+
+/* Per-digest metadata, as held by class sdbf (sdbf/sdbf_class.h:95-117). */
+typedef struct {
+    const char *name;        /* hashname */
+    uint64_t orig_size;      /* orig_file_size */
+    uint32_t bf_count;       /* number of 256-byte filters */
+    uint32_t last_count;     /* stream mode: elements in the last filter */
+    uint32_t dd_block_size;  /* 0 for stream mode, else bytes per dd-block */
+    uint32_t max_elem;       /* 160 stream, 192 block */
+} sdbfx_digest_info;
+
+/* ---- context ------------------------------------------------------- */
+
+/* Opens device `device`; mem_budget (bytes, 0 = no limit) bounds the
+ * scratch the context allocates.  Replaces the process-global
+ * sdbf::config set-up (sdbf/sdbf_class.cc:25, sdhash-src/sdhash.cc:227). */
+int sdbfx_ctx_create(int device, uint64_t mem_budget, sdbfx_ctx **out);
+void sdbfx_ctx_destroy(sdbfx_ctx *ctx);
+const char *sdbfx_last_error(const sdbfx_ctx *ctx);
+/* waits for all work of the context */
+int sdbfx_sync(sdbfx_ctx *ctx);
+
+/* ---- digest sets ---------------------------------------------------- */
+
+/* Empty set (sdbf_set::sdbf_set(), sdbf/sdbf_set.cc:29-34). */
+int sdbfx_set_create(sdbfx_ctx *ctx, sdbfx_set **out);
+void sdbfx_set_destroy(sdbfx_set *set);
+uint32_t sdbfx_set_size(const sdbfx_set *set);
+uint64_t sdbfx_set_filter_count(const sdbfx_set *set);
+
+/*
+ * Digests n host buffers and appends the digests, in order, to `set`.
+ * dd_block[i] == 0 selects stream mode, else block mode with that dd-block
+ * size in bytes (<= 16384); dd_block == NULL means stream mode for all.
+ * status[i] receives 0 or -3 (too small: nothing appended for that input).
+ * Replaces sdbf::sdbf(name, char*, dd, len, NULL) (sdbf/sdbf_class.cc:115-136)
+ * and sdbf::sdbf(name, istream*, dd, size, NULL) (sdbf/sdbf_class.cc:71-102)
+ * as called in batches by sdbf_hash_files / sdbf_hash_files_dd
+ * (sdhash-src/sdhash_threads.cc:241-373) followed by sdbf_set::add
+ * (sdbf/sdbf_set.cc:105-109).
+ */
+int sdbfx_set_digest(sdbfx_ctx *ctx, sdbfx_set *set, uint32_t n, const uint8_t *const *bufs,
+                     const uint64_t *lens, const char *const *names, const uint32_t *dd_block,
+                     int32_t *status);
+
+/*
+ * As sdbfx_set_digest, for inputs already in device memory: input i is at
+ * d_base + offsets[i].  Offsets must be multiples of 256 and the 256 bytes
+ * after each input must be readable (sdbfx_device_alloc pads for this).
+ */
+int sdbfx_set_digest_device(sdbfx_ctx *ctx, sdbfx_set *set, uint32_t n, const void *d_base,
+                            const uint64_t *offsets, const uint64_t *lens, const char *const *names,
+                            const uint32_t *dd_block, int32_t *status);
+
+/*
+ * Appends one already-computed digest (e.g. parsed from sdbf text).
+ * elem_counts is required for block mode (dd_block_size > 0) and ignored
+ * for stream mode.  Replaces sdbf::sdbf(const std::string&)'s fill of the
+ * object (sdbf/sdbf_class.cc:144-215) followed by sdbf_set::add.
+ */
+int sdbfx_set_add(sdbfx_ctx *ctx, sdbfx_set *set, const char *name, uint64_t orig_size,
+                  uint32_t bf_count, uint32_t last_count, uint32_t dd_block_size, uint32_t max_elem,
+                  const uint8_t *filters, const uint16_t *elem_counts);
+
+/* Metadata of digest idx (sdbf::name/size/input_size/filter_count,
+ * sdbf/sdbf_class.cc:236-256,418-420).  info->name stays valid while the set lives. */
+int sdbfx_set_info(const sdbfx_set *set, uint32_t idx, sdbfx_digest_info *info);
+
+/* Copies digest idx back to the host: bf_count*256 filter bytes, bf_count
+ * element counts (sdbf::get_elem_count, sdbf/sdbf_class.cc:407-415) and
+ * bf_count Hamming weights (sdbf::compute_hamming, sdbf/sdbf_class.cc:389-401).
+ * Any output pointer may be NULL. */
+int sdbfx_set_read(sdbfx_ctx *ctx, const sdbfx_set *set, uint32_t idx, uint8_t *filters,
+                   uint16_t *elem_counts, uint16_t *hamming);
+
+/* ---- comparison ----------------------------------------------------- */
+
+/*
+ * All pairs i < j of `set`, in set order: scores[k] for the k-th pair in
+ * row-major i<j order is items[i].compare(items[j], 0), an integer in
+ * -1..100.  Replaces sdbf_set::compare_all / compare_all_quiet
+ * (sdbf/sdbf_set.cc:240-308) -> sdbf::compare -> sdbf_score ->
+ * sdbf_max_score (sdbf/sdbf_core.cc:331-455).  scores holds n*(n-1)/2 ints.
+ */
+int sdbfx_compare_all(sdbfx_ctx *ctx, const sdbfx_set *set, int32_t *scores);
+
+/*
+ * Query x target: scores[i*nt + j] = q[i].compare(t[j], 0).  Replaces
+ * sdbf_set::compare_to / compare_to_quiet (sdbf/sdbf_set.cc:318-394).
+ */
+int sdbfx_compare_to(sdbfx_ctx *ctx, const sdbfx_set *q, const sdbfx_set *t, int32_t *scores);
+
+/* ---- device memory for device-resident inputs ------------------------ */
+
+/* Allocates `bytes` of device memory plus 256 readable padding bytes. */
+int sdbfx_device_alloc(sdbfx_ctx *ctx, uint64_t bytes, void **dptr);
+int sdbfx_device_free(sdbfx_ctx *ctx, void *dptr);
+int sdbfx_upload(sdbfx_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+
+/* ---- kernel timing -------------------------------------------------- */
+
+/* When enabled, every kernel launch of the context is bracketed by HIP
+ * events on the context's stream.  sdbfx_kernel_time returns the summed
+ * duration (ms) and launch count of kernel `name` since the last reset. */
+int sdbfx_set_profiling(sdbfx_ctx *ctx, int on);
+int sdbfx_kernel_time(sdbfx_ctx *ctx, const char *name, double *ms, uint64_t *launches);
+int sdbfx_reset_kernel_times(sdbfx_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,265 @@
+"""MI355X sdbf engine: Python front end over the C ABI of include/sdbfx.h.
+
+The compute path is libsdbfx.so (gfx950 kernels); there is no CPU fallback:
+if the library is missing or no GPU is present, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsdbfx.so")
+
+# C ABI entry points (all declared in include/sdbfx.h)
+SYMBOLS = (
+    "sdbfx_ctx_create", "sdbfx_ctx_destroy", "sdbfx_last_error", "sdbfx_sync",
+    "sdbfx_set_create", "sdbfx_set_destroy", "sdbfx_set_size", "sdbfx_set_filter_count",
+    "sdbfx_set_digest", "sdbfx_set_digest_device", "sdbfx_set_add", "sdbfx_set_info",
+    "sdbfx_set_read", "sdbfx_compare_all", "sdbfx_compare_to", "sdbfx_device_alloc",
+    "sdbfx_device_free", "sdbfx_upload", "sdbfx_set_profiling", "sdbfx_kernel_time",
+    "sdbfx_reset_kernel_times",
+)
+
+_lib = None
+
+
+class SdbfxError(RuntimeError):
+    pass
+
+
+class _Info(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("orig_size", C.c_uint64), ("bf_count", C.c_uint32),
+                ("last_count", C.c_uint32), ("dd_block_size", C.c_uint32), ("max_elem", C.c_uint32)]
+
+
+def load():
+    """Loads libsdbfx.so (built by sdhash_amd.build); raises when absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SdbfxError(f"{LIB_PATH} is not built (run python -m sdhash_amd.build)")
+    L = C.CDLL(LIB_PATH)
+    P, U32, U64, I32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int32
+    L.sdbfx_ctx_create.argtypes = [C.c_int, U64, C.POINTER(P)]
+    L.sdbfx_ctx_destroy.argtypes = [P]
+    L.sdbfx_ctx_destroy.restype = None
+    L.sdbfx_last_error.argtypes = [P]
+    L.sdbfx_last_error.restype = C.c_char_p
+    L.sdbfx_sync.argtypes = [P]
+    L.sdbfx_set_create.argtypes = [P, C.POINTER(P)]
+    L.sdbfx_set_destroy.argtypes = [P]
+    L.sdbfx_set_destroy.restype = None
+    L.sdbfx_set_size.argtypes = [P]
+    L.sdbfx_set_size.restype = U32
+    L.sdbfx_set_filter_count.argtypes = [P]
+    L.sdbfx_set_filter_count.restype = U64
+    L.sdbfx_set_digest.argtypes = [P, P, U32, P, P, P, P, P]
+    L.sdbfx_set_digest_device.argtypes = [P, P, U32, P, P, P, P, P, P]
+    L.sdbfx_set_add.argtypes = [P, P, C.c_char_p, U64, U32, U32, U32, U32, P, P]
+    L.sdbfx_set_info.argtypes = [P, U32, C.POINTER(_Info)]
+    L.sdbfx_set_read.argtypes = [P, P, U32, P, P, P]
+    L.sdbfx_compare_all.argtypes = [P, P, P]
+    L.sdbfx_compare_to.argtypes = [P, P, P, P]
+    L.sdbfx_device_alloc.argtypes = [P, U64, C.POINTER(P)]
+    L.sdbfx_device_free.argtypes = [P, P]
+    L.sdbfx_upload.argtypes = [P, P, P, U64]
+    L.sdbfx_set_profiling.argtypes = [P, C.c_int]
+    L.sdbfx_kernel_time.argtypes = [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(U64)]
+    L.sdbfx_reset_kernel_times.argtypes = [P]
+    for name in SYMBOLS:
+        getattr(L, name).restype = getattr(L, name).restype or I32
+    for name in ("sdbfx_ctx_create", "sdbfx_sync", "sdbfx_set_create", "sdbfx_set_digest",
+                 "sdbfx_set_digest_device", "sdbfx_set_add", "sdbfx_set_info", "sdbfx_set_read",
+                 "sdbfx_compare_all", "sdbfx_compare_to", "sdbfx_device_alloc", "sdbfx_device_free",
+                 "sdbfx_upload", "sdbfx_set_profiling", "sdbfx_kernel_time",
+                 "sdbfx_reset_kernel_times"):
+        getattr(L, name).restype = I32
+    _lib = L
+    return L
+
+
+def _ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+def _bytes_array(items) -> tuple[list, C.Array]:
+    arrs = [np.ascontiguousarray(np.frombuffer(x, np.uint8) if isinstance(x, (bytes, bytearray))
+                                 else x, dtype=np.uint8) for x in items]
+    ptrs = (C.c_void_p * max(len(arrs), 1))(*[a.ctypes.data for a in arrs])
+    return arrs, ptrs
+
+
+@dataclass
+class DigestInfo:
+    name: str
+    orig_size: int
+    bf_count: int
+    last_count: int
+    dd_block_size: int
+    max_elem: int
+
+
+class DeviceBuffer:
+    """Device memory with 256 readable padding bytes after `nbytes`."""
+
+    def __init__(self, engine: "Engine", nbytes: int):
+        self.engine = engine
+        self.nbytes = nbytes
+        p = C.c_void_p()
+        engine._check(engine.lib.sdbfx_device_alloc(engine.ctx, nbytes, C.byref(p)))
+        self.ptr = p
+
+    def upload(self, offset: int, data: np.ndarray) -> None:
+        a = np.ascontiguousarray(data, dtype=np.uint8)
+        assert offset + a.size <= self.nbytes
+        self.engine._check(self.engine.lib.sdbfx_upload(
+            self.engine.ctx, C.c_void_p(self.ptr.value + offset), _ptr(a), a.size))
+
+    def free(self) -> None:
+        if self.ptr:
+            self.engine.lib.sdbfx_device_free(self.engine.ctx, self.ptr)
+            self.ptr = None
+
+
+class DigestSet:
+    """A device-resident collection of sdbf digests (the engine's sdbf_set)."""
+
+    def __init__(self, engine: "Engine"):
+        self.engine = engine
+        p = C.c_void_p()
+        engine._check(engine.lib.sdbfx_set_create(engine.ctx, C.byref(p)))
+        self.handle = p
+
+    def __len__(self) -> int:
+        return int(self.engine.lib.sdbfx_set_size(self.handle))
+
+    def filter_count(self) -> int:
+        return int(self.engine.lib.sdbfx_set_filter_count(self.handle))
+
+    def info(self, i: int) -> DigestInfo:
+        x = _Info()
+        if self.engine.lib.sdbfx_set_info(self.handle, i, C.byref(x)) != 0:
+            raise IndexError(i)
+        return DigestInfo(x.name.decode("utf-8", "surrogateescape"), x.orig_size, x.bf_count,
+                          x.last_count, x.dd_block_size, x.max_elem)
+
+    def read(self, i: int):
+        """(filters uint8[bf_count*256], elem_counts uint16[bf_count], hamming uint16[bf_count])"""
+        inf = self.info(i)
+        f = np.zeros(inf.bf_count * 256, np.uint8)
+        ec = np.zeros(inf.bf_count, np.uint16)
+        hm = np.zeros(inf.bf_count, np.uint16)
+        self.engine._check(self.engine.lib.sdbfx_set_read(self.engine.ctx, self.handle, i, _ptr(f),
+                                                          _ptr(ec), _ptr(hm)))
+        return f, ec, hm
+
+    def digest(self, bufs, names=None, dd_block=0) -> np.ndarray:
+        """Digests host buffers into this set; returns the per-input status."""
+        n = len(bufs)
+        arrs, ptrs = _bytes_array(bufs)
+        lens = np.array([a.size for a in arrs], np.uint64)
+        dd = self._dd(dd_block, n)
+        nm = self._names(names, n)
+        st = np.zeros(max(n, 1), np.int32)
+        self.engine._check(self.engine.lib.sdbfx_set_digest(
+            self.engine.ctx, self.handle, n, ptrs, _ptr(lens), nm, _ptr(dd), _ptr(st)))
+        return st[:n]
+
+    def digest_device(self, dbuf: DeviceBuffer, offsets, lens, names=None, dd_block=0) -> np.ndarray:
+        n = len(lens)
+        off = np.ascontiguousarray(offsets, np.uint64)
+        ln = np.ascontiguousarray(lens, np.uint64)
+        dd = self._dd(dd_block, n)
+        nm = self._names(names, n)
+        st = np.zeros(max(n, 1), np.int32)
+        self.engine._check(self.engine.lib.sdbfx_set_digest_device(
+            self.engine.ctx, self.handle, n, dbuf.ptr, _ptr(off), _ptr(ln), nm, _ptr(dd), _ptr(st)))
+        return st[:n]
+
+    def add(self, name: bytes, orig_size: int, bf_count: int, last_count: int, dd_block_size: int,
+            max_elem: int, filters: np.ndarray, elem_counts: np.ndarray | None) -> None:
+        f = np.ascontiguousarray(filters, np.uint8)
+        ec = None if elem_counts is None else np.ascontiguousarray(elem_counts, np.uint16)
+        self.engine._check(self.engine.lib.sdbfx_set_add(
+            self.engine.ctx, self.handle, name, orig_size, bf_count, last_count, dd_block_size,
+            max_elem, _ptr(f), None if ec is None else _ptr(ec)))
+
+    @staticmethod
+    def _dd(dd_block, n):
+        if np.isscalar(dd_block):
+            return np.full(max(n, 1), int(dd_block), np.uint32)
+        return np.ascontiguousarray(dd_block, np.uint32)
+
+    def _names(self, names, n):
+        if names is None:
+            names = [b"f%06d" % i for i in range(n)]
+        self._keep = [x if isinstance(x, bytes) else str(x).encode() for x in names]
+        return (C.c_char_p * max(n, 1))(*self._keep)
+
+    def close(self) -> None:
+        if self.handle:
+            self.engine.lib.sdbfx_set_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Engine:
+    """One device context (HIP stream) of the engine."""
+
+    def __init__(self, device: int = 0, mem_budget: int = 0):
+        self.lib = load()
+        p = C.c_void_p()
+        if self.lib.sdbfx_ctx_create(device, mem_budget, C.byref(p)) != 0:
+            raise SdbfxError(f"cannot open HIP device {device}")
+        self.ctx = p
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise SdbfxError(self.lib.sdbfx_last_error(self.ctx).decode())
+
+    def new_set(self) -> DigestSet:
+        return DigestSet(self)
+
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    def compare_all(self, s: DigestSet) -> np.ndarray:
+        n = len(s)
+        out = np.zeros(max(n * (n - 1) // 2, 1), np.int32)
+        self._check(self.lib.sdbfx_compare_all(self.ctx, s.handle, _ptr(out)))
+        return out[: n * (n - 1) // 2]
+
+    def compare_to(self, q: DigestSet, t: DigestSet) -> np.ndarray:
+        out = np.zeros((max(len(q), 1), max(len(t), 1)), np.int32)
+        self._check(self.lib.sdbfx_compare_to(self.ctx, q.handle, t.handle, _ptr(out)))
+        return out[: len(q), : len(t)]
+
+    def sync(self) -> None:
+        self._check(self.lib.sdbfx_sync(self.ctx))
+
+    def profiling(self, on: bool) -> None:
+        self._check(self.lib.sdbfx_set_profiling(self.ctx, 1 if on else 0))
+
+    def kernel_time(self, name: str) -> tuple[float, int]:
+        ms = C.c_double()
+        n = C.c_uint64()
+        self._check(self.lib.sdbfx_kernel_time(self.ctx, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def reset_kernel_times(self) -> None:
+        self._check(self.lib.sdbfx_reset_kernel_times(self.ctx))
+
+    def close(self) -> None:
+        if self.ctx:
+            self.lib.sdbfx_ctx_destroy(self.ctx)
+            self.ctx = None

@@ -1,0 +1,788 @@
+// Host scheduler of the sdbf engine: implements the C ABI of include/sdbfx.h
+// on top of the gfx950 kernels (compiled in this translation unit).
+//
+// Digesting: inputs (packed at 256-byte aligned offsets in device memory) are
+// cut into stream-mode tiles or block-mode dd-blocks; the tile kernels run,
+// the ordered insertion pass builds stream-mode filters, and everything lands
+// in the set's device arrays (filters, per-filter element counts, Hamming
+// weights) in input order.
+//
+// Comparing: every digest pair is oriented on the host with the reference's
+// swap rule (sdbf/sdbf_core.cc:349-359), pairs are grouped by target digest,
+// and the compare kernel + ordered-sum kernel produce one score per pair.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/sdbfx.h"
+#include "sdbfx_kernels.hip"
+
+namespace {
+
+// Largest input volume digested by one device pass (bounds scratch memory).
+constexpr uint64_t kPassBytes = 4ull << 30;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+}  // namespace
+
+struct sdbfx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    std::string err;
+    bool prof = false;
+    struct Ev {
+        const char* name;
+        hipEvent_t a, b;
+    };
+    std::vector<Ev> pending;
+    std::map<std::string, std::pair<double, uint64_t>> times;
+    std::vector<hipEvent_t> evpool;
+    DevBuf stage, blocks, tiles, tstate, units, files, cand, sfilt, sham, counts, gsrc, gdst, gcnt,
+        glast, groups, pairs, partial, scores;
+};
+
+struct sdbfx_set {
+    int device = 0;
+    std::vector<std::string> names;
+    std::vector<uint64_t> orig, first;
+    std::vector<uint32_t> bfc, last, dd, maxe, last_ec;
+    uint64_t nfilt = 0;
+    DevBuf filt, ecf, hamf;
+};
+
+namespace {
+
+int fail(sdbfx_ctx* ctx, const std::string& msg)
+{
+    if (ctx)
+        ctx->err = msg;
+    return -1;
+}
+
+#define HIPC(ctx, call)                                                                    \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(ctx, std::string(#call) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+// grows a device buffer to at least `bytes` (contents not kept)
+int ensure(sdbfx_ctx* ctx, DevBuf& b, size_t bytes)
+{
+    if (b.cap >= bytes && b.p)
+        return 0;
+    if (b.p) {
+        HIPC(ctx, hipStreamSynchronize(ctx->stream));
+        HIPC(ctx, hipFree(b.p));
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    size_t cap = std::max<size_t>(bytes, 256);
+    cap = (cap + 4095) & ~(size_t)4095;
+    HIPC(ctx, hipMalloc(&b.p, cap));
+    b.cap = cap;
+    return 0;
+}
+
+// grows a device buffer keeping its first `keep` bytes
+int grow_keep(sdbfx_ctx* ctx, DevBuf& b, size_t bytes, size_t keep)
+{
+    if (b.cap >= bytes && b.p)
+        return 0;
+    size_t cap = std::max<size_t>(bytes, b.cap + b.cap / 2);
+    cap = (std::max<size_t>(cap, 4096) + 4095) & ~(size_t)4095;
+    void* np = nullptr;
+    HIPC(ctx, hipMalloc(&np, cap));
+    if (b.p) {
+        if (keep)
+            HIPC(ctx, hipMemcpyAsync(np, b.p, keep, hipMemcpyDeviceToDevice, ctx->stream));
+        HIPC(ctx, hipStreamSynchronize(ctx->stream));
+        HIPC(ctx, hipFree(b.p));
+    }
+    b.p = np;
+    b.cap = cap;
+    return 0;
+}
+
+template <typename T>
+int upload(sdbfx_ctx* ctx, DevBuf& b, const std::vector<T>& v)
+{
+    if (int rc = ensure(ctx, b, v.size() * sizeof(T) + 16))
+        return rc;
+    if (!v.empty())
+        HIPC(ctx, hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice,
+                                 ctx->stream));
+    return 0;
+}
+
+hipEvent_t get_event(sdbfx_ctx* ctx)
+{
+    if (!ctx->evpool.empty()) {
+        hipEvent_t e = ctx->evpool.back();
+        ctx->evpool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess)
+        return nullptr;
+    return e;
+}
+
+int collect_times(sdbfx_ctx* ctx)
+{
+    if (ctx->pending.empty())
+        return 0;
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto& ev : ctx->pending) {
+        float ms = 0.f;
+        HIPC(ctx, hipEventElapsedTime(&ms, ev.a, ev.b));
+        auto& t = ctx->times[ev.name];
+        t.first += ms;
+        t.second += 1;
+        ctx->evpool.push_back(ev.a);
+        ctx->evpool.push_back(ev.b);
+    }
+    ctx->pending.clear();
+    return 0;
+}
+
+// Launches a kernel on the context stream, bracketed by events when profiling.
+#define LAUNCH(ctx, kname, grid, block, ...)                                               \
+    do {                                                                                   \
+        sdbfx_ctx::Ev ev_{#kname, nullptr, nullptr};                                       \
+        if ((ctx)->prof) {                                                                 \
+            ev_.a = get_event(ctx);                                                        \
+            ev_.b = get_event(ctx);                                                        \
+            HIPC(ctx, hipEventRecord(ev_.a, (ctx)->stream));                               \
+        }                                                                                  \
+        hipLaunchKernelGGL(kname, dim3(grid), dim3(block), 0, (ctx)->stream, __VA_ARGS__); \
+        HIPC(ctx, hipGetLastError());                                                      \
+        if ((ctx)->prof) {                                                                 \
+            HIPC(ctx, hipEventRecord(ev_.b, (ctx)->stream));                               \
+            (ctx)->pending.push_back(ev_);                                                 \
+        }                                                                                  \
+    } while (0)
+
+int set_reserve(sdbfx_ctx* ctx, sdbfx_set* s, uint64_t total)
+{
+    if (int rc = grow_keep(ctx, s->filt, total * SDBFX_BF + 256, s->nfilt * SDBFX_BF))
+        return rc;
+    if (int rc = grow_keep(ctx, s->ecf, total * 2 + 256, s->nfilt * 2))
+        return rc;
+    if (int rc = grow_keep(ctx, s->hamf, total * 2 + 256, s->nfilt * 2))
+        return rc;
+    return 0;
+}
+
+uint64_t block_count(uint64_t len, uint32_t dd)
+{
+    return len / dd + ((len % dd) >= SDBFX_MIN_FILE ? 1 : 0);
+}
+
+// One device pass over inputs [i0, i1) that are resident at d_base + off[i].
+int digest_pass(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t i0, uint32_t i1, const uint8_t* d_base,
+                const uint64_t* off, const uint64_t* lens, const char* const* names, const uint32_t* dd,
+                const int32_t* status)
+{
+    // ---- stream-mode work list
+    std::vector<TileDesc> tiles;
+    std::vector<UnitDesc> units;
+    std::vector<StreamFile> sfiles;
+    std::vector<uint32_t> sidx;   // input index of each stream file
+    uint64_t sfilt_total = 0;
+    for (uint32_t i = i0; i < i1; i++) {
+        if (status[i] != 0 || (dd && dd[i]))
+            continue;
+        StreamFile sf{};
+        sf.tile_begin = (uint32_t)tiles.size();
+        const uint64_t len = lens[i];
+        for (uint64_t u0 = 0; u0 < len; u0 += SDBFX_CHUNK) {
+            const uint32_t n = (uint32_t)std::min<uint64_t>(SDBFX_CHUNK, len - u0);
+            if (n <= 64)
+                continue;
+            const uint32_t tb = (uint32_t)tiles.size();
+            for (uint32_t t0 = 0; t0 < n - 64; t0 += SDBFX_TILE) {
+                TileDesc td{};
+                td.unit_off = off[i] + u0;
+                td.n = n;
+                td.t0 = t0;
+                td.T = std::min<uint32_t>(SDBFX_TILE, n - 64 - t0);
+                td.first = t0 == 0;
+                tiles.push_back(td);
+            }
+            if (tiles.size() - tb > 1)
+                units.push_back(UnitDesc{tb, (uint32_t)tiles.size()});
+        }
+        sf.tile_end = (uint32_t)tiles.size();
+        sf.filt_base = sfilt_total;
+        sf.filt_cap = (uint32_t)((len >> 11) + 2);
+        sfilt_total += sf.filt_cap;
+        sfiles.push_back(sf);
+        sidx.push_back(i);
+    }
+
+    std::vector<uint32_t> counts(3 * sfiles.size());
+    if (!sfiles.empty()) {
+        int rc;
+        if ((rc = upload(ctx, ctx->tiles, tiles)) || (rc = upload(ctx, ctx->units, units)) ||
+            (rc = upload(ctx, ctx->files, sfiles)))
+            return rc;
+        if ((rc = ensure(ctx, ctx->cand, tiles.size() * 5ull * SDBFX_CAND_CAP * 4 + 256)) ||
+            (rc = ensure(ctx, ctx->tstate, tiles.size() * sizeof(TileState) + 256)) ||
+            (rc = ensure(ctx, ctx->sfilt, sfilt_total * SDBFX_BF + 256)) ||
+            (rc = ensure(ctx, ctx->sham, sfilt_total * 2 + 256)) ||
+            (rc = ensure(ctx, ctx->counts, counts.size() * 4 + 256)))
+            return rc;
+        if (!tiles.empty())
+            LAUNCH(ctx, stream_tile_kernel, (uint32_t)tiles.size(), SDBFX_NT, d_base,
+                   (const TileDesc*)ctx->tiles.p, (uint32_t*)ctx->cand.p, (TileState*)ctx->tstate.p);
+        if (!units.empty())
+            LAUNCH(ctx, stream_fixup_kernel, (uint32_t)units.size(), SDBFX_NT, d_base,
+                   (const TileDesc*)ctx->tiles.p, (const UnitDesc*)ctx->units.p, (uint32_t*)ctx->cand.p,
+                   (TileState*)ctx->tstate.p);
+        LAUNCH(ctx, stream_insert_kernel, (uint32_t)sfiles.size(), 64, (const StreamFile*)ctx->files.p,
+               (const TileDesc*)ctx->tiles.p, (const TileState*)ctx->tstate.p,
+               (const uint32_t*)ctx->cand.p, (uint8_t*)ctx->sfilt.p, (uint16_t*)ctx->sham.p,
+               (uint32_t*)ctx->counts.p);
+        HIPC(ctx, hipMemcpyAsync(counts.data(), ctx->counts.p, counts.size() * 4, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+        HIPC(ctx, hipStreamSynchronize(ctx->stream));
+        for (size_t f = 0; f < sfiles.size(); f++)
+            if (counts[3 * f + 2])
+                return fail(ctx, "stream digest exceeded its filter capacity");
+    }
+
+    // ---- final filter ranges in input order
+    uint64_t total = 0;
+    std::vector<uint64_t> first(i1 - i0, 0), cnt(i1 - i0, 0);
+    {
+        size_t f = 0;
+        for (uint32_t i = i0; i < i1; i++) {
+            if (status[i] != 0)
+                continue;
+            uint64_t c;
+            if (dd && dd[i]) {
+                c = block_count(lens[i], dd[i]);
+            } else {
+                c = counts[3 * f];
+                f++;
+            }
+            first[i - i0] = set->nfilt + total;
+            cnt[i - i0] = c;
+            total += c;
+        }
+    }
+    if (int rc = set_reserve(ctx, set, set->nfilt + total))
+        return rc;
+
+    // ---- block mode straight into the set
+    std::vector<BlockDesc> blocks;
+    for (uint32_t i = i0; i < i1; i++) {
+        if (status[i] != 0 || !dd || !dd[i])
+            continue;
+        const uint64_t len = lens[i];
+        const uint32_t d = dd[i];
+        const uint64_t qt = len / d, rem = len % d;
+        for (uint64_t b = 0; b < qt; b++)
+            blocks.push_back(BlockDesc{off[i] + b * d, d, 0, first[i - i0] + b});
+        if (rem >= SDBFX_MIN_FILE)
+            blocks.push_back(BlockDesc{off[i] + qt * d, (uint32_t)rem, 1, first[i - i0] + qt});
+    }
+    if (!blocks.empty()) {
+        if (int rc = upload(ctx, ctx->blocks, blocks))
+            return rc;
+        LAUNCH(ctx, block_digest_kernel, (uint32_t)blocks.size(), SDBFX_NT, d_base,
+               (const BlockDesc*)ctx->blocks.p, (uint8_t*)set->filt.p, (uint16_t*)set->ecf.p,
+               (uint16_t*)set->hamf.p);
+    }
+
+    // ---- stream filters into the set
+    if (!sfiles.empty()) {
+        std::vector<uint64_t> gsrc(sfiles.size()), gdst(sfiles.size());
+        std::vector<uint32_t> gcnt(sfiles.size()), glast(sfiles.size());
+        for (size_t f = 0; f < sfiles.size(); f++) {
+            gsrc[f] = sfiles[f].filt_base;
+            gdst[f] = first[sidx[f] - i0];
+            gcnt[f] = counts[3 * f];
+            glast[f] = counts[3 * f + 1];
+        }
+        int rc;
+        if ((rc = upload(ctx, ctx->gsrc, gsrc)) || (rc = upload(ctx, ctx->gdst, gdst)) ||
+            (rc = upload(ctx, ctx->gcnt, gcnt)) || (rc = upload(ctx, ctx->glast, glast)))
+            return rc;
+        LAUNCH(ctx, gather_filters_kernel, (uint32_t)sfiles.size(), 64, (const uint8_t*)ctx->sfilt.p,
+               (const uint16_t*)ctx->sham.p, (const uint64_t*)ctx->gsrc.p, (const uint64_t*)ctx->gdst.p,
+               (const uint32_t*)ctx->gcnt.p, (const uint32_t*)ctx->glast.p, (uint8_t*)set->filt.p,
+               (uint16_t*)set->hamf.p, (uint16_t*)set->ecf.p);
+    }
+
+    // ---- metadata (block digests need their last element count on the host)
+    std::vector<uint16_t> ec_all(total);
+    if (total) {
+        HIPC(ctx, hipMemcpyAsync(ec_all.data(), (uint16_t*)set->ecf.p + set->nfilt, total * 2,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+        HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    size_t f = 0;
+    for (uint32_t i = i0; i < i1; i++) {
+        if (status[i] != 0)
+            continue;
+        const bool blk = dd && dd[i];
+        set->names.push_back(names && names[i] ? names[i] : "");
+        set->orig.push_back(lens[i]);
+        set->first.push_back(first[i - i0]);
+        set->bfc.push_back((uint32_t)cnt[i - i0]);
+        set->dd.push_back(blk ? dd[i] : 0);
+        set->maxe.push_back(blk ? SDBFX_MAX_ELEM_DD : SDBFX_MAX_ELEM);
+        if (blk) {
+            set->last.push_back(0);
+            set->last_ec.push_back(ec_all[first[i - i0] + cnt[i - i0] - 1 - set->nfilt]);
+        } else {
+            set->last.push_back(counts[3 * f + 1]);
+            set->last_ec.push_back(counts[3 * f + 1]);
+            f++;
+        }
+    }
+    set->nfilt += total;
+    return 0;
+}
+
+int digest_common(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t n, const uint8_t* d_base, const uint64_t* off,
+                  const uint64_t* lens, const char* const* names, const uint32_t* dd, int32_t* status,
+                  const uint8_t* const* host_bufs)
+{
+    if (!ctx || !set || (n && (!lens || !status)))
+        return fail(ctx, "null argument");
+    for (uint32_t i = 0; i < n; i++) {
+        if (dd && dd[i] > SDBFX_TILE + 64)
+            return fail(ctx, "dd block size above 16384");
+        if (dd && dd[i] && dd[i] < SDBFX_MIN_FILE)
+            return fail(ctx, "dd block size below 512");
+        status[i] = lens[i] < SDBFX_MIN_FILE ? -3 : 0;
+    }
+    uint32_t i0 = 0;
+    std::vector<uint64_t> hoff;
+    while (i0 < n) {
+        uint32_t i1 = i0;
+        uint64_t bytes = 0;
+        while (i1 < n && (i1 == i0 || bytes + lens[i1] <= kPassBytes)) {
+            bytes += (lens[i1] + SDBFX_PAD - 1) / SDBFX_PAD * SDBFX_PAD;
+            i1++;
+        }
+        const uint8_t* base = d_base;
+        const uint64_t* o = off;
+        if (host_bufs) {
+            // stage this pass's inputs into padded device memory
+            hoff.assign(n, 0);
+            uint64_t pos = SDBFX_PAD;
+            for (uint32_t i = i0; i < i1; i++) {
+                hoff[i] = pos;
+                pos += (lens[i] + SDBFX_PAD - 1) / SDBFX_PAD * SDBFX_PAD;
+            }
+            if (int rc = ensure(ctx, ctx->stage, pos + SDBFX_PAD))
+                return rc;
+            HIPC(ctx, hipMemsetAsync(ctx->stage.p, 0, SDBFX_PAD, ctx->stream));
+            for (uint32_t i = i0; i < i1; i++)
+                if (lens[i])
+                    HIPC(ctx, hipMemcpyAsync((uint8_t*)ctx->stage.p + hoff[i], host_bufs[i], lens[i],
+                                             hipMemcpyHostToDevice, ctx->stream));
+            base = (const uint8_t*)ctx->stage.p;
+            o = hoff.data();
+        }
+        if (int rc = digest_pass(ctx, set, i0, i1, base, o, lens, names, dd, status))
+            return rc;
+        i0 = i1;
+    }
+    return 0;
+}
+
+// elem count of the last filter (sdbf::get_elem_count(s, bf_count-1))
+struct PairRef {
+    uint32_t set;   // 0 = A, 1 = B
+    uint32_t idx;
+};
+
+int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool all, int32_t* out)
+{
+    const uint32_t na = (uint32_t)A->names.size();
+    const uint32_t nb = all ? na : (uint32_t)B->names.size();
+    const uint64_t npairs = all ? (uint64_t)na * (na ? na - 1 : 0) / 2 : (uint64_t)na * nb;
+    if (npairs == 0)
+        return 0;
+    if (npairs > 0xFFFFFFFFull)
+        return fail(ctx, "too many digest pairs in one call");
+    const sdbfx_set* S[2] = {A, all ? A : B};
+
+    // orientation of every pair, in output order
+    std::vector<PairRef> ref(npairs), tgt(npairs);
+    {
+        uint64_t k = 0;
+        for (uint32_t i = 0; i < na; i++) {
+            const uint32_t j0 = all ? i + 1 : 0;
+            for (uint32_t j = j0; j < nb; j++, k++) {
+                PairRef x{0, i}, y{all ? 0u : 1u, j};
+                const sdbfx_set* X = S[x.set];
+                const sdbfx_set* Y = S[y.set];
+                const uint32_t n1 = X->bfc[x.idx], n2 = Y->bfc[y.idx];
+                const bool swap = n1 > n2 ||
+                                  (n1 == n2 && (X->last_ec[x.idx] > Y->last_ec[y.idx] ||
+                                                strcmp(X->names[x.idx].c_str(), Y->names[y.idx].c_str()) > 0));
+                ref[k] = swap ? y : x;
+                tgt[k] = swap ? x : y;
+            }
+        }
+    }
+    // group pairs by target digest (counting sort on a dense key)
+    const uint32_t nkeys = na + (all ? 0 : nb);
+    auto key = [&](const PairRef& r) { return r.set ? na + r.idx : r.idx; };
+    std::vector<uint32_t> kc(nkeys + 1, 0);
+    for (uint64_t k = 0; k < npairs; k++)
+        kc[key(tgt[k]) + 1]++;
+    for (uint32_t q = 0; q < nkeys; q++)
+        kc[q + 1] += kc[q];
+    std::vector<uint32_t> order(npairs);
+    {
+        std::vector<uint32_t> pos(kc.begin(), kc.end() - 1);
+        for (uint64_t k = 0; k < npairs; k++)
+            order[pos[key(tgt[k])]++] = (uint32_t)k;
+    }
+    std::vector<CmpPair> pairs(npairs);
+    std::vector<CmpGroup> groups;
+    uint64_t inst = 0, wi = 0;
+    for (uint32_t q = 0; q < nkeys; q++) {
+        if (kc[q] == kc[q + 1])
+            continue;
+        const PairRef t{q >= na ? 1u : 0u, q >= na ? q - na : q};
+        const sdbfx_set* T = S[t.set];
+        CmpGroup g{};
+        g.tgt_first = T->first[t.idx];
+        g.tgt_count = T->bfc[t.idx];
+        g.nch = (g.tgt_count + SDBFX_CMP_CHUNK - 1) / SDBFX_CMP_CHUNK;
+        g.inst_begin = inst;
+        g.pair_begin = kc[q];
+        g.pair_end = kc[q + 1];
+        g.tgt_set = t.set;
+        for (uint32_t s = kc[q]; s < kc[q + 1]; s++) {
+            const PairRef r = ref[order[s]];
+            const sdbfx_set* R = S[r.set];
+            CmpPair& p = pairs[s];
+            p.ref_first = R->first[r.idx];
+            p.ref_count = R->bfc[r.idx];
+            p.nch = g.nch;
+            p.inst_begin = inst;
+            p.ref_set = r.set;
+            p.pad = 0;
+            inst += p.ref_count;
+        }
+        const uint64_t ic = inst - g.inst_begin;
+        if (ic > 0xFFFFFFFFull)
+            return fail(ctx, "too many filter instances for one target digest");
+        g.inst_count = (uint32_t)ic;
+        g.nblk = (g.inst_count + 63) / 64;
+        g.wi_begin = wi;
+        wi += (uint64_t)g.nblk * g.nch;
+        groups.push_back(g);
+    }
+    uint64_t partial_n = 0;
+    for (const auto& g : groups)
+        partial_n += (uint64_t)g.inst_count * g.nch;
+
+    int rc;
+    if ((rc = upload(ctx, ctx->groups, groups)) || (rc = upload(ctx, ctx->pairs, pairs)) ||
+        (rc = ensure(ctx, ctx->partial, partial_n * 4 + 256)) || (rc = ensure(ctx, ctx->scores, npairs * 4 + 256)))
+        return rc;
+    SetView va{(const uint32_t*)S[0]->filt.p, (const uint16_t*)S[0]->ecf.p, (const uint16_t*)S[0]->hamf.p};
+    SetView vb{(const uint32_t*)S[1]->filt.p, (const uint16_t*)S[1]->ecf.p, (const uint16_t*)S[1]->hamf.p};
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(wi, 16384);
+    if (wi)
+        LAUNCH(ctx, cmp_kernel, grid, 64, va, vb, (const CmpGroup*)ctx->groups.p, (uint32_t)groups.size(),
+               (const CmpPair*)ctx->pairs.p, wi, (uint32_t*)ctx->partial.p);
+    LAUNCH(ctx, pair_sum_kernel, (uint32_t)((npairs + 255) / 256), 256, (const CmpPair*)ctx->pairs.p,
+           (uint32_t)npairs, va.ecf, vb.ecf, (const uint32_t*)ctx->partial.p, (int32_t*)ctx->scores.p);
+    std::vector<int32_t> sorted(npairs);
+    HIPC(ctx, hipMemcpyAsync(sorted.data(), ctx->scores.p, npairs * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    for (uint64_t s = 0; s < npairs; s++)
+        out[order[s]] = sorted[s];
+    return 0;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+
+extern "C" {
+
+int sdbfx_ctx_create(int device, uint64_t mem_budget, sdbfx_ctx** out)
+{
+    if (!out)
+        return -1;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return -1;
+    if (hipSetDevice(device) != hipSuccess)
+        return -1;
+    sdbfx_ctx* c = new sdbfx_ctx();
+    c->device = device;
+    c->budget = mem_budget;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return -1;
+    }
+    *out = c;
+    return 0;
+}
+
+void sdbfx_ctx_destroy(sdbfx_ctx* ctx)
+{
+    if (!ctx)
+        return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    DevBuf* bufs[] = {&ctx->stage, &ctx->blocks, &ctx->tiles, &ctx->tstate, &ctx->units, &ctx->files,
+                      &ctx->cand, &ctx->sfilt, &ctx->sham, &ctx->counts, &ctx->gsrc, &ctx->gdst,
+                      &ctx->gcnt, &ctx->glast, &ctx->groups, &ctx->pairs, &ctx->partial, &ctx->scores};
+    for (DevBuf* b : bufs)
+        if (b->p)
+            (void)hipFree(b->p);
+    for (auto& ev : ctx->pending) {
+        (void)hipEventDestroy(ev.a);
+        (void)hipEventDestroy(ev.b);
+    }
+    for (hipEvent_t e : ctx->evpool)
+        (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* sdbfx_last_error(const sdbfx_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int sdbfx_sync(sdbfx_ctx* ctx)
+{
+    if (!ctx)
+        return -1;
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int sdbfx_set_create(sdbfx_ctx* ctx, sdbfx_set** out)
+{
+    if (!ctx || !out)
+        return -1;
+    sdbfx_set* s = new sdbfx_set();
+    s->device = ctx->device;
+    *out = s;
+    return 0;
+}
+
+void sdbfx_set_destroy(sdbfx_set* set)
+{
+    if (!set)
+        return;
+    (void)hipSetDevice(set->device);
+    (void)hipDeviceSynchronize();
+    if (set->filt.p)
+        (void)hipFree(set->filt.p);
+    if (set->ecf.p)
+        (void)hipFree(set->ecf.p);
+    if (set->hamf.p)
+        (void)hipFree(set->hamf.p);
+    delete set;
+}
+
+uint32_t sdbfx_set_size(const sdbfx_set* set) { return set ? (uint32_t)set->names.size() : 0; }
+
+uint64_t sdbfx_set_filter_count(const sdbfx_set* set) { return set ? set->nfilt : 0; }
+
+int sdbfx_set_digest(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t n, const uint8_t* const* bufs,
+                     const uint64_t* lens, const char* const* names, const uint32_t* dd_block,
+                     int32_t* status)
+{
+    if (!ctx)
+        return -1;
+    if (n && !bufs)
+        return fail(ctx, "null buffers");
+    HIPC(ctx, hipSetDevice(ctx->device));
+    return digest_common(ctx, set, n, nullptr, nullptr, lens, names, dd_block, status, bufs);
+}
+
+int sdbfx_set_digest_device(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t n, const void* d_base,
+                            const uint64_t* offsets, const uint64_t* lens, const char* const* names,
+                            const uint32_t* dd_block, int32_t* status)
+{
+    if (!ctx)
+        return -1;
+    if (n && (!d_base || !offsets))
+        return fail(ctx, "null device input");
+    for (uint32_t i = 0; i < n; i++)
+        if (offsets[i] % SDBFX_PAD)
+            return fail(ctx, "device input offsets must be multiples of 256");
+    HIPC(ctx, hipSetDevice(ctx->device));
+    return digest_common(ctx, set, n, (const uint8_t*)d_base, offsets, lens, names, dd_block, status,
+                         nullptr);
+}
+
+int sdbfx_set_add(sdbfx_ctx* ctx, sdbfx_set* set, const char* name, uint64_t orig_size, uint32_t bf_count,
+                  uint32_t last_count, uint32_t dd_block_size, uint32_t max_elem, const uint8_t* filters,
+                  const uint16_t* elem_counts)
+{
+    if (!ctx || !set || !filters)
+        return fail(ctx, "null argument");
+    if (bf_count == 0)
+        return fail(ctx, "digest without filters");
+    if (dd_block_size && !elem_counts)
+        return fail(ctx, "block-mode digest needs element counts");
+    HIPC(ctx, hipSetDevice(ctx->device));
+    if (int rc = set_reserve(ctx, set, set->nfilt + bf_count))
+        return rc;
+    std::vector<uint16_t> ec(bf_count), ham(bf_count);
+    for (uint32_t i = 0; i < bf_count; i++) {
+        if (dd_block_size)
+            ec[i] = elem_counts[i];
+        else
+            ec[i] = (uint16_t)(i + 1 < bf_count ? max_elem : last_count);
+        uint32_t h = 0;
+        for (int b = 0; b < SDBFX_BF; b++)
+            h += (uint32_t)__builtin_popcount(filters[(uint64_t)i * SDBFX_BF + b]);
+        ham[i] = (uint16_t)h;
+    }
+    HIPC(ctx, hipMemcpyAsync((uint8_t*)set->filt.p + set->nfilt * SDBFX_BF, filters,
+                             (uint64_t)bf_count * SDBFX_BF, hipMemcpyHostToDevice, ctx->stream));
+    HIPC(ctx, hipMemcpyAsync((uint16_t*)set->ecf.p + set->nfilt, ec.data(), bf_count * 2,
+                             hipMemcpyHostToDevice, ctx->stream));
+    HIPC(ctx, hipMemcpyAsync((uint16_t*)set->hamf.p + set->nfilt, ham.data(), bf_count * 2,
+                             hipMemcpyHostToDevice, ctx->stream));
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    set->names.push_back(name ? name : "");
+    set->orig.push_back(orig_size);
+    set->first.push_back(set->nfilt);
+    set->bfc.push_back(bf_count);
+    set->last.push_back(dd_block_size ? 0 : last_count);
+    set->dd.push_back(dd_block_size);
+    set->maxe.push_back(max_elem);
+    set->last_ec.push_back(ec[bf_count - 1]);
+    set->nfilt += bf_count;
+    return 0;
+}
+
+int sdbfx_set_info(const sdbfx_set* set, uint32_t idx, sdbfx_digest_info* info)
+{
+    if (!set || !info || idx >= set->names.size())
+        return -1;
+    info->name = set->names[idx].c_str();
+    info->orig_size = set->orig[idx];
+    info->bf_count = set->bfc[idx];
+    info->last_count = set->last[idx];
+    info->dd_block_size = set->dd[idx];
+    info->max_elem = set->maxe[idx];
+    return 0;
+}
+
+int sdbfx_set_read(sdbfx_ctx* ctx, const sdbfx_set* set, uint32_t idx, uint8_t* filters,
+                   uint16_t* elem_counts, uint16_t* hamming)
+{
+    if (!ctx || !set || idx >= set->names.size())
+        return fail(ctx, "bad digest index");
+    HIPC(ctx, hipSetDevice(ctx->device));
+    const uint64_t f = set->first[idx], c = set->bfc[idx];
+    if (filters)
+        HIPC(ctx, hipMemcpyAsync(filters, (const uint8_t*)set->filt.p + f * SDBFX_BF, c * SDBFX_BF,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+    if (elem_counts)
+        HIPC(ctx, hipMemcpyAsync(elem_counts, (const uint16_t*)set->ecf.p + f, c * 2, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    if (hamming)
+        HIPC(ctx, hipMemcpyAsync(hamming, (const uint16_t*)set->hamf.p + f, c * 2, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int sdbfx_compare_all(sdbfx_ctx* ctx, const sdbfx_set* set, int32_t* scores)
+{
+    if (!ctx || !set || !scores)
+        return fail(ctx, "null argument");
+    HIPC(ctx, hipSetDevice(ctx->device));
+    return compare_impl(ctx, set, set, true, scores);
+}
+
+int sdbfx_compare_to(sdbfx_ctx* ctx, const sdbfx_set* q, const sdbfx_set* t, int32_t* scores)
+{
+    if (!ctx || !q || !t || !scores)
+        return fail(ctx, "null argument");
+    HIPC(ctx, hipSetDevice(ctx->device));
+    return compare_impl(ctx, q, t, false, scores);
+}
+
+int sdbfx_device_alloc(sdbfx_ctx* ctx, uint64_t bytes, void** dptr)
+{
+    if (!ctx || !dptr)
+        return -1;
+    HIPC(ctx, hipSetDevice(ctx->device));
+    HIPC(ctx, hipMalloc(dptr, bytes + SDBFX_PAD));
+    HIPC(ctx, hipMemset((uint8_t*)*dptr + bytes, 0, SDBFX_PAD));
+    return 0;
+}
+
+int sdbfx_device_free(sdbfx_ctx* ctx, void* dptr)
+{
+    if (!ctx)
+        return -1;
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    HIPC(ctx, hipFree(dptr));
+    return 0;
+}
+
+int sdbfx_upload(sdbfx_ctx* ctx, void* dst, const void* src, uint64_t bytes)
+{
+    if (!ctx)
+        return -1;
+    HIPC(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int sdbfx_set_profiling(sdbfx_ctx* ctx, int on)
+{
+    if (!ctx)
+        return -1;
+    ctx->prof = on != 0;
+    return 0;
+}
+
+int sdbfx_kernel_time(sdbfx_ctx* ctx, const char* name, double* ms, uint64_t* launches)
+{
+    if (!ctx || !name)
+        return -1;
+    if (int rc = collect_times(ctx))
+        return rc;
+    auto it = ctx->times.find(name);
+    if (ms)
+        *ms = it == ctx->times.end() ? 0.0 : it->second.first;
+    if (launches)
+        *launches = it == ctx->times.end() ? 0 : it->second.second;
+    return 0;
+}
+
+int sdbfx_reset_kernel_times(sdbfx_ctx* ctx)
+{
+    if (!ctx)
+        return -1;
+    if (int rc = collect_times(ctx))
+        return rc;
+    ctx->times.clear();
+    return 0;
+}
+
+}  // extern "C"

@@ -76,6 +76,8 @@ struct CmpGroup {
     uint32_t pair_begin;   // pairs of the group (sorted by instance offset)
     uint32_t pair_end;
     uint64_t wi_begin;     // first work item of the group
+    uint32_t tgt_set;      // 0: target filters in set A, 1: in set B
+    uint32_t pad;
 };
 
 // Compare: one (reference digest, target digest) pair after orientation.
@@ -84,6 +86,8 @@ struct CmpPair {
     uint32_t ref_count;    // its filters
     uint32_t nch;          // chunks of the target digest
     uint64_t inst_begin;   // first instance index
+    uint32_t ref_set;      // 0: reference filters in set A, 1: in set B
+    uint32_t pad;
 };
 
 #endif

@@ -1,0 +1,937 @@
+// gfx950 kernels of the sdbf digest and comparison path.
+//
+// Digest (one 256-thread workgroup per 16 KiB dd-block, or per 16 Ki-position
+// tile of a stream-mode unit):
+//   1. ranks   : rank of every 64-byte window = ENTR64_RANKS[entropy >> 10],
+//                entropy = sum over byte values of E[count]; each of 128
+//                lanes rolls a private byte histogram over its run of windows
+//                (sdbf/sdbf_core.cc:22-45, sdbf/entr64.cc:28-71).
+//   2. F       : for every window start i, the position the reference's full
+//                rescan (sdbf_core.cc:69-78) settles on: first occurrence of the
+//                smallest non-zero rank in [i, i+63], moved to the end of the
+//                run of equal ranks that starts there.  Computed with wave-wide
+//                suffix/prefix-min scans over 64-aligned blocks and run-start
+//                ballots.  Packed with the rank as rank(10 bits) | F-i (6 bits).
+//   3. scores  : the popularity state machine of sdbf_core.cc:51-89.  Its state
+//                before step i is the remembered minimum position m (the rank
+//                at m is implied), so 256 lanes each run one segment of steps
+//                after a speculative 128-step warm-up.  Every lane's start
+//                state is checked against its predecessor's end state and
+//                mismatching segments are undone and re-run until all agree
+//                (fix-point).  The first segment of a stream tile that does not
+//                start at position 0 is certified by running all 64 possible
+//                incoming states through the warm-up: if they coalesce, the
+//                state is exact whatever came before.
+//   4. block mode: score histogram, threshold/allowance (sdbf_core.cc:271-276),
+//                candidate SHA-1 in parallel, then the ordered Bloom insert of
+//                gen_block_hash (sdbf_core.cc:142-191) on one wave with the
+//                filter spread one 32-bit word per lane.
+//      stream mode: candidates (score > 16) are SHA-1 hashed in parallel and
+//                written per tile; stream_insert_kernel then replays
+//                gen_chunk_hash (sdbf_core.cc:94-136) in order, one wave per
+//                input, with the 16 KiB de-duplication filter in LDS.
+//
+// Compare: one wave per (64 reference-filter instances, target chunk); each
+// lane holds its reference filter in 64 VGPRs and the target filters stream
+// through SGPRs (uniform scalar loads).  AND + v_bcnt with the early-exit
+// checkpoints of bf_bitcount_cut_256 (sdbf/bf_utils.cc:115-206) and the score
+// of sdbf_max_score (sdbf_core.cc:396-455), keeping the maximum as an exact
+// fraction.  pair_sum_kernel reproduces sdbf_score's ordered double sum
+// (sdbf_core.cc:331-389).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sdbf_tables.h"
+#include "sdbfx_internal.h"
+
+#define RANK_LANES 128
+#define LMAX (SDBFX_TILE + 64 + SDBFX_WARM + 64)
+#define BCAND_CAP 1152   // block mode: positions with score >= 15 in <= 16320 positions
+#define NWAVE (SDBFX_NT / 64)
+
+__constant__ uint16_t c_ranks[SDBFX_RANKS_LEN] = {SDBFX_ENTR64_RANKS_INIT};
+__constant__ uint16_t c_cut[SDBFX_CUTOFFS_LEN] = {SDBFX_CUTOFFS256_INIT};
+__constant__ uint32_t c_entropy[65] = {SDBFX_ENTROPY64_INIT};
+
+struct __align__(16) TileSmem {
+    uint16_t P[LMAX];                       // rank | (F - i) << 10, positions [base, base+L)
+    union {
+        uint8_t hist[RANK_LANES * 256];     // ranks phase: one byte histogram per lane
+        struct {
+            uint32_t score[SDBFX_TILE / 4]; // owned positions' scores, 4 per word
+            uint16_t cand[BCAND_CAP];       // candidate positions (relative to t0)
+            uint32_t hsha[5][SDBFX_NT];     // block mode: SHA-1 words of a batch
+            uint32_t csc[SDBFX_NT];         // block mode: scores of a batch
+        } s;
+    } u;
+    uint16_t rtab[1008];
+    int32_t dtab[72];
+    uint32_t start_m[SDBFX_NT];
+    uint32_t end_m[SDBFX_NT];
+    uint32_t histo[68];
+    uint32_t wsum[NWAVE + 4];
+    uint32_t misc[16];
+};
+
+// ------------------------------------------------------------------ SHA-1
+
+__device__ __forceinline__ uint32_t rol32(uint32_t x, int n) { return (x << n) | (x >> (32 - n)); }
+
+// FIPS 180-4 SHA-1 compression of one 512-bit block.
+__device__ __forceinline__ void sha1_compress(uint32_t H[5], uint32_t w[16])
+{
+    uint32_t a = H[0], b = H[1], c = H[2], d = H[3], e = H[4];
+#pragma unroll
+    for (int t = 0; t < 80; t++) {
+        uint32_t wt;
+        if (t < 16) {
+            wt = w[t];
+        } else {
+            wt = rol32(w[(t - 3) & 15] ^ w[(t - 8) & 15] ^ w[(t - 14) & 15] ^ w[t & 15], 1);
+            w[t & 15] = wt;
+        }
+        uint32_t f, k;
+        if (t < 20) {
+            f = (b & c) | (~b & d);
+            k = 0x5A827999u;
+        } else if (t < 40) {
+            f = b ^ c ^ d;
+            k = 0x6ED9EBA1u;
+        } else if (t < 60) {
+            f = (b & c) | (b & d) | (c & d);
+            k = 0x8F1BBCDCu;
+        } else {
+            f = b ^ c ^ d;
+            k = 0xCA62C1D6u;
+        }
+        uint32_t tmp = rol32(a, 5) + f + e + k + wt;
+        e = d;
+        d = c;
+        c = rol32(b, 30);
+        b = a;
+        a = tmp;
+    }
+    H[0] += a;
+    H[1] += b;
+    H[2] += c;
+    H[3] += d;
+    H[4] += e;
+}
+
+// SHA-1 of the 64 bytes at p (any alignment; the 4 bytes after p+63 must be
+// readable).  out[k] is digest word k read as a little-endian uint32, which is
+// how the reference consumes the digest (sdbf_core.cc:105-106).
+__device__ __forceinline__ void sha1_64(const uint8_t* p, uint32_t out[5])
+{
+    const uintptr_t ad = (uintptr_t)p;
+    const uint32_t* q = (const uint32_t*)(ad & ~(uintptr_t)3);
+    const uint32_t sh = (uint32_t)(ad & 3);
+    uint32_t d[17];
+#pragma unroll
+    for (int k = 0; k < 16; k++)
+        d[k] = q[k];
+    d[16] = sh ? q[16] : 0u;
+    uint32_t w[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++)
+        w[k] = __builtin_bswap32(__builtin_amdgcn_alignbyte(d[k + 1], d[k], sh));
+    uint32_t H[5] = {0x67452301u, 0xEFCDAB89u, 0x98BADCFEu, 0x10325476u, 0xC3D2E1F0u};
+    sha1_compress(H, w);
+    uint32_t pad[16] = {0x80000000u, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 512u};
+    sha1_compress(H, pad);
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+        out[k] = __builtin_bswap32(H[k]);
+}
+
+// ------------------------------------------------------------------ helpers
+
+__device__ __forceinline__ uint32_t rank_at(const uint16_t* P, uint32_t base, uint32_t pos)
+{
+    return P[pos - base] & 0x3FFu;
+}
+
+// Canonical state before step i: the remembered minimum m matters only while
+// the cheap slide is possible (m > i and rank(m) > 0); every other state makes
+// step i a full rescan and is written 0.
+__device__ __forceinline__ uint32_t canon(const uint16_t* P, uint32_t base, uint32_t i, uint32_t m)
+{
+    return (m > i && rank_at(P, base, m) > 0) ? m : 0u;
+}
+
+// smallest t > q with bit t set in the 128-bit value hi:lo; 128 if none
+__device__ __forceinline__ uint32_t first_after(uint64_t lo, uint64_t hi, uint32_t q)
+{
+    uint32_t s = q + 1;
+    if (s < 64) {
+        uint64_t m = lo >> s;
+        if (m)
+            return s + (uint32_t)__builtin_ctzll(m);
+        if (hi)
+            return 64 + (uint32_t)__builtin_ctzll(hi);
+        return 128;
+    }
+    s -= 64;
+    if (s >= 64)
+        return 128;
+    uint64_t m = hi >> s;
+    return m ? 64 + s + (uint32_t)__builtin_ctzll(m) : 128u;
+}
+
+__device__ __forceinline__ uint32_t score_at(const TileSmem& sm, uint32_t rel)
+{
+    return ((const uint8_t*)sm.u.s.score)[rel];
+}
+
+// Runs the score state machine over steps [i0, i1) from state m.  With CREDIT
+// each step adds SIGN to the score of the position it elects, when that
+// position is owned ([t0, t0+T)).  When step hand_i is run, the canonical state
+// before step hand_i+1 is stored to *hand.  Returns the state after i1-1.
+template <int SIGN, bool CREDIT>
+__device__ uint32_t sm_run(TileSmem& sm, uint32_t base, uint32_t t0, uint32_t T, uint32_t i0,
+                           uint32_t i1, uint32_t m, uint32_t hand_i, uint32_t* hand)
+{
+    const uint16_t* P = sm.P;
+    uint32_t rm = m ? rank_at(P, base, m) : 0u;
+    for (uint32_t i = i0; i < i1; i++) {
+        const uint32_t r64 = P[i + 64 - base] & 0x3FFu;
+        uint32_t cpos;
+        bool has;
+        if (i > 0 && rm > 0 && r64 >= rm && i < m) {
+            // slide on the cheap (sdbf_core.cc:61-68)
+            if (r64 == rm)
+                m = i + 64;
+            cpos = m;
+            has = true;
+        } else {
+            // full rescan of the window (sdbf_core.cc:69-81)
+            const uint32_t w = P[i - base];
+            const uint32_t f = i + (w >> 10);
+            const uint32_t rf = P[f - base] & 0x3FFu;
+            m = f;
+            rm = rf;
+            cpos = f;
+            has = rf != 0;
+        }
+        if (CREDIT && has) {
+            const uint32_t rel = cpos - t0;
+            if (cpos >= t0 && rel < T)
+                atomicAdd(&sm.u.s.score[rel >> 2], ((uint32_t)SIGN) << (8 * (rel & 3)));
+        }
+        if (hand && i == hand_i)
+            *hand = canon(P, base, i + 1, m);
+    }
+    return m;
+}
+
+// ------------------------------------------------------------------ tile core
+
+// Computes ranks, F and the scores of owned positions [t0, t0+T) of a unit of
+// n bytes at u (n > 64, t0 + T <= n - 64, t0 == 0 or t0 >= 16384).  forced >= 0
+// gives the exact canonical state before step t0-64.  Returns 1 when every
+// score is exact (certified), 0 when the incoming state could not be proven.
+__device__ uint32_t tile_core(TileSmem& sm, const uint8_t* __restrict__ u, uint32_t n, uint32_t t0,
+                              uint32_t T, int64_t forced, uint32_t hand_i, uint32_t* hand_out)
+{
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const uint32_t a = t0 >= 64 ? t0 - 64 : 0;
+    const uint32_t t_end = t0 + T;
+    const uint32_t base = a >= SDBFX_WARM ? a - SDBFX_WARM : 0;
+    const uint32_t L = t_end + 64 - base;   // positions held
+    const uint32_t NI = t_end - base;       // steps whose F is needed
+    const uint32_t lim = n - 64;            // positions with a full window
+
+    for (int q = tid; q < SDBFX_RANKS_LEN; q += SDBFX_NT)
+        sm.rtab[q] = c_ranks[q];
+    if (tid < 65)
+        sm.dtab[tid] = tid ? (int32_t)c_entropy[tid] - (int32_t)c_entropy[tid - 1] : 0;
+    __syncthreads();
+
+    // ---- 1. ranks
+    if (tid < RANK_LANES) {
+        const uint32_t PR = (L + RANK_LANES - 1) / RANK_LANES;
+        const uint32_t r0 = tid * PR;
+        const uint32_t r1 = min(r0 + PR, L);
+        uint8_t* H = sm.u.hist + tid * 256;
+        const uint32_t swz = (uint32_t)(tid & 31) << 2;
+        uint32_t* H32 = (uint32_t*)H;
+        for (int q = 0; q < 64; q++)
+            H32[q] = 0;
+        int32_t e = 0;
+        bool init = false;
+        for (uint32_t rp = r0; rp < r1; rp++) {
+            const uint32_t p = base + rp;
+            uint16_t rank = 0;
+            if (p < lim) {
+                if (!init) {
+                    for (int k = 0; k < 64; k++) {
+                        const uint32_t b = u[p + k];
+                        const uint32_t c = ++H[b ^ swz];
+                        e += sm.dtab[c];
+                    }
+                    init = true;
+                } else {
+                    const uint32_t bo = u[p - 1];
+                    const uint32_t bi = u[p + 63];
+                    if (bo != bi) {
+                        const uint32_t co = H[bo ^ swz];
+                        const uint32_t ci = H[bi ^ swz];
+                        H[bo ^ swz] = (uint8_t)(co - 1);
+                        H[bi ^ swz] = (uint8_t)(ci + 1);
+                        e += sm.dtab[ci + 1] - sm.dtab[co];
+                    }
+                }
+                rank = sm.rtab[e >> 10];
+            }
+            sm.P[rp] = rank;
+        }
+    }
+    __syncthreads();
+
+    // ---- 2. F (and clear the score words, which alias the histograms)
+    for (int q = tid; q < SDBFX_TILE / 4; q += SDBFX_NT)
+        sm.u.s.score[q] = 0;
+    for (uint32_t blk = wave; blk * 64 < NI; blk += NWAVE) {
+        const uint32_t pa = blk * 64 + lane;
+        const uint32_t pb = pa + 64;
+        const uint32_t ra = pa < L ? (sm.P[pa] & 0x3FFu) : 0u;
+        const uint32_t rb = pb < L ? (sm.P[pb] & 0x3FFu) : 0u;
+        const uint32_t rpa = (pa >= 1 && pa - 1 < L) ? (sm.P[pa - 1] & 0x3FFu) : 0xFFFFu;
+        const uint32_t rpb = (pb - 1 < L) ? (sm.P[pb - 1] & 0x3FFu) : 0xFFFFu;
+        const uint32_t ka = ra ? ((ra << 16) | pa) : 0xFFFFFFFFu;
+        const uint32_t kb = rb ? ((rb << 16) | pb) : 0xFFFFFFFFu;
+        uint32_t sa = ka;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t o = __shfl_down(sa, off);
+            if (lane + off < 64)
+                sa = min(sa, o);
+        }
+        uint32_t pm = kb;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t o = __shfl_up(pm, off);
+            if (lane >= off)
+                pm = min(pm, o);
+        }
+        uint32_t pm1 = __shfl_up(pm, 1);
+        if (lane == 0)
+            pm1 = 0xFFFFFFFFu;
+        const uint32_t key = min(sa, pm1);
+        const uint64_t da = __ballot(pa < L && ra != rpa);
+        const uint64_t db = __ballot(pb < L && rb != rpb);
+        if (pa < NI) {
+            uint32_t fr = 0;
+            if (ra) {
+                const uint32_t q = (key & 0xFFFFu) - blk * 64;   // in [lane, lane+63]
+                const uint32_t t = first_after(da, db, q);
+                const uint32_t end = min(t - 1, (uint32_t)lane + 63);
+                fr = end - lane;
+            }
+            sm.P[pa] = (uint16_t)((sm.P[pa] & 0x3FFu) | (fr << 10));
+        }
+    }
+    __syncthreads();
+
+    // ---- 3. scores
+    const uint32_t NIter = t_end - a;
+    const uint32_t S = (NIter + SDBFX_NT - 1) / SDBFX_NT;
+    const uint32_t s = a + tid * S;
+    const uint32_t e = min(s + S, t_end);
+    const bool active = s < t_end;
+
+    if (wave == 0) {
+        uint32_t st = 0, cert = 1;
+        if (forced >= 0) {
+            st = (uint32_t)forced;
+        } else if (a > 0) {
+            // all 64 states possible before step `base` (a full rescan, or a
+            // remembered minimum at base+1..base+63) run through the warm-up
+            uint32_t m = lane ? base + lane : 0u;
+            if (m && rank_at(sm.P, base, m) == 0)
+                m = 0;
+            m = sm_run<0, false>(sm, base, t0, T, base, a, m, 0xFFFFFFFFu, nullptr);
+            const uint32_t c = canon(sm.P, base, a, m);
+            const uint32_t c0 = __shfl(c, 0);
+            cert = __ballot(c != c0) == 0 ? 1u : 0u;
+            st = c0;
+        }
+        if (lane == 0) {
+            sm.misc[0] = st;
+            sm.misc[1] = cert;
+        }
+    }
+    __syncthreads();
+
+    uint32_t mstart = 0;
+    if (tid == 0) {
+        mstart = sm.misc[0];
+    } else if (active) {
+        const uint32_t w = (s > base + SDBFX_WARM) ? s - SDBFX_WARM : base;
+        const uint32_t m = sm_run<0, false>(sm, base, t0, T, w, s, 0u, 0xFFFFFFFFu, nullptr);
+        mstart = canon(sm.P, base, s, m);
+    }
+    if (active) {
+        const uint32_t m = sm_run<1, true>(sm, base, t0, T, s, e, mstart, hand_i, hand_out);
+        sm.start_m[tid] = mstart;
+        sm.end_m[tid] = canon(sm.P, base, e, m);
+    }
+    __syncthreads();
+    for (;;) {
+        bool mis = false;
+        uint32_t pred = 0;
+        if (tid > 0 && active) {
+            pred = sm.end_m[tid - 1];
+            mis = pred != sm.start_m[tid];
+        }
+        __syncthreads();
+        if (mis) {
+            sm_run<-1, true>(sm, base, t0, T, s, e, sm.start_m[tid], 0xFFFFFFFFu, nullptr);
+            const uint32_t m = sm_run<1, true>(sm, base, t0, T, s, e, pred, hand_i, hand_out);
+            sm.start_m[tid] = pred;
+            sm.end_m[tid] = canon(sm.P, base, e, m);
+        }
+        if (!__syncthreads_or(mis))
+            break;
+    }
+    return sm.misc[1];
+}
+
+// exclusive block-wide prefix sum of v over the 256 threads; *total gets the sum
+__device__ __forceinline__ uint32_t block_scan(TileSmem& sm, uint32_t v, uint32_t* total)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = __shfl_up(x, off);
+        if (lane >= off)
+            x += o;
+    }
+    if (lane == 63)
+        sm.wsum[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+    for (int w = 0; w < NWAVE; w++) {
+        const uint32_t ws = sm.wsum[w];
+        if (w < wave)
+            before += ws;
+        tot += ws;
+    }
+    __syncthreads();
+    *total = tot;
+    return before + x - v;
+}
+
+// Ordered compaction of owned positions [0, cnt) satisfying pred(score) into
+// sm.u.s.cand.  Returns the number of candidates.
+template <typename Pred>
+__device__ uint32_t compact(TileSmem& sm, uint32_t cnt, Pred pred)
+{
+    const int tid = threadIdx.x;
+    const uint32_t CH = (cnt + SDBFX_NT - 1) / SDBFX_NT;
+    const uint32_t q0 = tid * CH, q1 = min(q0 + CH, cnt);
+    uint32_t c = 0;
+    for (uint32_t q = q0; q < q1; q++)
+        c += pred(score_at(sm, q)) ? 1u : 0u;
+    uint32_t total;
+    uint32_t o = block_scan(sm, c, &total);
+    for (uint32_t q = q0; q < q1; q++)
+        if (pred(score_at(sm, q)))
+            sm.u.s.cand[o++] = (uint16_t)q;
+    __syncthreads();
+    return total;
+}
+
+// ------------------------------------------------------------------ block mode
+
+// One workgroup per dd-block (sdbf_core.cc:252-282 per block, :305-319 tail).
+__global__ __launch_bounds__(SDBFX_NT) void block_digest_kernel(
+    const uint8_t* __restrict__ data, const BlockDesc* __restrict__ blocks, uint8_t* __restrict__ filters,
+    uint16_t* __restrict__ elem_counts, uint16_t* __restrict__ hamming)
+{
+    __shared__ TileSmem sm;
+    const BlockDesc bd = blocks[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint8_t* u = data + bd.off;
+    const uint32_t n = bd.n;
+    const uint32_t T = n - 64;
+
+    tile_core(sm, u, n, 0, T, -1, 0xFFFFFFFFu, nullptr);
+
+    // histogram of scores >= 16 over positions [0, n-64)
+    if (tid < 68)
+        sm.histo[tid] = 0;
+    __syncthreads();
+    for (uint32_t q = tid; q < T; q += SDBFX_NT) {
+        const uint32_t sc = score_at(sm, q);
+        if (sc >= SDBFX_THRESHOLD)
+            atomicAdd(&sm.histo[sc], 1u);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t k = SDBFX_THRESHOLD, sum = 0;
+        int32_t allowed = SDBFX_MAX_ELEM_DD;
+        if (!bd.tail) {
+            for (k = 65, sum = 0; k >= SDBFX_THRESHOLD; k--) {
+                if (sum <= SDBFX_MAX_ELEM_DD && sum + sm.histo[k] > SDBFX_MAX_ELEM_DD)
+                    break;
+                sum += sm.histo[k];
+            }
+            allowed = (int32_t)(SDBFX_MAX_ELEM_DD - sum);
+        }
+        sm.misc[3] = k;
+        sm.misc[4] = (uint32_t)allowed;
+        sm.misc[5] = 0;
+    }
+    __syncthreads();
+    const uint32_t k = sm.misc[3];
+    int32_t allowed = (int32_t)sm.misc[4];
+    const uint32_t C = compact(sm, T, [k](uint32_t sc) { return sc >= k; });
+
+    uint32_t fword = 0, hash_cnt = 0;
+    for (uint32_t c0 = 0; c0 < C; c0 += SDBFX_NT) {
+        const uint32_t c = c0 + tid;
+        if (c < C) {
+            const uint32_t pos = sm.u.s.cand[c];
+            uint32_t h[5];
+            sha1_64(u + pos, h);
+#pragma unroll
+            for (int j = 0; j < 5; j++)
+                sm.u.s.hsha[j][tid] = h[j];
+            sm.u.s.csc[tid] = score_at(sm, pos);
+        }
+        __syncthreads();
+        if (wave == 0) {
+            bool done = false;
+            for (uint32_t sub = 0; sub < SDBFX_NT && c0 + sub < C && !done; sub += 64) {
+                const uint32_t cnt = min(64u, C - c0 - sub);
+                uint32_t myh[5];
+#pragma unroll
+                for (int j = 0; j < 5; j++)
+                    myh[j] = sm.u.s.hsha[j][sub + lane];
+                const uint32_t mysc = sm.u.s.csc[sub + lane];
+                for (uint32_t cc = 0; cc < cnt; cc++) {
+                    const uint32_t sc = (uint32_t)__builtin_amdgcn_readlane((int)mysc, (int)cc);
+                    if (!(sc > k || (sc == k && allowed > 0)))
+                        continue;
+                    uint32_t nb = 0;
+#pragma unroll
+                    for (int j = 0; j < 5; j++) {
+                        const uint32_t hj = (uint32_t)__builtin_amdgcn_readlane((int)myh[j], (int)cc);
+                        const uint32_t pos = hj & 0x7FFu;
+                        const uint32_t wi = pos >> 5;
+                        const uint32_t bit = 1u << (pos & 31);
+                        const uint32_t cur = (uint32_t)__builtin_amdgcn_readlane((int)fword, (int)wi);
+                        nb += (cur & bit) ? 0u : 1u;
+                        if ((uint32_t)lane == wi)
+                            fword |= bit;
+                    }
+                    if (nb == 0)
+                        continue;
+                    hash_cnt++;
+                    if (sc == k)
+                        allowed--;
+                    if (hash_cnt >= SDBFX_MAX_ELEM_DD) {
+                        done = true;
+                        break;
+                    }
+                }
+            }
+            if (lane == 0)
+                sm.misc[5] = done ? 1u : 0u;
+        }
+        __syncthreads();
+        if (sm.misc[5])
+            break;
+    }
+    if (wave == 0) {
+        ((uint32_t*)(filters + bd.out * SDBFX_BF))[lane] = fword;
+        uint32_t pc = __popc(fword);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            pc += __shfl_down(pc, off);
+        if (lane == 0) {
+            elem_counts[bd.out] = (uint16_t)hash_cnt;
+            hamming[bd.out] = (uint16_t)pc;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ stream mode
+
+__device__ void stream_tile(TileSmem& sm, const uint8_t* __restrict__ data, const TileDesc& td,
+                            uint32_t tile, int64_t forced, uint32_t* __restrict__ cand,
+                            TileState* __restrict__ tstate)
+{
+    const int tid = threadIdx.x;
+    const uint8_t* u = data + td.unit_off;
+    const uint32_t t_end = td.t0 + td.T;
+    const uint32_t hand_i = (t_end < td.n - 64) ? t_end - 65 : 0xFFFFFFFFu;
+    if (tid == 0)
+        sm.misc[2] = 0;
+    const uint32_t cert = tile_core(sm, u, td.n, td.t0, td.T, forced, hand_i, &sm.misc[2]);
+    const uint32_t C = compact(sm, td.T, [](uint32_t sc) { return sc > SDBFX_THRESHOLD; });
+    uint32_t* cw = cand + (uint64_t)tile * 5 * SDBFX_CAND_CAP;
+    for (uint32_t c = tid; c < C; c += SDBFX_NT) {
+        uint32_t h[5];
+        sha1_64(u + td.t0 + sm.u.s.cand[c], h);
+#pragma unroll
+        for (int j = 0; j < 5; j++)
+            cw[j * SDBFX_CAND_CAP + c] = h[j];
+    }
+    if (tid == 0) {
+        TileState st;
+        st.count = C;
+        st.handoff = sm.misc[2];
+        st.certified = cert;
+        st.pad = 0;
+        tstate[tile] = st;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(SDBFX_NT) void stream_tile_kernel(
+    const uint8_t* __restrict__ data, const TileDesc* __restrict__ tiles, uint32_t* __restrict__ cand,
+    TileState* __restrict__ tstate)
+{
+    __shared__ TileSmem sm;
+    const TileDesc td = tiles[blockIdx.x];
+    stream_tile(sm, data, td, blockIdx.x, -1, cand, tstate);
+}
+
+// Re-runs, in order, every tile of a unit whose incoming state was not proven,
+// from the exact hand-off state of the tile before it.
+__global__ __launch_bounds__(SDBFX_NT) void stream_fixup_kernel(
+    const uint8_t* __restrict__ data, const TileDesc* __restrict__ tiles, const UnitDesc* __restrict__ units,
+    uint32_t* __restrict__ cand, TileState* __restrict__ tstate)
+{
+    __shared__ TileSmem sm;
+    __shared__ uint32_t flag[2];
+    const UnitDesc ud = units[blockIdx.x];
+    for (uint32_t t = ud.tile_begin; t < ud.tile_end; t++) {
+        // one thread reads the flags; every thread takes the same branch
+        if (threadIdx.x == 0) {
+            flag[0] = ((volatile TileState*)tstate)[t].certified;
+            flag[1] = ((volatile TileState*)tstate)[t - 1].handoff;
+        }
+        __syncthreads();
+        const uint32_t cert = flag[0], h = flag[1];
+        __syncthreads();
+        if (cert)
+            continue;
+        const TileDesc td = tiles[t];
+        stream_tile(sm, data, td, t, (int64_t)h, cand, tstate);
+        __threadfence();
+        __syncthreads();
+    }
+}
+
+// Ordered insertion of one stream-mode input (gen_chunk_hash per unit,
+// sdbf_core.cc:94-136, with the de-duplication filter of sdbf_class.cc:378-380
+// and bloom_filter.cc:334-359), then the final trim (sdbf_core.cc:235-238).
+// One wave per input; lane w holds word w of the current 256-byte filter.
+__global__ __launch_bounds__(64) void stream_insert_kernel(
+    const StreamFile* __restrict__ files, const TileDesc* __restrict__ tiles,
+    const TileState* __restrict__ tstate, const uint32_t* __restrict__ cand, uint8_t* __restrict__ stage,
+    uint16_t* __restrict__ stage_ham, uint32_t* __restrict__ out_counts)
+{
+    __shared__ uint32_t big[SDBFX_BIG_BYTES / 4];
+    const StreamFile sf = files[blockIdx.x];
+    const int lane = threadIdx.x;
+    for (int q = lane; q < SDBFX_BIG_BYTES / 4; q += 64)
+        big[q] = 0;
+    __syncthreads();
+    uint32_t fword = 0, bfc = 1, last = 0, bigfi = 0, overflow = 0;
+
+    auto flush = [&](uint32_t idx) {
+        uint32_t pc = __popc(fword);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            pc += __shfl_down(pc, off);
+        if (idx < sf.filt_cap) {
+            ((uint32_t*)(stage + (sf.filt_base + idx) * SDBFX_BF))[lane] = fword;
+            if (lane == 0)
+                stage_ham[sf.filt_base + idx] = (uint16_t)pc;
+        } else {
+            overflow = 1;
+        }
+    };
+
+    for (uint32_t t = sf.tile_begin; t < sf.tile_end; t++) {
+        if (tiles[t].first)
+            bigfi = 0;
+        const uint32_t C = tstate[t].count;
+        const uint32_t* cw = cand + (uint64_t)t * 5 * SDBFX_CAND_CAP;
+        for (uint32_t c0 = 0; c0 < C; c0 += 64) {
+            const uint32_t cnt = min(64u, C - c0);
+            uint32_t myh[5];
+#pragma unroll
+            for (int j = 0; j < 5; j++)
+                myh[j] = (c0 + lane < C) ? cw[j * SDBFX_CAND_CAP + c0 + lane] : 0u;
+            for (uint32_t cc = 0; cc < cnt; cc++) {
+                uint32_t h[5];
+#pragma unroll
+                for (int j = 0; j < 5; j++)
+                    h[j] = (uint32_t)__builtin_amdgcn_readlane((int)myh[j], (int)cc);
+                uint32_t nb = 0;
+#pragma unroll
+                for (int j = 0; j < 5; j++) {
+                    const uint32_t pos = h[j] & 0x7FFu;
+                    const uint32_t wi = pos >> 5;
+                    const uint32_t bit = 1u << (pos & 31);
+                    const uint32_t cur = (uint32_t)__builtin_amdgcn_readlane((int)fword, (int)wi);
+                    nb += (cur & bit) ? 0u : 1u;
+                    if ((uint32_t)lane == wi)
+                        fword |= bit;
+                }
+                if (nb == 0)
+                    continue;
+                // de-duplication filter: inserted unless all 5 bits were set
+                uint32_t bw[5];
+#pragma unroll
+                for (int j = 0; j < 5; j++)
+                    bw[j] = big[(h[j] & 0x1FFFFu) >> 5];
+                uint32_t setc = 0;
+#pragma unroll
+                for (int j = 0; j < 5; j++) {
+                    const uint32_t bp = h[j] & 0x1FFFFu;
+                    uint32_t cur = bw[j];
+#pragma unroll
+                    for (int jj = 0; jj < j; jj++) {
+                        const uint32_t bq = h[jj] & 0x1FFFFu;
+                        if ((bq >> 5) == (bp >> 5))
+                            cur |= 1u << (bq & 31);
+                    }
+                    setc += (cur >> (bp & 31)) & 1u;
+                }
+                if (lane == 0) {
+#pragma unroll
+                    for (int j = 0; j < 5; j++) {
+                        const uint32_t bp = h[j] & 0x1FFFFu;
+                        atomicOr(&big[bp >> 5], 1u << (bp & 31));
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (setc == 5)
+                    continue;
+                last++;
+                bigfi++;
+                if (last == SDBFX_MAX_ELEM) {
+                    flush(bfc - 1);
+                    fword = 0;
+                    bfc++;
+                    last = 0;
+                }
+                if (bigfi == SDBFX_BIG_ELEM) {
+                    for (int q = lane; q < SDBFX_BIG_BYTES / 4; q += 64)
+                        big[q] = 0;
+                    __builtin_amdgcn_wave_barrier();
+                    bigfi = 0;
+                }
+            }
+        }
+    }
+    flush(bfc - 1);
+    if (bfc > 1 && last < SDBFX_MAX_ELEM / 8) {
+        bfc--;
+        last = SDBFX_MAX_ELEM;
+    }
+    if (lane == 0) {
+        out_counts[3 * blockIdx.x] = bfc;
+        out_counts[3 * blockIdx.x + 1] = last;
+        out_counts[3 * blockIdx.x + 2] = overflow;
+    }
+}
+
+// Copies staged stream filters into their final slots and writes their
+// element counts (160 for every filter but the last, which holds last_count;
+// sdbf/sdbf_class.cc:407-415).  One wave per input.
+__global__ __launch_bounds__(64) void gather_filters_kernel(
+    const uint8_t* __restrict__ stage, const uint16_t* __restrict__ stage_ham,
+    const uint64_t* __restrict__ src_first, const uint64_t* __restrict__ dst_first,
+    const uint32_t* __restrict__ count, const uint32_t* __restrict__ last_count, uint8_t* __restrict__ out,
+    uint16_t* __restrict__ ham, uint16_t* __restrict__ ecf)
+{
+    const uint32_t f = blockIdx.x;
+    const uint64_t s = src_first[f], d = dst_first[f];
+    const uint32_t nf = count[f], lc = last_count[f];
+    for (uint32_t i = 0; i < nf; i++) {
+        ((uint32_t*)(out + (d + i) * SDBFX_BF))[threadIdx.x] =
+            ((const uint32_t*)(stage + (s + i) * SDBFX_BF))[threadIdx.x];
+        if (threadIdx.x == 0) {
+            ham[d + i] = stage_ham[s + i];
+            ecf[d + i] = (uint16_t)(i + 1 < nf ? SDBFX_MAX_ELEM : lc);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ compare
+
+// Per reference-filter instance and target chunk: the maximum over the chunk's
+// target filters (element count >= 16) of sdbf_max_score's per-pair score,
+// stored as an exact fraction (num << 16 | den); 0xFFFFFFFF when no target
+// filter qualified (the reference's -1).
+// A filter set on the device: filters (64 words each), per-filter element
+// counts and Hamming weights.
+struct SetView {
+    const uint32_t* filt;
+    const uint16_t* ecf;
+    const uint16_t* hamf;
+};
+
+__global__ __launch_bounds__(64) void cmp_kernel(
+    SetView A, SetView B, const CmpGroup* __restrict__ groups, uint32_t ngroups,
+    const CmpPair* __restrict__ pairs, uint64_t nwork, uint32_t* __restrict__ partial)
+{
+    __shared__ uint16_t cut[1025];
+    const int lane = threadIdx.x;
+    for (int s = lane; s < 1025; s += 64) {
+        const uint32_t idx = s >= 28 ? 4096u / (uint32_t)s : 0u;
+        cut[s] = c_cut[idx];
+    }
+    __syncthreads();
+
+    for (uint64_t w = blockIdx.x; w < nwork; w += gridDim.x) {
+        // group of this work item
+        uint32_t lo = 0, hi = ngroups;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (groups[mid].wi_begin <= w)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const CmpGroup g = groups[lo];
+        const uint64_t local = w - g.wi_begin;
+        const uint32_t b = (uint32_t)(local / g.nch);
+        const uint32_t c = (uint32_t)(local % g.nch);
+        const uint32_t li = b * 64 + lane;
+        const bool valid = li < g.inst_count;
+        const uint64_t inst = g.inst_begin + li;
+
+        uint32_t a[64];
+        uint32_t s1 = 0, e1 = 0;
+        uint32_t p = g.pair_begin;
+        if (valid) {
+            uint32_t plo = g.pair_begin, phi = g.pair_end;
+            while (phi - plo > 1) {
+                const uint32_t mid = (plo + phi) >> 1;
+                if (pairs[mid].inst_begin <= inst)
+                    plo = mid;
+                else
+                    phi = mid;
+            }
+            p = plo;
+            const CmpPair pr = pairs[p];
+            const SetView R = pr.ref_set ? B : A;
+            const uint64_t rf = pr.ref_first + (inst - pr.inst_begin);
+            s1 = R.ecf[rf];
+            e1 = R.hamf[rf];
+            const uint4* src = (const uint4*)(R.filt + rf * SDBFX_BFW);
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const uint4 v = src[q];
+                a[4 * q] = v.x;
+                a[4 * q + 1] = v.y;
+                a[4 * q + 2] = v.z;
+                a[4 * q + 3] = v.w;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 64; q++)
+                a[q] = 0;
+        }
+
+        uint32_t best_n = 0, best_d = 0;
+        const uint64_t j0 = g.tgt_first + (uint64_t)c * SDBFX_CMP_CHUNK;
+        const uint64_t j1 = min(j0 + SDBFX_CMP_CHUNK, g.tgt_first + g.tgt_count);
+        const uint32_t* __restrict__ tf = g.tgt_set ? B.filt : A.filt;
+        const uint16_t* __restrict__ tec = g.tgt_set ? B.ecf : A.ecf;
+        const uint16_t* __restrict__ tham = g.tgt_set ? B.hamf : A.hamf;
+        for (uint64_t j = j0; j < j1; j++) {
+            const uint32_t s2 = tec[j];
+            if (s2 < SDBFX_MIN_ELEM)
+                continue;
+            const uint32_t e2 = tham[j];
+            const uint32_t* bj = tf + j * SDBFX_BFW;
+            uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                c0 += __popc(a[q] & bj[q]);
+#pragma unroll
+            for (int q = 8; q < 16; q++)
+                c1 += __popc(a[q] & bj[q]);
+#pragma unroll
+            for (int q = 16; q < 32; q++)
+                c2 += __popc(a[q] & bj[q]);
+#pragma unroll
+            for (int q = 32; q < 64; q++)
+                c3 += __popc(a[q] & bj[q]);
+            const uint32_t p32 = c0, p64 = c0 + c1, p128 = p64 + c2, p256 = p128 + c3;
+            const uint32_t sum = s1 + s2;
+            const uint32_t cutoff = sum <= 1024 ? cut[sum] : 1250u;
+            const bool pass = (8 * p32 + 48 >= cutoff) && (4 * p64 + 48 >= cutoff) &&
+                              (2 * p128 + 48 >= cutoff);
+            const uint32_t match = pass ? p256 : 0u;
+            uint32_t num = 0, den = 1;
+            if (match > cutoff) {
+                num = match - cutoff;
+                den = min(e1, e2) - cutoff;
+            }
+            if (best_d == 0 || num * best_d > best_n * den) {
+                best_n = num;
+                best_d = den;
+            }
+        }
+        if (valid)
+            partial[inst * g.nch + c] = best_d ? ((best_n << 16) | best_d) : 0xFFFFFFFFu;
+    }
+}
+
+// sdbf_score (sdbf_core.cc:331-389) for each oriented pair: ordered double
+// sum of the per-filter maxima with the reference's -1 sentinel handling,
+// sparse-filter denominator and floor(x + 0.5) rounding.
+__global__ void pair_sum_kernel(const CmpPair* __restrict__ pairs, uint32_t npairs,
+                                const uint16_t* __restrict__ ecfA, const uint16_t* __restrict__ ecfB,
+                                const uint32_t* __restrict__ partial, int32_t* __restrict__ scores)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs)
+        return;
+    const CmpPair P = pairs[p];
+    const uint16_t* __restrict__ ecf = P.ref_set ? ecfB : ecfA;
+    double sum = -1.0;
+    uint32_t sparse = 0;
+    for (uint32_t i = 0; i < P.ref_count; i++) {
+        const uint32_t s1 = ecf[P.ref_first + i];
+        double m;
+        if (s1 < SDBFX_MIN_ELEM) {
+            m = 0.0;
+            sparse++;
+        } else {
+            uint32_t bn = 0, bd = 0;
+            const uint32_t* v = partial + (P.inst_begin + i) * P.nch;
+            for (uint32_t c = 0; c < P.nch; c++) {
+                const uint32_t x = v[c];
+                if (x == 0xFFFFFFFFu)
+                    continue;
+                const uint32_t n = x >> 16, d = x & 0xFFFFu;
+                if (bd == 0 || n * bd > bn * d) {
+                    bn = n;
+                    bd = d;
+                }
+            }
+            m = bd ? (double)bn / (double)bd : -1.0;
+        }
+        sum = (sum < 0) ? m : sum + m;
+    }
+    uint64_t denom = P.ref_count;
+    if (P.ref_count > 1)
+        denom -= sparse;
+    if (denom == 0)
+        sum = -1.0;
+    scores[p] = sum < 0 ? -1 : (int32_t)floor(100.0 * sum / (double)denom + 0.5);
+}

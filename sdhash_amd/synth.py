@@ -18,10 +18,12 @@ SEG = 4096
 KIB = 1024
 MIB = 1 << 20
 
-# Zipf(s=1.1) over 64 ranks as an integer CDF with total 2**24.
+# Zipf(s=1.1) over 64 ranks as an integer CDF with total 2**16, and its
+# inverse as a 65536-entry lookup table (rank of a 16-bit uniform draw).
 _W = np.arange(1, 65, dtype=np.float64) ** -1.1
-_CDF = np.round(np.cumsum(_W) / _W.sum() * (1 << 24)).astype(np.int64)
-_CDF[-1] = 1 << 24
+_CDF = np.round(np.cumsum(_W) / _W.sum() * (1 << 16)).astype(np.int64)
+_CDF[-1] = 1 << 16
+_INV = np.searchsorted(_CDF, np.arange(1 << 16), side="right").astype(np.uint8)
 
 
 def _rng(seed: int) -> np.random.Generator:
@@ -41,9 +43,9 @@ def segment_mix(rng: np.random.Generator, nbytes: int) -> np.ndarray:
     if zpf.size:
         keys = rng.integers(0, 1 << 32, (zpf.size, 256), dtype=np.uint64)
         alpha = np.argsort(keys, axis=1, kind="stable")[:, :64].astype(np.uint8)
-        draw = rng.integers(0, 1 << 24, (zpf.size, SEG), dtype=np.int64)
-        sym = np.searchsorted(_CDF, draw, side="right")
-        out[zpf] = np.take_along_axis(alpha, sym, axis=1)
+        draw = rng.integers(0, 1 << 16, (zpf.size, SEG), dtype=np.uint16)
+        sym = _INV[draw]
+        out[zpf] = alpha[np.arange(zpf.size)[:, None], sym]
     if mot.size:
         motif = rng.integers(0, 256, (mot.size, 16), dtype=np.uint8)
         out[mot] = np.tile(motif, (1, SEG // 16))

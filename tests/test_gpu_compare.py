@@ -1,0 +1,97 @@
+"""Comparison parity on the GPU: every 0-100 score (and -1) identical with
+the CPU oracle, including the reference's orientation rule."""
+import numpy as np
+import pytest
+
+import oracle
+from sdhash_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+KIB = 1024
+MIB = 1 << 20
+
+
+def upload_oracle(engine, ds):
+    s = engine.new_set()
+    for d in ds:
+        s.add(d.name, d.orig_size, d.bf_count, d.last_count, d.dd_block_size, d.max_elem, d.filters,
+              d.elem_counts)
+    return s
+
+
+def test_compare_all_stream(engine):
+    files = synth.file_set(31, 24, 256 * KIB)
+    names = [b"f%02d" % i for i in range(len(files))]
+    ods = [oracle.digest_stream(f, n) for f, n in zip(files, names)]
+    want = oracle.compare_all(ods)
+    s = engine.new_set()
+    assert np.all(s.digest(files, names) == 0)
+    got = engine.compare_all(s)
+    assert np.array_equal(got, want), np.flatnonzero(got != want)[:10]
+    assert (want > 0).sum() > 10
+
+
+def test_compare_all_mixed_modes_and_names(engine):
+    files = synth.file_set(32, 10, 200 * KIB)
+    # names chosen so that the strcmp part of the orientation rule matters
+    names = [b"z", b"a", b"m", b"b", b"y", b"c", b"x", b"d", b"w", b"e"]
+    ods = []
+    for i, (f, n) in enumerate(zip(files, names)):
+        ods.append(oracle.digest_block(f, 16384, n) if i % 3 == 0 else oracle.digest_stream(f, n))
+    s = upload_oracle(engine, ods)
+    want = oracle.compare_all(ods)
+    got = engine.compare_all(s)
+    assert np.array_equal(got, want)
+
+
+def test_compare_to_fragments_vs_image(engine):
+    img = synth.disk_image(33, 2 * MIB)
+    frags = synth.fragments(34, img, 256)
+    tgt = [oracle.digest_block(img[:MIB], 16384, b"img.0000M"),
+           oracle.digest_block(img[MIB:], 16384, b"img.0001M")]
+    q = [oracle.digest_stream(f, b"frag%04d" % i) for i, f in enumerate(frags)]
+    want = oracle.compare_to(q, tgt)
+    gq = engine.new_set()
+    assert np.all(gq.digest(frags, [d.name for d in q]) == 0)
+    gt = engine.new_set()
+    assert np.all(gt.digest([img[:MIB], img[MIB:]], [b"img.0000M", b"img.0001M"], dd_block=16384) == 0)
+    got = engine.compare_to(gq, gt)
+    assert np.array_equal(got, want)
+    assert (want[:128] > 0).sum() > 50
+
+
+def test_compare_hand_cases(engine):
+    def bits(*ranges):
+        v = np.zeros(2048, np.uint8)
+        for lo, hi in ranges:
+            v[lo:hi] = 1
+        return np.packbits(v, bitorder="little")
+    a = oracle.Digest(b"a", bits((0, 600)), 1, 160, None, 1000, 0)
+    b = oracle.Digest(b"b", bits((0, 500), (1500, 1600)), 1, 160, None, 1000, 0)
+    c = oracle.Digest(b"c", bits((0, 90), (1024, 1624)), 1, 160, None, 1000, 0)
+    d = oracle.Digest(b"d", bits((0, 90), (1024, 1624)), 1, 160, None, 1000, 0)
+    e = oracle.Digest(b"e", np.concatenate([bits((0, 600)), bits((0, 600))]), 2, 0,
+                      np.array([5, 5], np.uint16), 1000, 16384)
+    ds = [a, b, c, d, e]
+    s = upload_oracle(engine, ds)
+    got = engine.compare_all(s)
+    want = oracle.compare_all(ds)
+    assert np.array_equal(got, want)
+    assert got[0] == 72          # a|b
+    assert got[3] == -1          # a|e: every target filter sparse
+
+
+def test_compare_large_target_chunks(engine):
+    # a target digest of > 1024 filters is split over several work items
+    img = synth.disk_image(35, 20 * MIB)
+    frags = synth.fragments(36, img, 64, size=64 * KIB)
+    t = [oracle.digest_block(img, 16384, b"img")]
+    q = [oracle.digest_stream(f, b"q%02d" % i) for i, f in enumerate(frags)]
+    want = oracle.compare_to(q, t)
+    gq = upload_oracle(engine, q)
+    gt = upload_oracle(engine, t)
+    got = engine.compare_to(gq, gt)
+    assert np.array_equal(got, want)
+    # and the transposed call exercises the other orientation
+    assert np.array_equal(engine.compare_to(gt, gq), oracle.compare_to(t, q))

@@ -1,0 +1,117 @@
+"""Digest parity on the GPU: filters, element counts and Hamming weights of
+every digest are bit-identical with the CPU oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from sdhash_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+KIB = 1024
+MIB = 1 << 20
+
+
+def check_same(gset, i, od, label):
+    f, ec, hm = gset.read(i)
+    info = gset.info(i)
+    assert info.bf_count == od.bf_count, (label, info.bf_count, od.bf_count)
+    if od.elem_counts is None:
+        assert info.last_count == od.last_count, label
+    bad = np.flatnonzero(f != od.filters)
+    assert bad.size == 0, (label, "first differing filter", bad[0] // 256 if bad.size else None)
+    od.cstruct()
+    assert np.array_equal(hm, od.hamming[: od.bf_count]), label
+    if od.elem_counts is not None:
+        assert np.array_equal(ec, od.elem_counts), label
+    else:
+        want = np.full(od.bf_count, 160, np.uint16)
+        want[-1] = od.last_count
+        assert np.array_equal(ec, want), label
+
+
+def test_block_mode_parity(engine):
+    img = synth.disk_image(21, 4 * MIB + 16 * KIB * 3 + 700)
+    pieces = [img, img[: 16 * KIB * 5 + 300], img[:512], img[:513], img[: 16 * KIB - 1],
+              img[100: 100 + 40000]]
+    print("block sizes", [p.size for p in pieces], flush=True)
+    s = engine.new_set()
+    st = s.digest(pieces, dd_block=16384)
+    assert np.all(st == 0)
+    for i, p in enumerate(pieces):
+        check_same(s, i, oracle.digest_block(p, 16384), f"block {i} size {p.size}")
+
+
+def test_block_mode_small_dd(engine):
+    img = synth.disk_image(22, 300 * KIB + 1000)
+    s = engine.new_set()
+    assert np.all(s.digest([img], dd_block=4096) == 0)
+    check_same(s, 0, oracle.digest_block(img, 4096), "dd 4096")
+
+
+def test_stream_mode_parity_sizes(engine):
+    rng = np.random.Generator(np.random.PCG64(23))
+    sizes = [512, 600, 1000, 4 * KIB, 16 * KIB + 64, 16 * KIB + 65, 100 * KIB, MIB, 3 * MIB + 17]
+    bufs = [synth.segment_mix(rng, n) for n in sizes]
+    print("stream sizes", sizes, flush=True)
+    s = engine.new_set()
+    st = s.digest(bufs)
+    assert np.all(st == 0)
+    for i, b in enumerate(bufs):
+        check_same(s, i, oracle.digest_stream(b), f"stream size {b.size}")
+
+
+def test_stream_mode_planted_set(engine):
+    files = synth.file_set(24, 16, MIB)
+    s = engine.new_set()
+    assert np.all(s.digest(files) == 0)
+    for i, b in enumerate(files):
+        check_same(s, i, oracle.digest_stream(b), f"file {i}")
+
+
+def test_stream_mode_multi_chunk(engine):
+    # 32 MiB restart boundary: two units, de-duplication count reset between
+    b = synth.disk_image(25, 32 * MIB + 3 * MIB + 5)
+    print("stream 35 MiB", flush=True)
+    s = engine.new_set()
+    assert np.all(s.digest([b]) == 0)
+    check_same(s, 0, oracle.digest_stream(b), "35 MiB")
+
+
+def test_too_small_inputs_are_skipped(engine):
+    s = engine.new_set()
+    st = s.digest([np.zeros(100, np.uint8), np.zeros(511, np.uint8), np.ones(512, np.uint8)])
+    assert list(st) == [-3, -3, 0]
+    assert len(s) == 1
+
+
+def test_degenerate_inputs(engine):
+    bufs = [np.zeros(100000, np.uint8), np.full(70000, 7, np.uint8),
+            np.tile(np.arange(64, dtype=np.uint8), 2000), np.tile(np.arange(16, dtype=np.uint8), 5000)]
+    s = engine.new_set()
+    assert np.all(s.digest(bufs) == 0)
+    for i, b in enumerate(bufs):
+        check_same(s, i, oracle.digest_stream(b), f"degenerate {i}")
+    t = engine.new_set()
+    assert np.all(t.digest(bufs, dd_block=16384) == 0)
+    for i, b in enumerate(bufs):
+        check_same(t, i, oracle.digest_block(b, 16384), f"degenerate block {i}")
+
+
+def test_device_resident_input(engine):
+    files = synth.file_set(26, 4, 300 * KIB)
+    offs = []
+    pos = 0
+    for f in files:
+        offs.append(pos)
+        pos += (f.size + 255) // 256 * 256
+    buf = engine.alloc(pos)
+    for o, f in zip(offs, files):
+        buf.upload(o, f)
+    s = engine.new_set()
+    st = s.digest_device(buf, offs, [f.size for f in files], dd_block=[0, 16384, 0, 16384])
+    assert np.all(st == 0)
+    for i, f in enumerate(files):
+        od = oracle.digest_block(f, 16384) if i % 2 else oracle.digest_stream(f)
+        check_same(s, i, od, f"device {i}")
+    buf.free()
