@@ -11,6 +11,9 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsdbfx.so")
+CXXLIB = os.path.join(HERE, "libsdbf_amd.so")
+CLI = os.path.join(HERE, "sdhash-amd")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -23,16 +26,29 @@ FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> str:
-    cmd = [HIPCC, *FLAGS, "-x", "hip", os.path.join(CSRC, "sdbfx_host.cpp"), "-o", LIB + ".tmp"]
+def _run(cmd, verbose):
     r = subprocess.run(cmd, capture_output=True, text=True)
     noise = ("packed-fp32-ops",)
     msg = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if not any(n in l for n in noise))
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + msg)
+        raise RuntimeError(f"{cmd[0]} failed:\n" + msg)
     if verbose and msg.strip():
         print(msg, file=sys.stderr)
+
+
+def build(verbose: bool = False) -> str:
+    # the engine: kernels + C ABI
+    _run([HIPCC, *FLAGS, "-I", INCLUDE, "-x", "hip", os.path.join(CSRC, "sdbfx_host.cpp"), "-o",
+          LIB + ".tmp"], verbose)
     os.replace(LIB + ".tmp", LIB)
+    # the C++ sdbf / sdbf_set classes over the C ABI, and the sdhash CLI
+    cxx = ["g++", "-std=c++17", "-O2", "-fPIC", "-Wall", "-I", INCLUDE]
+    _run([*cxx, "-shared", os.path.join(CSRC, "sdbf_api.cpp"), "-o", CXXLIB + ".tmp", "-L", HERE,
+          "-l:libsdbfx.so", "-Wl,-rpath,$ORIGIN"], verbose)
+    os.replace(CXXLIB + ".tmp", CXXLIB)
+    _run([*cxx, os.path.join(CSRC, "sdhash_cli.cpp"), "-o", CLI + ".tmp", "-L", HERE, "-l:libsdbf_amd.so",
+          "-l:libsdbfx.so", "-Wl,-rpath,$ORIGIN"], verbose)
+    os.replace(CLI + ".tmp", CLI)
     return LIB
 
 

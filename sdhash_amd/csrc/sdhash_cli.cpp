@@ -1,0 +1,431 @@
+// sdhash-amd: the sdhash command line (sdhash-src/sdhash.cc) for the hashing
+// and comparison paths, running on the gfx950 engine.
+//
+//   sdhash-amd [options] files...      hash; print digests
+//   sdhash-amd -g files...             hash, then compare all pairs
+//   sdhash-amd -c a.sdbf [b.sdbf]      compare digests of one or two files
+//
+// Options: -r/--deep, -g/--gen-compare, -c/--compare, -t/--threshold N,
+// -b/--block-size KB, -p/--threads N (accepted; the device does the work),
+// -z/--segment-size MB, -o/--output BASE, --separator pipe|csv|tab,
+// --hash-name NAME (stdin), --validate, --verbose, --version, -h/--help.
+// File policy as sdhash.cc:502-618: inputs below 16 MiB are stream-hashed
+// unless -b > 0; larger ones use 16 KiB blocks unless -b 0 (or -b N KiB),
+// cut into segments named "<file>.NNNNM" (sdhash_threads.cc:300-373).
+#include <dirent.h>
+#include <sys/stat.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "sdbf_amd.h"
+
+namespace {
+
+const uint64_t KB = 1024, MB = 1024 * 1024;
+
+struct Opts {
+    bool deep = false, gen = false, cmp = false, validate = false, verbose = false;
+    int32_t threshold = 1;
+    int32_t dd = -1;               // KiB; -1 = automatic
+    uint64_t segment = 128 * MB;
+    std::string output, separator = "pipe", hash_name;
+    std::vector<std::string> inputs;
+};
+
+void usage()
+{
+    std::cout << "sdhash-amd (sdbf engine for AMD MI355X)\n\n"
+                 "Usage: sdhash-amd <options> <files>\n"
+                 "  -r [ --deep ]           generate SDBFs from directories and files\n"
+                 "  -c [ --compare ]        compare SDBFs in file, or two SDBF files\n"
+                 "  -g [ --gen-compare ]    compare all pairs in source data\n"
+                 "  -t [ --threshold ] N    only show results >= N (default 1)\n"
+                 "  -b [ --block-size ] N   hash input files in N KiB blocks\n"
+                 "  -p [ --threads ] N      accepted for compatibility\n"
+                 "  -z [ --segment-size ] N segment size in MiB (default 128)\n"
+                 "  -o [ --output ] BASE    write BASE.sdbf / BASE.compare\n"
+                 "  --separator S           pipe, csv or tab\n"
+                 "  --hash-name NAME        name of the digest of stdin\n"
+                 "  --validate              parse SDBF files and report their contents\n"
+                 "  --verbose               progress output\n";
+}
+
+bool parse(int argc, char **argv, Opts &o)
+{
+    for (int i = 1; i < argc; i++) {
+        std::string a = argv[i], val;
+        auto need = [&](const char *optname) -> bool {
+            size_t eq = a.find('=');
+            if (a.rfind("--", 0) == 0 && eq != std::string::npos) {
+                val = a.substr(eq + 1);
+                return true;
+            }
+            if (a.size() > 2 && a[0] == '-' && a[1] != '-') {
+                val = a.substr(2);
+                return true;
+            }
+            if (i + 1 >= argc) {
+                std::cerr << "sdhash-amd: missing value for " << optname << std::endl;
+                return false;
+            }
+            val = argv[++i];
+            return true;
+        };
+        auto is = [&](const char *s, const char *l) {
+            const std::string L = l;
+            return (s && a.rfind(s, 0) == 0 && (a.size() == 2 || (a.size() > 2 && std::string(s).size() == 2 &&
+                                                                   strchr("tbpzo", s[1]))) ) ||
+                   a == L || a.rfind(L + "=", 0) == 0;
+        };
+        if (a == "-h" || a == "--help") {
+            usage();
+            exit(0);
+        } else if (a == "--version") {
+            std::cout << "sdhash-amd 4.0-compatible (gfx950)" << std::endl;
+            exit(0);
+        } else if (a == "-r" || a == "--deep") {
+            o.deep = true;
+        } else if (a == "-g" || a == "--gen-compare") {
+            o.gen = true;
+        } else if (a == "-c" || a == "--compare") {
+            o.cmp = true;
+        } else if (a == "--validate") {
+            o.validate = true;
+        } else if (a == "--verbose") {
+            o.verbose = true;
+        } else if (is("-t", "--threshold")) {
+            if (!need("threshold"))
+                return false;
+            o.threshold = atoi(val.c_str());
+        } else if (is("-b", "--block-size")) {
+            if (!need("block-size"))
+                return false;
+            o.dd = atoi(val.c_str());
+        } else if (is("-p", "--threads")) {
+            if (!need("threads"))
+                return false;
+        } else if (is("-z", "--segment-size")) {
+            if (!need("segment-size"))
+                return false;
+            o.segment = strtoull(val.c_str(), nullptr, 10) * MB;
+        } else if (is("-o", "--output")) {
+            if (!need("output"))
+                return false;
+            o.output = val;
+        } else if (is(nullptr, "--separator")) {
+            if (!need("separator"))
+                return false;
+            o.separator = val;
+        } else if (is(nullptr, "--hash-name")) {
+            if (!need("hash-name"))
+                return false;
+            o.hash_name = val;
+        } else if (a.size() > 1 && a[0] == '-' && a != "-") {
+            std::cerr << "sdhash-amd: unknown option " << a << std::endl;
+            return false;
+        } else {
+            o.inputs.push_back(a);
+        }
+    }
+    // sdhash.cc:209-218
+    if (o.dd > 16)
+        o.dd = 16;
+    if (o.dd < -1)
+        o.dd = -1;
+    return true;
+}
+
+char sep_char(const std::string &s)
+{
+    if (s == "csv")
+        return ',';
+    if (s == "tab" || s == "tabs")
+        return '\t';
+    return '|';
+}
+
+bool regular(const std::string &p, uint64_t *size)
+{
+    struct stat st;
+    if (stat(p.c_str(), &st) != 0 || !S_ISREG(st.st_mode))
+        return false;
+    *size = (uint64_t)st.st_size;
+    return true;
+}
+
+void walk(const std::string &dir, std::vector<std::string> &out)
+{
+    DIR *d = opendir(dir.c_str());
+    if (!d)
+        return;
+    std::vector<std::string> names;
+    while (dirent *e = readdir(d)) {
+        if (!strcmp(e->d_name, ".") || !strcmp(e->d_name, ".."))
+            continue;
+        names.push_back(dir + "/" + e->d_name);
+    }
+    closedir(d);
+    for (const auto &n : names) {
+        struct stat st;
+        if (lstat(n.c_str(), &st) != 0)
+            continue;
+        if (S_ISDIR(st.st_mode))
+            walk(n, out);
+        else if (S_ISREG(st.st_mode))
+            out.push_back(n);
+    }
+}
+
+bool read_file(const std::string &p, std::vector<uint8_t> &data)
+{
+    std::ifstream is(p, std::ios::binary);
+    if (!is)
+        return false;
+    is.seekg(0, std::ios::end);
+    const std::streamoff n = is.tellg();
+    is.seekg(0);
+    data.resize((size_t)n);
+    is.read((char *)data.data(), n);
+    return is.gcount() == n;
+}
+
+// One digest to compute: name, bytes (zero-padded to `len`), actual size read.
+struct Job {
+    std::string name;
+    std::vector<uint8_t> data;
+    uint64_t got;
+};
+
+// Segmentation of sdhash_threads.cc:300-373 for one file of `filesize`
+// bytes, with its byte-exact quirk: the last segment after a short tail is
+// read with the tail counted twice (zero-filled past the end of the file).
+void segment_jobs(const std::string &name, const std::vector<uint8_t> &file, uint64_t chunk,
+                  std::vector<Job> &jobs)
+{
+    const uint64_t filesize = file.size();
+    if (!(filesize > chunk && chunk > 0)) {
+        jobs.push_back(Job{name, file, filesize});
+        return;
+    }
+    int64_t chunks = (int64_t)(filesize / chunk);
+    int tailflag = 0;
+    if (filesize - chunk <= 512)
+        chunks = 0;
+    if (filesize - (uint64_t)chunks * chunk <= 512) {
+        chunks--;
+        tailflag = 1;
+    }
+    uint64_t pos = 0;
+    for (int64_t j = 0; j <= chunks; j++) {
+        std::stringstream ns;
+        ns << name;
+        if (j > 0 || chunks > 0) {
+            ns.fill('0');
+            ns << "." << std::setw(4) << (uint64_t)j * chunk / MB << "M";
+        }
+        uint64_t csize;
+        if (j == chunks) {
+            csize = filesize - (uint64_t)chunks * chunk;
+            if (tailflag)
+                csize = csize + (filesize - (uint64_t)(chunks + 1) * chunk);
+        } else {
+            csize = chunk;
+        }
+        const uint64_t got = std::min<uint64_t>(csize, filesize - pos);
+        Job jb{ns.str(), std::vector<uint8_t>(csize, 0), got};
+        memcpy(jb.data.data(), file.data() + pos, got);
+        pos += got;
+        jobs.push_back(std::move(jb));
+    }
+}
+
+// Digests jobs in one device batch and appends them to `set` (or prints).
+int run_jobs(std::vector<Job> &jobs, uint32_t dd_bytes, sdbf_set *set)
+{
+    std::vector<std::string> names;
+    std::vector<const uint8_t *> bufs;
+    std::vector<uint64_t> lens;
+    std::vector<size_t> idx;
+    for (size_t i = 0; i < jobs.size(); i++) {
+        if (jobs[i].got < 512)   // the constructors' -3: skipped silently
+            continue;
+        names.push_back(jobs[i].name);
+        bufs.push_back(jobs[i].data.data());
+        lens.push_back(jobs[i].data.size());
+        idx.push_back(i);
+    }
+    std::vector<sdbf *> out;
+    if (!bufs.empty())
+        sdbf::hash_batch(names, bufs, lens, dd_bytes, out);
+    for (size_t k = 0; k < out.size(); k++) {
+        if (!out[k])
+            continue;
+        (void)idx;
+        if (set)
+            set->add(out[k]);
+        else {
+            std::cout << out[k];
+            delete out[k];
+        }
+    }
+    jobs.clear();
+    return 0;
+}
+
+int write_file(const std::string &s, const std::string &fn)
+{
+    std::ofstream os(fn, std::ios::binary);
+    if (!os) {
+        std::cerr << "sdhash: ERROR cannot write to file " << fn << std::endl;
+        return -1;
+    }
+    os << s;
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    Opts o;
+    if (!parse(argc, argv, o))
+        return 1;
+    sdbf::config->warnings = o.verbose ? 1 : 0;
+    const char sep = sep_char(o.separator);
+    try {
+        if (o.cmp) {
+            if (o.inputs.size() == 1) {
+                sdbf_set *s1 = new sdbf_set(o.inputs[0].c_str());
+                s1->set_separator(sep);
+                if (!o.output.empty())
+                    write_file(s1->compare_all_quiet(o.threshold, 0, false), o.output + ".compare");
+                else
+                    s1->compare_all(o.threshold, false);
+            } else if (o.inputs.size() == 2) {
+                sdbf_set *s1 = new sdbf_set(o.inputs[0].c_str());
+                sdbf_set *s2 = new sdbf_set(o.inputs[1].c_str());
+                s1->set_separator(sep);
+                if (!o.output.empty())
+                    write_file(s1->compare_to_quiet(s2, o.threshold, 0, 0, false), o.output + ".compare");
+                else
+                    s1->compare_to(s2, o.threshold, 0, false);
+            } else {
+                std::cerr << "sdhash: ERROR: Comparison requires 1 or 2 arguments." << std::endl;
+                return -1;
+            }
+            return 0;
+        }
+        if (o.validate) {
+            for (const auto &f : o.inputs) {
+                uint64_t sz;
+                if (!regular(f, &sz)) {
+                    std::cout << "sdhash: ERROR file " << f << " not readable or not found." << std::endl;
+                    continue;
+                }
+                sdbf_set s(f.c_str());
+                std::cout << "sdhash: file " << f << " SDBFs valid, contains " << s.size() << " hashes." << std::endl;
+                std::cout << "contains " << s.filter_count() << " bloom filters." << std::endl;
+                std::cout << s.input_size() << " total data size. " << std::endl;
+            }
+            return 0;
+        }
+        const bool keep = o.gen || !o.output.empty();
+        sdbf_set *set1 = keep ? new sdbf_set() : nullptr;
+        std::vector<Job> stream_jobs, block_jobs;
+        uint32_t block_bytes = 0;
+        if (o.inputs.size() == 1 && o.inputs[0] == "-") {
+            // stdin, always block mode (sdhash.cc:453-465, sdhash_threads.cc:189-233)
+            const uint32_t dd = o.dd > 0 ? (uint32_t)o.dd : 16;
+            const std::string base = o.hash_name.empty() ? "stdin" : o.hash_name;
+            std::vector<uint8_t> buf(o.segment);
+            for (uint64_t j = 0;; j++) {
+                std::fill(buf.begin(), buf.end(), 0);
+                const size_t got = fread(buf.data(), 1, buf.size(), stdin);
+                std::stringstream ns;
+                ns.fill('0');
+                ns << base << "." << std::setw(4) << j * o.segment / MB << "M";
+                std::vector<Job> jobs{Job{ns.str(), std::vector<uint8_t>(buf.begin(), buf.begin() + got), got}};
+                run_jobs(jobs, dd * KB, set1);
+                if (got != buf.size())
+                    break;
+            }
+        } else {
+            std::vector<std::string> small, large;
+            for (const auto &in : o.inputs) {
+                std::vector<std::string> found;
+                uint64_t sz;
+                if (o.deep) {
+                    struct stat st;
+                    if (stat(in.c_str(), &st) == 0 && S_ISDIR(st.st_mode))
+                        walk(in, found);
+                }
+                if (regular(in, &sz))
+                    found.push_back(in);
+                for (const auto &f : found) {
+                    if (!regular(f, &sz))
+                        continue;
+                    (sz < 16 * MB ? small : large).push_back(f);
+                }
+            }
+            // sdhash.cc:554-618
+            auto add_stream = [&](const std::string &f) {
+                std::vector<uint8_t> d;
+                if (read_file(f, d))
+                    stream_jobs.push_back(Job{f, std::move(d), 0}), stream_jobs.back().got =
+                                                                        stream_jobs.back().data.size();
+            };
+            auto add_block = [&](const std::string &f, uint32_t kb) {
+                std::vector<uint8_t> d;
+                if (read_file(f, d))
+                    segment_jobs(f, d, o.segment, block_jobs);
+                block_bytes = kb * KB;
+            };
+            for (const auto &f : small) {
+                if (o.dd < 1)
+                    add_stream(f);
+                else
+                    add_block(f, (uint32_t)o.dd);
+            }
+            if (!stream_jobs.empty())
+                run_jobs(stream_jobs, 0, set1);
+            if (!block_jobs.empty())
+                run_jobs(block_jobs, block_bytes, set1);
+            for (const auto &f : large) {
+                if (o.dd == 0)
+                    add_stream(f);
+                else
+                    add_block(f, o.dd == -1 ? 16 : (uint32_t)o.dd);
+                // large inputs are digested one file at a time (bounded host memory)
+                if (!stream_jobs.empty())
+                    run_jobs(stream_jobs, 0, set1);
+                if (!block_jobs.empty())
+                    run_jobs(block_jobs, block_bytes, set1);
+            }
+        }
+        if (o.gen) {
+            set1->set_separator(sep);
+            if (!o.output.empty())
+                write_file(set1->compare_all_quiet(o.threshold, 0, false), o.output + ".compare");
+            else
+                set1->compare_all(o.threshold, false);
+        } else if (!o.output.empty()) {
+            write_file(set1->to_string(), o.output + ".sdbf");
+        }
+    } catch (int e) {
+        if (e == -2) {
+            std::cerr << "sdhash: ERROR: invalid SDBF input" << std::endl;
+            return -2;
+        }
+        std::cerr << "sdhash: ERROR (" << e << ")" << std::endl;
+        return -1;
+    }
+    return 0;
+}
