@@ -458,7 +458,7 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
     }
     std::vector<CmpPair> pairs(npairs);
     std::vector<CmpGroup> groups;
-    uint64_t inst = 0, wi = 0;
+    uint64_t inst = 0, wi = 0, partial_n = 0;
     for (uint32_t q = 0; q < nkeys; q++) {
         if (kc[q] == kc[q + 1])
             continue;
@@ -472,6 +472,7 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
         g.pair_begin = kc[q];
         g.pair_end = kc[q + 1];
         g.tgt_set = t.set;
+        g.partial_begin = partial_n;
         for (uint32_t s = kc[q]; s < kc[q + 1]; s++) {
             const PairRef r = ref[order[s]];
             const sdbfx_set* R = S[r.set];
@@ -480,6 +481,7 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
             p.ref_count = R->bfc[r.idx];
             p.nch = g.nch;
             p.inst_begin = inst;
+            p.partial_begin = g.partial_begin + (inst - g.inst_begin) * g.nch;
             p.ref_set = r.set;
             p.pad = 0;
             inst += p.ref_count;
@@ -491,11 +493,9 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
         g.nblk = (g.inst_count + 63) / 64;
         g.wi_begin = wi;
         wi += (uint64_t)g.nblk * g.nch;
+        partial_n += (uint64_t)g.inst_count * g.nch;
         groups.push_back(g);
     }
-    uint64_t partial_n = 0;
-    for (const auto& g : groups)
-        partial_n += (uint64_t)g.inst_count * g.nch;
 
     int rc;
     if ((rc = upload(ctx, ctx->groups, groups)) || (rc = upload(ctx, ctx->pairs, pairs)) ||

@@ -19,7 +19,9 @@
 // digest workgroups
 #define SDBFX_NT 256             // threads per digest workgroup
 #define SDBFX_TILE 16384         // owned score positions per stream tile
+#ifndef SDBFX_WARM
 #define SDBFX_WARM 128           // speculative warm-up steps
+#endif
 #define SDBFX_CAND_CAP 1024      // candidate slots per stream tile (>= (TILE+64)/17)
 #define SDBFX_PAD 256            // alignment / padding of inputs in device memory
 
@@ -76,6 +78,7 @@ struct CmpGroup {
     uint32_t pair_begin;   // pairs of the group (sorted by instance offset)
     uint32_t pair_end;
     uint64_t wi_begin;     // first work item of the group
+    uint64_t partial_begin;// first partial-maximum slot of the group
     uint32_t tgt_set;      // 0: target filters in set A, 1: in set B
     uint32_t pad;
 };
@@ -86,6 +89,7 @@ struct CmpPair {
     uint32_t ref_count;    // its filters
     uint32_t nch;          // chunks of the target digest
     uint64_t inst_begin;   // first instance index
+    uint64_t partial_begin;// its first partial-maximum slot (nch per filter)
     uint32_t ref_set;      // 0: reference filters in set A, 1: in set B
     uint32_t pad;
 };

@@ -612,10 +612,14 @@ __global__ __launch_bounds__(SDBFX_NT) void stream_fixup_kernel(
     __shared__ uint32_t flag[2];
     const UnitDesc ud = units[blockIdx.x];
     for (uint32_t t = ud.tile_begin; t < ud.tile_end; t++) {
-        // one thread reads the flags; every thread takes the same branch
+        // one thread reads the flags; every thread takes the same branch.
+        // The first tile of a unit starts at step 0 and is always exact, so
+        // a tile that needs a hand-off always has a predecessor in the unit.
         if (threadIdx.x == 0) {
-            flag[0] = ((volatile TileState*)tstate)[t].certified;
-            flag[1] = ((volatile TileState*)tstate)[t - 1].handoff;
+            const uint32_t c = ((volatile TileState*)tstate)[t].certified;
+            const bool redo = !c && t > ud.tile_begin;
+            flag[0] = redo ? 0u : 1u;
+            flag[1] = redo ? ((volatile TileState*)tstate)[t - 1].handoff : 0u;
         }
         __syncthreads();
         const uint32_t cert = flag[0], h = flag[1];
@@ -707,6 +711,8 @@ __global__ __launch_bounds__(64) void stream_insert_kernel(
                     }
                     setc += (cur >> (bp & 31)) & 1u;
                 }
+                // every lane has read the words before lane 0 updates them
+                __builtin_amdgcn_wave_barrier();
                 if (lane == 0) {
 #pragma unroll
                     for (int j = 0; j < 5; j++) {
@@ -887,7 +893,7 @@ __global__ __launch_bounds__(64) void cmp_kernel(
             }
         }
         if (valid)
-            partial[inst * g.nch + c] = best_d ? ((best_n << 16) | best_d) : 0xFFFFFFFFu;
+            partial[g.partial_begin + (uint64_t)li * g.nch + c] = best_d ? ((best_n << 16) | best_d) : 0xFFFFFFFFu;
     }
 }
 
@@ -913,7 +919,7 @@ __global__ void pair_sum_kernel(const CmpPair* __restrict__ pairs, uint32_t npai
             sparse++;
         } else {
             uint32_t bn = 0, bd = 0;
-            const uint32_t* v = partial + (P.inst_begin + i) * P.nch;
+            const uint32_t* v = partial + P.partial_begin + (uint64_t)i * P.nch;
             for (uint32_t c = 0; c < P.nch; c++) {
                 const uint32_t x = v[c];
                 if (x == 0xFFFFFFFFu)

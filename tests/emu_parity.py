@@ -1,0 +1,96 @@
+"""Parity cases run against the CPU emulation of the kernels (see
+tests/hipemu): the unmodified kernel source, built for the host with
+AddressSanitizer, digests and compares small inputs that are checked
+against the oracle.  Run by tests/test_kernels_emulated.py in a subprocess
+with libasan preloaded:  python tests/emu_parity.py <libsdbfx_emu.so>"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+import sdhash_amd  # noqa: E402
+from sdhash_amd import synth  # noqa: E402
+
+KIB = 1024
+
+
+def same(gset, i, od, label):
+    f, ec, hm = gset.read(i)
+    info = gset.info(i)
+    assert info.bf_count == od.bf_count, (label, info.bf_count, od.bf_count)
+    assert np.array_equal(f, od.filters), label
+    od.cstruct()
+    assert np.array_equal(hm, od.hamming[: od.bf_count]), label
+    if od.elem_counts is not None:
+        assert np.array_equal(ec, od.elem_counts), label
+    else:
+        assert info.last_count == od.last_count, label
+
+
+def main(lib):
+    sdhash_amd.LIB_PATH = lib
+    eng = sdhash_amd.Engine(0)
+    rng = np.random.Generator(np.random.PCG64(5))
+
+    # block mode: full blocks, tails >= 512 and < 512, tiny inputs, dd 4096
+    img = synth.disk_image(6, 3 * 16 * KIB + 700)
+    pieces = [img, img[: 2 * 16 * KIB + 300], img[:512], img[:513], img[: 16 * KIB - 1]]
+    s = eng.new_set()
+    assert np.all(s.digest(pieces, dd_block=16384) == 0)
+    for i, p in enumerate(pieces):
+        same(s, i, oracle.digest_block(p, 16384), "block %d" % i)
+    s = eng.new_set()
+    assert np.all(s.digest([img], dd_block=4096) == 0)
+    same(s, 0, oracle.digest_block(img, 4096), "dd 4096")
+    print("block ok", flush=True)
+
+    # stream mode: one tile, tile boundaries, several tiles, degenerate data
+    sizes = [512, 600, 4 * KIB, 16 * KIB + 64, 16 * KIB + 65, 40000, 70000]
+    bufs = [synth.segment_mix(rng, n) for n in sizes]
+    bufs += [np.zeros(20000, np.uint8), np.full(20000, 7, np.uint8),
+             np.tile(np.arange(64, dtype=np.uint8), 400), np.tile(np.arange(16, dtype=np.uint8), 2000)]
+    s = eng.new_set()
+    assert np.all(s.digest(bufs) == 0)
+    for i, b in enumerate(bufs):
+        same(s, i, oracle.digest_stream(b), "stream %d (%d bytes)" % (i, b.size))
+    st = eng.new_set().digest([np.zeros(100, np.uint8), np.ones(512, np.uint8)])
+    assert list(st) == [-3, 0]
+    print("stream ok", flush=True)
+
+    # compare: all pairs of mixed-mode digests, orientation by name
+    files = synth.file_set(7, 8, 40 * KIB)
+    names = [b"z", b"a", b"m", b"b", b"y", b"c", b"x", b"d"]
+    modes = [0, 16384, 0, 0, 16384, 0, 0, 16384]
+    s = eng.new_set()
+    assert np.all(s.digest(files, names, dd_block=modes) == 0)
+    ods = [oracle.digest_block(f, m, n) if m else oracle.digest_stream(f, n)
+           for f, n, m in zip(files, names, modes)]
+    for i, od in enumerate(ods):
+        same(s, i, od, "set %d" % i)
+    assert np.array_equal(eng.compare_all(s), oracle.compare_all(ods))
+    # compare_to with a target digest of more than 1024 filters (chunked)
+    big = synth.disk_image(8, 1100 * 16 * KIB)
+    t = [oracle.digest_block(big, 16384, b"big")]
+    q = [oracle.digest_stream(big[o:o + 20000], b"q%d" % k) for k, o in enumerate((0, 5000000, 17000000))]
+    q.append(oracle.digest_stream(files[0], b"other"))
+    gt, gq = eng.new_set(), eng.new_set()
+    for d in t:
+        gt.add(d.name, d.orig_size, d.bf_count, d.last_count, d.dd_block_size, d.max_elem, d.filters,
+               d.elem_counts)
+    for d in q:
+        gq.add(d.name, d.orig_size, d.bf_count, d.last_count, d.dd_block_size, d.max_elem, d.filters,
+               d.elem_counts)
+    want = oracle.compare_to(q, t)
+    assert np.array_equal(eng.compare_to(gq, gt), want), (eng.compare_to(gq, gt), want)
+    assert np.array_equal(eng.compare_to(gt, gq), oracle.compare_to(t, q))
+    assert (want > 0).sum() >= 3
+    print("compare ok", flush=True)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main(sys.argv[1])

@@ -1,0 +1,369 @@
+// CPU emulation of the subset of HIP used by sdhash_amd/csrc, so that the
+// unmodified kernels can run on the host under AddressSanitizer: device
+// memory is host memory, each workgroup runs as blockDim.x cooperative
+// fibers (x86-64 only), and wave operations (shuffles, ballots, readlane) exchange values
+// through per-wave slots between two wave barriers.  Workgroups of a launch
+// run one after another.  Test infrastructure only.
+#ifndef HIPEMU_HIP_RUNTIME_H
+#define HIPEMU_HIP_RUNTIME_H
+
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <barrier>
+#include <cmath>
+#include <functional>
+#include <memory>
+#include <thread>
+#include <vector>
+
+#if defined(__SANITIZE_ADDRESS__)
+#include <sanitizer/common_interface_defs.h>
+#endif
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __shared__ static
+#define __constant__
+#define __launch_bounds__(...)
+#define __align__(n) alignas(n)
+
+struct dim3 {
+    uint32_t x, y, z;
+    dim3(uint32_t a = 1, uint32_t b = 1, uint32_t c = 1) : x(a), y(b), z(c) {}
+};
+struct uint4 {
+    uint32_t x, y, z, w;
+};
+
+// Minimal x86-64 context switch: saves the callee-saved registers on the
+// current stack, stores its pointer in *save, and resumes the stack `next`.
+extern "C" void hipemu_switch(void** save, void* next);
+asm(R"(
+    .text
+    .globl hipemu_switch
+    .type hipemu_switch,@function
+hipemu_switch:
+    pushq %rbp
+    pushq %rbx
+    pushq %r12
+    pushq %r13
+    pushq %r14
+    pushq %r15
+    movq %rsp, (%rdi)
+    movq %rsi, %rsp
+    popq %r15
+    popq %r14
+    popq %r13
+    popq %r12
+    popq %rbx
+    popq %rbp
+    ret
+    .size hipemu_switch, .-hipemu_switch
+)");
+
+namespace hipemu {
+
+// Each workgroup runs as blockDim.x cooperative fibers on one host thread.
+// A fiber gives up the CPU at every barrier and wave operation; the scheduler
+// resumes only fibers whose barrier has opened, so the lanes of a wave stay
+// in lock step as far as the kernels rely on it.
+constexpr int kWave = 64;
+constexpr size_t kStack = 256 * 1024;
+
+struct Fiber {
+    void* sp = nullptr;
+    std::vector<char> stack;
+    bool done = false;
+    int wait = 0;          // 0 runnable, 1 block barrier, 2 wave barrier
+    uint64_t gen = 0;      // generation it waits to see pass
+};
+
+struct Block {
+    uint32_t nthreads = 0;
+    std::vector<Fiber> fib;
+    void* sched_sp = nullptr;
+    uint32_t cur = 0;
+    uint32_t bar_count = 0;
+    uint64_t bar_gen = 0;
+    std::vector<uint32_t> wcount;
+    std::vector<uint64_t> wgen;
+    std::vector<uint64_t> slot;
+    int orv = 0;
+};
+
+inline thread_local dim3 tl_threadIdx, tl_blockIdx, tl_blockDim, tl_gridDim;
+inline thread_local Block* tl_block = nullptr;
+inline thread_local std::function<void()>* tl_body = nullptr;
+inline thread_local const void* tl_sched_bottom = nullptr;
+inline thread_local size_t tl_sched_size = 0;
+
+inline void yield()
+{
+    Block* b = tl_block;
+    Fiber& f = b->fib[b->cur];
+#if defined(__SANITIZE_ADDRESS__)
+    void* fs = nullptr;
+    __sanitizer_start_switch_fiber(&fs, tl_sched_bottom, tl_sched_size);
+    hipemu_switch(&f.sp, b->sched_sp);
+    __sanitizer_finish_switch_fiber(fs, nullptr, nullptr);
+#else
+    hipemu_switch(&f.sp, b->sched_sp);
+#endif
+}
+
+inline int lane() { return (int)(tl_threadIdx.x % kWave); }
+inline int wave() { return (int)(tl_threadIdx.x / kWave); }
+inline int wave_size()
+{
+    const int w = wave();
+    const int n = (int)tl_block->nthreads - w * kWave;
+    return n < kWave ? n : kWave;
+}
+
+inline void block_sync()
+{
+    Block* b = tl_block;
+    const uint64_t g = b->bar_gen;
+    if (++b->bar_count == b->nthreads) {
+        b->bar_count = 0;
+        b->bar_gen++;
+        return;
+    }
+    Fiber& f = b->fib[b->cur];
+    f.wait = 1;
+    f.gen = g;
+    yield();
+}
+
+inline void wave_sync()
+{
+    Block* b = tl_block;
+    const int w = wave();
+    const uint64_t g = b->wgen[w];
+    if (++b->wcount[w] == (uint32_t)wave_size()) {
+        b->wcount[w] = 0;
+        b->wgen[w]++;
+        return;
+    }
+    Fiber& f = b->fib[b->cur];
+    f.wait = 2;
+    f.gen = g;
+    yield();
+}
+
+// every lane of the wave publishes v; returns the value of lane src
+inline uint64_t exchange(uint64_t v, int src)
+{
+    Block* b = tl_block;
+    const int base = wave() * kWave;
+    b->slot[tl_threadIdx.x] = v;
+    wave_sync();
+    const uint64_t r = (src >= 0 && src < wave_size()) ? b->slot[base + src] : v;
+    wave_sync();
+    return r;
+}
+
+inline uint64_t ballot(bool p)
+{
+    Block* b = tl_block;
+    const int base = wave() * kWave;
+    b->slot[tl_threadIdx.x] = p ? 1 : 0;
+    wave_sync();
+    uint64_t m = 0;
+    for (int l = 0; l < wave_size(); l++)
+        if (b->slot[base + l])
+            m |= 1ull << l;
+    wave_sync();
+    return m;
+}
+
+[[noreturn]] inline void fiber_entry()
+{
+#if defined(__SANITIZE_ADDRESS__)
+    __sanitizer_finish_switch_fiber(nullptr, &tl_sched_bottom, &tl_sched_size);
+#endif
+    (*tl_body)();
+    Block* b = tl_block;
+    Fiber& f = b->fib[b->cur];
+    f.done = true;
+#if defined(__SANITIZE_ADDRESS__)
+    __sanitizer_start_switch_fiber(nullptr, tl_sched_bottom, tl_sched_size);
+#endif
+    void* dummy;
+    hipemu_switch(&dummy, b->sched_sp);
+    __builtin_unreachable();
+}
+
+template <typename F>
+void launch(const dim3& grid, const dim3& block, F&& body)
+{
+    const uint32_t nt = block.x;
+    std::function<void()> fn = body;
+    for (uint32_t bx = 0; bx < grid.x; bx++) {
+        Block b;
+        b.nthreads = nt;
+        b.fib.resize(nt);
+        const uint32_t nw = (nt + kWave - 1) / kWave;
+        b.wcount.assign(nw, 0);
+        b.wgen.assign(nw, 0);
+        b.slot.assign(nt, 0);
+        tl_block = &b;
+        tl_body = &fn;
+        tl_blockIdx = dim3(bx);
+        tl_blockDim = block;
+        tl_gridDim = grid;
+        for (uint32_t t = 0; t < nt; t++) {
+            Fiber& f = b.fib[t];
+            f.stack.resize(kStack);
+            uintptr_t top = ((uintptr_t)f.stack.data() + kStack) & ~(uintptr_t)15;
+            void** sp = (void**)top;
+            *--sp = nullptr;                   // fake return address of the entry
+            *--sp = (void*)&fiber_entry;       // popped by `ret`
+            for (int r = 0; r < 6; r++)
+                *--sp = nullptr;               // rbp rbx r12 r13 r14 r15
+            f.sp = sp;
+        }
+        uint32_t live = nt;
+        while (live) {
+            bool ran = false;
+            for (uint32_t t = 0; t < nt; t++) {
+                Fiber& f = b.fib[t];
+                if (f.done)
+                    continue;
+                if (f.wait == 1 && b.bar_gen == f.gen)
+                    continue;
+                if (f.wait == 2 && b.wgen[t / kWave] == f.gen)
+                    continue;
+                f.wait = 0;
+                b.cur = t;
+                tl_threadIdx = dim3(t);
+                ran = true;
+#if defined(__SANITIZE_ADDRESS__)
+                void* fs = nullptr;
+                __sanitizer_start_switch_fiber(&fs, f.stack.data(), kStack);
+                hipemu_switch(&b.sched_sp, f.sp);
+                __sanitizer_finish_switch_fiber(fs, nullptr, nullptr);
+#else
+                hipemu_switch(&b.sched_sp, f.sp);
+#endif
+                if (f.done)
+                    live--;
+            }
+            if (!ran) {
+                fprintf(stderr, "hipemu: deadlock in workgroup %u (fibers wait on barriers that cannot open)\n", bx);
+                abort();
+            }
+        }
+        tl_block = nullptr;
+    }
+}
+
+}  // namespace hipemu
+
+#define threadIdx (hipemu::tl_threadIdx)
+#define blockIdx (hipemu::tl_blockIdx)
+#define blockDim (hipemu::tl_blockDim)
+#define gridDim (hipemu::tl_gridDim)
+
+inline void __syncthreads() { hipemu::block_sync(); }
+inline int __syncthreads_or(int p)
+{
+    hipemu::Block* b = hipemu::tl_block;
+    hipemu::block_sync();
+    if (threadIdx.x == 0)
+        b->orv = 0;
+    hipemu::block_sync();
+    if (p)
+        b->orv = 1;
+    hipemu::block_sync();
+    const int r = b->orv;
+    hipemu::block_sync();
+    return r;
+}
+inline void __threadfence() {}
+inline void __threadfence_block() {}
+inline void __builtin_amdgcn_wave_barrier() { hipemu::wave_sync(); }
+
+template <typename T>
+inline T __shfl_down(T v, unsigned off)
+{
+    return (T)hipemu::exchange((uint64_t)v, hipemu::lane() + (int)off);
+}
+template <typename T>
+inline T __shfl_up(T v, unsigned off)
+{
+    return (T)hipemu::exchange((uint64_t)v, hipemu::lane() - (int)off);
+}
+template <typename T>
+inline T __shfl(T v, int src)
+{
+    return (T)hipemu::exchange((uint64_t)v, src);
+}
+inline unsigned long long __ballot(int p) { return hipemu::ballot(p != 0); }
+inline int __builtin_amdgcn_readlane(int v, int l) { return (int)hipemu::exchange((uint32_t)v, l); }
+inline uint32_t __builtin_amdgcn_alignbyte(uint32_t hi, uint32_t lo, uint32_t s)
+{
+    return (uint32_t)((((uint64_t)hi << 32) | lo) >> (8 * (s & 3)));
+}
+inline int __popc(uint32_t x) { return __builtin_popcount(x); }
+
+inline uint32_t atomicAdd(uint32_t* p, uint32_t v) { const uint32_t o = *p; *p = o + v; return o; }
+inline uint32_t atomicOr(uint32_t* p, uint32_t v) { const uint32_t o = *p; *p = o | v; return o; }
+
+inline uint32_t min(uint32_t a, uint32_t b) { return a < b ? a : b; }
+inline uint64_t min(uint64_t a, uint64_t b) { return a < b ? a : b; }
+inline int min(int a, int b) { return a < b ? a : b; }
+
+// ------------------------------------------------------------- runtime API
+
+typedef int hipError_t;
+enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorOutOfMemory = 2 };
+typedef void* hipStream_t;
+typedef void* hipEvent_t;
+enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice };
+#define hipStreamNonBlocking 1
+
+inline const char* hipGetErrorString(hipError_t e) { return e ? "emulated HIP error" : "no error"; }
+inline hipError_t hipGetLastError() { return hipSuccess; }
+inline hipError_t hipGetDeviceCount(int* n) { *n = 1; return hipSuccess; }
+inline hipError_t hipSetDevice(int) { return hipSuccess; }
+inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
+inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = (void*)1; return hipSuccess; }
+inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipMalloc(void** p, size_t n)
+{
+    *p = malloc(n ? n : 1);
+    return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+template <typename T>
+inline hipError_t hipMalloc(T** p, size_t n) { return hipMalloc((void**)p, n); }
+inline hipError_t hipFree(void* p) { free(p); return hipSuccess; }
+inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t)
+{
+    memmove(d, s, n);
+    return hipSuccess;
+}
+inline hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) { memset(d, v, n); return hipSuccess; }
+inline hipError_t hipMemset(void* d, int v, size_t n) { memset(d, v, n); return hipSuccess; }
+inline hipError_t hipEventCreate(hipEvent_t* e) { *e = (void*)new int(0); return hipSuccess; }
+inline hipError_t hipEventDestroy(hipEvent_t e) { delete (int*)e; return hipSuccess; }
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) { *ms = 0.f; return hipSuccess; }
+
+#define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...)                          \
+    do {                                                                                     \
+        if (getenv("HIPEMU_TRACE"))                                                          \
+            fprintf(stderr, "[hipemu] %s grid %u block %u\n", #kernel, (unsigned)dim3(grid).x, \
+                    (unsigned)dim3(block).x);                                                \
+        hipemu::launch((grid), (block), [&]() { kernel(__VA_ARGS__); });                     \
+    } while (0)
+
+#endif
