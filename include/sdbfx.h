@@ -60,7 +60,8 @@ uint64_t sdbfx_set_filter_count(const sdbfx_set *set);
 /*
  * Digests n host buffers and appends the digests, in order, to `set`.
  * dd_block[i] == 0 selects stream mode, else block mode with that dd-block
- * size in bytes (<= 16384); dd_block == NULL means stream mode for all.
+ * size in bytes (a multiple of 16 in [512, 16384]; sdhash's -b N gives
+ * N KiB); dd_block == NULL means stream mode for all.
  * status[i] receives 0 or -3 (too small: nothing appended for that input).
  * Replaces sdbf::sdbf(name, char*, dd, len, NULL) (sdbf/sdbf_class.cc:115-136)
  * and sdbf::sdbf(name, istream*, dd, size, NULL) (sdbf/sdbf_class.cc:71-102)

@@ -368,6 +368,8 @@ int digest_common(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t n, const uint8_t* d_b
             return fail(ctx, "dd block size above 16384");
         if (dd && dd[i] && dd[i] < SDBFX_MIN_FILE)
             return fail(ctx, "dd block size below 512");
+        if (dd && dd[i] % 16)
+            return fail(ctx, "dd block size must be a multiple of 16");
         status[i] = lens[i] < SDBFX_MIN_FILE ? -3 : 0;
     }
     uint32_t i0 = 0;

@@ -59,7 +59,7 @@ struct __align__(16) TileSmem {
         uint8_t hist[RANK_LANES * 256];     // ranks phase: one byte histogram per lane
         struct {
             uint32_t score[SDBFX_TILE / 4]; // owned positions' scores, 4 per word
-            uint16_t cand[BCAND_CAP];       // candidate positions (relative to t0)
+            uint32_t cand[BCAND_CAP];       // candidates: position (relative to t0) | score << 16
             uint32_t hsha[5][SDBFX_NT];     // block mode: SHA-1 words of a batch
             uint32_t csc[SDBFX_NT];         // block mode: scores of a batch
         } s;
@@ -69,7 +69,9 @@ struct __align__(16) TileSmem {
     uint32_t start_m[SDBFX_NT];
     uint32_t end_m[SDBFX_NT];
     uint32_t histo[68];
-    uint32_t wsum[NWAVE + 4];
+    uint32_t filt[64];                      // block mode: the filter being built
+    uint32_t chunk_cnt[64];
+    uint32_t chunk_off[64];
     uint32_t misc[16];
 };
 
@@ -183,53 +185,122 @@ __device__ __forceinline__ uint32_t score_at(const TileSmem& sm, uint32_t rel)
     return ((const uint8_t*)sm.u.s.score)[rel];
 }
 
+// Wave-wide scans with DPP row shifts (16-lane rows) and row read-lanes.
+#define DPP_MIN(v, ctrl) v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)0xFFFFFFFFu, (int)(v), ctrl, 0xF, 0xF, false))
+#define DPP_ADD(v, ctrl) v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(v), ctrl, 0xF, 0xF, false)
+
+// inclusive minimum over lanes [0, lane]
+__device__ __forceinline__ uint32_t wave_prefix_min(uint32_t v, int lane)
+{
+    DPP_MIN(v, 0x111);
+    DPP_MIN(v, 0x112);
+    DPP_MIN(v, 0x114);
+    DPP_MIN(v, 0x118);
+    const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 15);
+    const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 31);
+    const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 47);
+    const uint32_t p2 = min(r0, r1), p3 = min(p2, r2);
+    const int row = lane >> 4;
+    const uint32_t add = row == 0 ? 0xFFFFFFFFu : row == 1 ? r0 : row == 2 ? p2 : p3;
+    return min(v, add);
+}
+
+// inclusive minimum over lanes [lane, 63]
+__device__ __forceinline__ uint32_t wave_suffix_min(uint32_t v, int lane)
+{
+    DPP_MIN(v, 0x101);
+    DPP_MIN(v, 0x102);
+    DPP_MIN(v, 0x104);
+    DPP_MIN(v, 0x108);
+    const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
+    const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 32);
+    const uint32_t r3 = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+    const uint32_t s1 = min(r2, r3), s0 = min(r1, s1);
+    const int row = lane >> 4;
+    const uint32_t add = row == 3 ? 0xFFFFFFFFu : row == 2 ? r3 : row == 1 ? s1 : s0;
+    return min(v, add);
+}
+
+// exclusive sum over lanes [0, lane); *total gets the wave's sum
+__device__ __forceinline__ uint32_t wave_excl_sum(uint32_t v, int lane, uint32_t* total)
+{
+    uint32_t x = v;
+    DPP_ADD(x, 0x111);
+    DPP_ADD(x, 0x112);
+    DPP_ADD(x, 0x114);
+    DPP_ADD(x, 0x118);
+    const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)x, 15);
+    const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)x, 31);
+    const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)x, 47);
+    const uint32_t r3 = (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+    const int row = lane >> 4;
+    const uint32_t add = row == 0 ? 0u : row == 1 ? r0 : row == 2 ? r0 + r1 : r0 + r1 + r2;
+    *total = r0 + r1 + r2 + r3;
+    return x + add - v;
+}
+
 // Runs the score state machine over steps [i0, i1) from state m.  With CREDIT
 // each step adds SIGN to the score of the position it elects, when that
 // position is owned ([t0, t0+T)).  When step hand_i is run, the canonical state
 // before step hand_i+1 is stored to *hand.  Returns the state after i1-1.
+// The words a step reads do not depend on the state, so they are loaded four
+// steps at a time ahead of the (branch-free) transitions.
 template <int SIGN, bool CREDIT>
 __device__ uint32_t sm_run(TileSmem& sm, uint32_t base, uint32_t t0, uint32_t T, uint32_t i0,
                            uint32_t i1, uint32_t m, uint32_t hand_i, uint32_t* hand)
 {
     const uint16_t* P = sm.P;
     uint32_t rm = m ? rank_at(P, base, m) : 0u;
-    for (uint32_t i = i0; i < i1; i++) {
-        const uint32_t r64 = P[i + 64 - base] & 0x3FFu;
-        uint32_t cpos;
-        bool has;
-        if (i > 0 && rm > 0 && r64 >= rm && i < m) {
-            // slide on the cheap (sdbf_core.cc:61-68)
-            if (r64 == rm)
-                m = i + 64;
-            cpos = m;
-            has = true;
-        } else {
-            // full rescan of the window (sdbf_core.cc:69-81)
-            const uint32_t w = P[i - base];
-            const uint32_t f = i + (w >> 10);
-            const uint32_t rf = P[f - base] & 0x3FFu;
-            m = f;
-            rm = rf;
-            cpos = f;
-            has = rf != 0;
+    for (uint32_t i = i0; i < i1; i += 4) {
+        uint32_t w[4], r64[4], rf[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t ii = min(i + j, i1 - 1);
+            w[j] = P[ii - base];
+            r64[j] = P[ii + 64 - base] & 0x3FFu;
         }
-        if (CREDIT && has) {
-            const uint32_t rel = cpos - t0;
-            if (cpos >= t0 && rel < T)
-                atomicAdd(&sm.u.s.score[rel >> 2], ((uint32_t)SIGN) << (8 * (rel & 3)));
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t ii = min(i + j, i1 - 1);
+            rf[j] = P[ii + (w[j] >> 10) - base] & 0x3FFu;
         }
-        if (hand && i == hand_i)
-            *hand = canon(P, base, i + 1, m);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t ii = i + j;
+            if (ii < i1) {
+                // cheap slide (sdbf_core.cc:61-68) or full rescan (:69-81)
+                const bool cheap = ii > 0 && rm > 0 && r64[j] >= rm && ii < m;
+                const uint32_t mc = r64[j] == rm ? ii + 64 : m;
+                const uint32_t f = ii + (w[j] >> 10);
+                m = cheap ? mc : f;
+                const bool has = cheap || rf[j] != 0;
+                rm = cheap ? rm : rf[j];
+                if (CREDIT && has) {
+                    const uint32_t rel = m - t0;
+                    if (m >= t0 && rel < T)
+                        atomicAdd(&sm.u.s.score[rel >> 2], ((uint32_t)SIGN) << (8 * (rel & 3)));
+                }
+                if (hand && ii == hand_i)
+                    *hand = canon(P, base, ii + 1, m);
+            }
+        }
     }
     return m;
+}
+
+__device__ __forceinline__ uint32_t byte_of(const uint4& v, int j)
+{
+    const uint32_t w = (j < 4) ? v.x : (j < 8) ? v.y : (j < 12) ? v.z : v.w;
+    return (w >> (8 * (j & 3))) & 0xFFu;
 }
 
 // ------------------------------------------------------------------ tile core
 
 // Computes ranks, F and the scores of owned positions [t0, t0+T) of a unit of
-// n bytes at u (n > 64, t0 + T <= n - 64, t0 == 0 or t0 >= 16384).  forced >= 0
-// gives the exact canonical state before step t0-64.  Returns 1 when every
-// score is exact (certified), 0 when the incoming state could not be proven.
+// n bytes at u (n > 64, t0 + T <= n - 64, t0 == 0 or t0 >= 16384; u is 16-byte
+// aligned and 16 bytes past n are readable).  forced >= 0 gives the exact
+// canonical state before step t0-64.  Returns 1 when every score is exact
+// (certified), 0 when the incoming state could not be proven.
 __device__ uint32_t tile_core(TileSmem& sm, const uint8_t* __restrict__ u, uint32_t n, uint32_t t0,
                               uint32_t T, int64_t forced, uint32_t hand_i, uint32_t* hand_out)
 {
@@ -249,44 +320,65 @@ __device__ uint32_t tile_core(TileSmem& sm, const uint8_t* __restrict__ u, uint3
         sm.dtab[tid] = tid ? (int32_t)c_entropy[tid] - (int32_t)c_entropy[tid - 1] : 0;
     __syncthreads();
 
-    // ---- 1. ranks
+    // ---- 1. ranks: each lane rolls a byte histogram over a run of windows
+    // that starts at a multiple of 16; input bytes arrive 16 at a time, one
+    // chunk ahead of their use.
     if (tid < RANK_LANES) {
-        const uint32_t PR = (L + RANK_LANES - 1) / RANK_LANES;
-        const uint32_t r0 = tid * PR;
+        const uint32_t PR = ((L + RANK_LANES - 1) / RANK_LANES + 15) & ~15u;
+        const uint32_t r0 = min(tid * PR, L);
         const uint32_t r1 = min(r0 + PR, L);
         uint8_t* H = sm.u.hist + tid * 256;
         const uint32_t swz = (uint32_t)(tid & 31) << 2;
         uint32_t* H32 = (uint32_t*)H;
         for (int q = 0; q < 64; q++)
             H32[q] = 0;
-        int32_t e = 0;
-        bool init = false;
-        for (uint32_t rp = r0; rp < r1; rp++) {
-            const uint32_t p = base + rp;
-            uint16_t rank = 0;
-            if (p < lim) {
-                if (!init) {
-                    for (int k = 0; k < 64; k++) {
-                        const uint32_t b = u[p + k];
-                        const uint32_t c = ++H[b ^ swz];
-                        e += sm.dtab[c];
-                    }
-                    init = true;
-                } else {
-                    const uint32_t bo = u[p - 1];
-                    const uint32_t bi = u[p + 63];
-                    if (bo != bi) {
-                        const uint32_t co = H[bo ^ swz];
-                        const uint32_t ci = H[bi ^ swz];
-                        H[bo ^ swz] = (uint8_t)(co - 1);
-                        H[bi ^ swz] = (uint8_t)(ci + 1);
-                        e += sm.dtab[ci + 1] - sm.dtab[co];
+        const uint32_t p0 = base + r0;
+        const uint32_t pv = max(p0, min(base + r1, lim));   // windows exist for [p0, pv)
+        if (p0 < pv) {
+            int32_t e = 0;
+            const uint4* q0 = (const uint4*)(u + p0);
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const uint4 v = q0[c];
+#pragma unroll
+                for (int j = 0; j < 16; j++) {
+                    const uint32_t cnt = ++H[byte_of(v, j) ^ swz];
+                    e += sm.dtab[cnt];
+                }
+            }
+            sm.P[r0] = sm.rtab[e >> 10];
+            uint4 O = q0[0], I = q0[4];
+            uint32_t oc = 0, ic = 0;
+            for (uint32_t cs = p0; cs < pv; cs += 16) {
+                uint4 On = O, In = I;
+                if (cs + 16 < pv) {
+                    On = *(const uint4*)(u + cs + 16);
+                    In = *(const uint4*)(u + cs + 80);
+                }
+#pragma unroll
+                for (int j = 0; j < 16; j++) {
+                    const uint32_t p = cs + j;
+                    if ((j > 0 || cs > p0) && p < pv) {
+                        const uint32_t bo = j ? byte_of(O, j - 1) : oc;
+                        const uint32_t bi = j ? byte_of(I, j - 1) : ic;
+                        if (bo != bi) {
+                            const uint32_t co = H[bo ^ swz];
+                            const uint32_t ci = H[bi ^ swz];
+                            H[bo ^ swz] = (uint8_t)(co - 1);
+                            H[bi ^ swz] = (uint8_t)(ci + 1);
+                            e += sm.dtab[ci + 1] - sm.dtab[co];
+                        }
+                        sm.P[p - base] = sm.rtab[e >> 10];
                     }
                 }
-                rank = sm.rtab[e >> 10];
+                oc = byte_of(O, 15);
+                ic = byte_of(I, 15);
+                O = On;
+                I = In;
             }
-            sm.P[rp] = rank;
         }
+        for (uint32_t p = max(pv, p0); p < base + r1; p++)
+            sm.P[p - base] = 0;
     }
     __syncthreads();
 
@@ -302,23 +394,9 @@ __device__ uint32_t tile_core(TileSmem& sm, const uint8_t* __restrict__ u, uint3
         const uint32_t rpb = (pb - 1 < L) ? (sm.P[pb - 1] & 0x3FFu) : 0xFFFFu;
         const uint32_t ka = ra ? ((ra << 16) | pa) : 0xFFFFFFFFu;
         const uint32_t kb = rb ? ((rb << 16) | pb) : 0xFFFFFFFFu;
-        uint32_t sa = ka;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t o = __shfl_down(sa, off);
-            if (lane + off < 64)
-                sa = min(sa, o);
-        }
-        uint32_t pm = kb;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t o = __shfl_up(pm, off);
-            if (lane >= off)
-                pm = min(pm, o);
-        }
-        uint32_t pm1 = __shfl_up(pm, 1);
-        if (lane == 0)
-            pm1 = 0xFFFFFFFFu;
+        const uint32_t sa = wave_suffix_min(ka, lane);
+        const uint32_t pm = wave_prefix_min(kb, lane);
+        const uint32_t pm1 = (uint32_t)__builtin_amdgcn_update_dpp((int)0xFFFFFFFFu, (int)pm, 0x138, 0xF, 0xF, false);
         const uint32_t key = min(sa, pm1);
         const uint64_t da = __ballot(pa < L && ra != rpa);
         const uint64_t db = __ballot(pb < L && rb != rpb);
@@ -354,7 +432,7 @@ __device__ uint32_t tile_core(TileSmem& sm, const uint8_t* __restrict__ u, uint3
                 m = 0;
             m = sm_run<0, false>(sm, base, t0, T, base, a, m, 0xFFFFFFFFu, nullptr);
             const uint32_t c = canon(sm.P, base, a, m);
-            const uint32_t c0 = __shfl(c, 0);
+            const uint32_t c0 = (uint32_t)__builtin_amdgcn_readlane((int)c, 0);
             cert = __ballot(c != c0) == 0 ? 1u : 0u;
             st = c0;
         }
@@ -399,50 +477,55 @@ __device__ uint32_t tile_core(TileSmem& sm, const uint8_t* __restrict__ u, uint3
     return sm.misc[1];
 }
 
-// exclusive block-wide prefix sum of v over the 256 threads; *total gets the sum
-__device__ __forceinline__ uint32_t block_scan(TileSmem& sm, uint32_t v, uint32_t* total)
-{
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t o = __shfl_up(x, off);
-        if (lane >= off)
-            x += o;
-    }
-    if (lane == 63)
-        sm.wsum[wave] = x;
-    __syncthreads();
-    uint32_t before = 0, tot = 0;
-    for (int w = 0; w < NWAVE; w++) {
-        const uint32_t ws = sm.wsum[w];
-        if (w < wave)
-            before += ws;
-        tot += ws;
-    }
-    __syncthreads();
-    *total = tot;
-    return before + x - v;
-}
-
-// Ordered compaction of owned positions [0, cnt) satisfying pred(score) into
-// sm.u.s.cand.  Returns the number of candidates.
+// Ordered compaction of owned positions [0, cnt) whose score satisfies pred
+// into sm.u.s.cand as (position | score << 16).  256 positions per chunk,
+// four per lane, read one score word at a time.  Returns the count.
 template <typename Pred>
 __device__ uint32_t compact(TileSmem& sm, uint32_t cnt, Pred pred)
 {
-    const int tid = threadIdx.x;
-    const uint32_t CH = (cnt + SDBFX_NT - 1) / SDBFX_NT;
-    const uint32_t q0 = tid * CH, q1 = min(q0 + CH, cnt);
-    uint32_t c = 0;
-    for (uint32_t q = q0; q < q1; q++)
-        c += pred(score_at(sm, q)) ? 1u : 0u;
-    uint32_t total;
-    uint32_t o = block_scan(sm, c, &total);
-    for (uint32_t q = q0; q < q1; q++)
-        if (pred(score_at(sm, q)))
-            sm.u.s.cand[o++] = (uint16_t)q;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t nchunk = (cnt + 255) / 256;
+    for (uint32_t c = wave; c < nchunk; c += NWAVE) {
+        const uint32_t q0 = c * 256 + lane * 4;
+        const uint32_t word = q0 < cnt ? sm.u.s.score[q0 >> 2] : 0u;
+        uint32_t k = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            k += (q0 + j < cnt && pred((word >> (8 * j)) & 0xFFu)) ? 1u : 0u;
+        uint32_t tot;
+        wave_excl_sum(k, lane, &tot);
+        if (lane == 0)
+            sm.chunk_cnt[c] = tot;
+    }
     __syncthreads();
-    return total;
+    if (wave == 0) {
+        const uint32_t v = (uint32_t)lane < nchunk ? sm.chunk_cnt[lane] : 0u;
+        uint32_t tot;
+        const uint32_t ex = wave_excl_sum(v, lane, &tot);
+        if ((uint32_t)lane < nchunk)
+            sm.chunk_off[lane] = ex;
+        if (lane == 0)
+            sm.misc[6] = tot;
+    }
+    __syncthreads();
+    for (uint32_t c = wave; c < nchunk; c += NWAVE) {
+        const uint32_t q0 = c * 256 + lane * 4;
+        const uint32_t word = q0 < cnt ? sm.u.s.score[q0 >> 2] : 0u;
+        uint32_t k = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            k += (q0 + j < cnt && pred((word >> (8 * j)) & 0xFFu)) ? 1u : 0u;
+        uint32_t tot;
+        uint32_t o = sm.chunk_off[c] + wave_excl_sum(k, lane, &tot);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t sc = (word >> (8 * j)) & 0xFFu;
+            if (q0 + j < cnt && pred(sc))
+                sm.u.s.cand[o++] = (q0 + j) | (sc << 16);
+        }
+    }
+    __syncthreads();
+    return sm.misc[6];
 }
 
 // ------------------------------------------------------------------ block mode
@@ -465,10 +548,16 @@ __global__ __launch_bounds__(SDBFX_NT) void block_digest_kernel(
     if (tid < 68)
         sm.histo[tid] = 0;
     __syncthreads();
-    for (uint32_t q = tid; q < T; q += SDBFX_NT) {
-        const uint32_t sc = score_at(sm, q);
-        if (sc >= SDBFX_THRESHOLD)
-            atomicAdd(&sm.histo[sc], 1u);
+    for (uint32_t w = tid; w * 4 < T; w += SDBFX_NT) {
+        const uint32_t word = sm.u.s.score[w];
+        if (word & 0xF0F0F0F0u) {   // some byte >= 16
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t sc = (word >> (8 * j)) & 0xFFu;
+                if (w * 4 + j < T && sc >= SDBFX_THRESHOLD)
+                    atomicAdd(&sm.histo[sc], 1u);
+            }
+        }
     }
     __syncthreads();
     if (tid == 0) {
@@ -484,70 +573,111 @@ __global__ __launch_bounds__(SDBFX_NT) void block_digest_kernel(
         }
         sm.misc[3] = k;
         sm.misc[4] = (uint32_t)allowed;
-        sm.misc[5] = 0;
     }
     __syncthreads();
     const uint32_t k = sm.misc[3];
     int32_t allowed = (int32_t)sm.misc[4];
     const uint32_t C = compact(sm, T, [k](uint32_t sc) { return sc >= k; });
 
-    uint32_t fword = 0, hash_cnt = 0;
-    for (uint32_t c0 = 0; c0 < C; c0 += SDBFX_NT) {
-        const uint32_t c = c0 + tid;
-        if (c < C) {
-            const uint32_t pos = sm.u.s.cand[c];
+    // Ordered insertion (gen_block_hash, sdbf_core.cc:157-183), 64 candidates
+    // per step on wave 0.  Each bit of a batch is owned by the first selected
+    // lane that sets it (LDS minimum); a lane counts when it owns a bit that
+    // was clear.  Lanes after the first one whose selection was speculative
+    // and wrong (allowance used up) or after the 192nd element are re-run.
+    uint32_t* own = sm.u.s.score;   // 2048 entries; the score words are dead now
+    if (tid < 64)
+        sm.filt[tid] = 0;
+    uint32_t hash_cnt = 0, next = 0;
+    bool done = false;
+    __syncthreads();
+    while (!done && next < C) {
+        const uint32_t wcount = min((uint32_t)SDBFX_NT, C - next);
+        if ((uint32_t)tid < wcount) {
+            const uint32_t ce = sm.u.s.cand[next + tid];
             uint32_t h[5];
-            sha1_64(u + pos, h);
+            sha1_64(u + (ce & 0xFFFFu), h);
 #pragma unroll
             for (int j = 0; j < 5; j++)
                 sm.u.s.hsha[j][tid] = h[j];
-            sm.u.s.csc[tid] = score_at(sm, pos);
+            sm.u.s.csc[tid] = ce >> 16;
         }
         __syncthreads();
         if (wave == 0) {
-            bool done = false;
-            for (uint32_t sub = 0; sub < SDBFX_NT && c0 + sub < C && !done; sub += 64) {
-                const uint32_t cnt = min(64u, C - c0 - sub);
-                uint32_t myh[5];
+            uint32_t wpos = 0;
+            while (wpos < wcount && !done) {
+                const uint32_t li = wpos + lane;
+                const bool act = li < wcount;
+                const uint32_t sc = act ? sm.u.s.csc[li] : 0u;
+                uint32_t b[5];
 #pragma unroll
                 for (int j = 0; j < 5; j++)
-                    myh[j] = sm.u.s.hsha[j][sub + lane];
-                const uint32_t mysc = sm.u.s.csc[sub + lane];
-                for (uint32_t cc = 0; cc < cnt; cc++) {
-                    const uint32_t sc = (uint32_t)__builtin_amdgcn_readlane((int)mysc, (int)cc);
-                    if (!(sc > k || (sc == k && allowed > 0)))
-                        continue;
-                    uint32_t nb = 0;
+                    b[j] = act ? (sm.u.s.hsha[j][li] & 0x7FFu) : 0u;
+                const bool sel = act && (sc > k || (sc == k && allowed > 0));
+                if (sel) {
+#pragma unroll
+                    for (int j = 0; j < 5; j++)
+                        own[b[j]] = 64;
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (sel) {
+#pragma unroll
+                    for (int j = 0; j < 5; j++)
+                        atomicMin(&own[b[j]], (uint32_t)lane);
+                }
+                __builtin_amdgcn_wave_barrier();
+                bool nw = false;
+                if (sel) {
 #pragma unroll
                     for (int j = 0; j < 5; j++) {
-                        const uint32_t hj = (uint32_t)__builtin_amdgcn_readlane((int)myh[j], (int)cc);
-                        const uint32_t pos = hj & 0x7FFu;
-                        const uint32_t wi = pos >> 5;
-                        const uint32_t bit = 1u << (pos & 31);
-                        const uint32_t cur = (uint32_t)__builtin_amdgcn_readlane((int)fword, (int)wi);
-                        nb += (cur & bit) ? 0u : 1u;
-                        if ((uint32_t)lane == wi)
-                            fword |= bit;
-                    }
-                    if (nb == 0)
-                        continue;
-                    hash_cnt++;
-                    if (sc == k)
-                        allowed--;
-                    if (hash_cnt >= SDBFX_MAX_ELEM_DD) {
-                        done = true;
-                        break;
+                        const uint32_t fw = sm.filt[b[j] >> 5];
+                        if (own[b[j]] == (uint32_t)lane && !((fw >> (b[j] & 31)) & 1u))
+                            nw = true;
                     }
                 }
+                const uint64_t mc = __ballot(nw);
+                const uint64_t mk = __ballot(nw && sc == k);
+                const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+                const uint32_t before = (uint32_t)__popcll(mc & lt);
+                const uint32_t kbefore = (uint32_t)__popcll(mk & lt);
+                const uint64_t mv = __ballot(sel && sc == k && allowed - (int32_t)kbefore <= 0);
+                const uint64_t mcap = __ballot(nw && hash_cnt + before + 1 == SDBFX_MAX_ELEM_DD);
+                uint32_t r = min(64u, wcount - wpos);
+                if (mv)
+                    r = min(r, (uint32_t)__builtin_ctzll(mv));
+                bool hit = false;
+                if (mcap) {
+                    const uint32_t r2 = (uint32_t)__builtin_ctzll(mcap) + 1;
+                    if (r2 <= r) {
+                        r = r2;
+                        hit = true;
+                    }
+                }
+                // commit lanes [0, r): their bits into the filter
+                __builtin_amdgcn_wave_barrier();
+                if (sel && (uint32_t)lane < r) {
+#pragma unroll
+                    for (int j = 0; j < 5; j++)
+                        atomicOr(&sm.filt[b[j] >> 5], 1u << (b[j] & 31));
+                }
+                __builtin_amdgcn_wave_barrier();
+                const uint64_t mr = r >= 64 ? ~0ull : ((1ull << r) - 1);
+                hash_cnt += (uint32_t)__popcll(mc & mr);
+                allowed -= (int32_t)__popcll(mk & mr);
+                wpos += r;
+                done = hit;
             }
-            if (lane == 0)
+            if (lane == 0) {
                 sm.misc[5] = done ? 1u : 0u;
+                sm.misc[7] = next + wpos;
+            }
         }
         __syncthreads();
-        if (sm.misc[5])
-            break;
+        done = sm.misc[5] != 0;
+        next = sm.misc[7];
+        __syncthreads();
     }
     if (wave == 0) {
+        const uint32_t fword = sm.filt[lane];
         ((uint32_t*)(filters + bd.out * SDBFX_BF))[lane] = fword;
         uint32_t pc = __popc(fword);
 #pragma unroll
@@ -577,7 +707,7 @@ __device__ void stream_tile(TileSmem& sm, const uint8_t* __restrict__ data, cons
     uint32_t* cw = cand + (uint64_t)tile * 5 * SDBFX_CAND_CAP;
     for (uint32_t c = tid; c < C; c += SDBFX_NT) {
         uint32_t h[5];
-        sha1_64(u + td.t0 + sm.u.s.cand[c], h);
+        sha1_64(u + td.t0 + (sm.u.s.cand[c] & 0xFFFFu), h);
 #pragma unroll
         for (int j = 0; j < 5; j++)
             cw[j * SDBFX_CAND_CAP + c] = h[j];

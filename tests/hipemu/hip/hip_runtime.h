@@ -307,19 +307,65 @@ inline T __shfl(T v, int src)
     return (T)hipemu::exchange((uint64_t)v, src);
 }
 inline unsigned long long __ballot(int p) { return hipemu::ballot(p != 0); }
+
+// DPP moves used by the kernels: row_shl:k (0x101-0x10F), row_shr:k
+// (0x111-0x11F), wave_shl:1 (0x130), wave_shr:1 (0x138); full row/bank masks.
+inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int row_mask, int bank_mask, bool bound_ctrl)
+{
+    (void)row_mask;
+    (void)bank_mask;
+    const int l = hipemu::lane();
+    int from = -1;
+    if (ctrl >= 0x101 && ctrl <= 0x10F) {
+        const int k = ctrl - 0x100;
+        if ((l & 15) + k < 16)
+            from = l + k;
+    } else if (ctrl >= 0x111 && ctrl <= 0x11F) {
+        const int k = ctrl - 0x110;
+        if ((l & 15) - k >= 0)
+            from = l - k;
+    } else if (ctrl == 0x130) {
+        from = l + 1 < hipemu::kWave ? l + 1 : -1;
+    } else if (ctrl == 0x138) {
+        from = l - 1;
+    } else {
+        fprintf(stderr, "hipemu: unsupported dpp control %#x\n", ctrl);
+        abort();
+    }
+    const uint64_t v = hipemu::exchange((uint32_t)src, from < 0 ? 1000 : from);
+    if (from < 0 || from >= hipemu::wave_size())
+        return bound_ctrl ? 0 : old;
+    return (int)(uint32_t)v;
+}
+inline uint32_t __lane_id() { return (uint32_t)hipemu::lane(); }
 inline int __builtin_amdgcn_readlane(int v, int l) { return (int)hipemu::exchange((uint32_t)v, l); }
 inline uint32_t __builtin_amdgcn_alignbyte(uint32_t hi, uint32_t lo, uint32_t s)
 {
     return (uint32_t)((((uint64_t)hi << 32) | lo) >> (8 * (s & 3)));
 }
 inline int __popc(uint32_t x) { return __builtin_popcount(x); }
+inline uint64_t __builtin_amdgcn_s_memrealtime() { return 0; }
+inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v)
+{
+    const unsigned long long o = *p;
+    *p = o + v;
+    return o;
+}
 
 inline uint32_t atomicAdd(uint32_t* p, uint32_t v) { const uint32_t o = *p; *p = o + v; return o; }
 inline uint32_t atomicOr(uint32_t* p, uint32_t v) { const uint32_t o = *p; *p = o | v; return o; }
+inline uint32_t atomicMin(uint32_t* p, uint32_t v)
+{
+    const uint32_t o = *p;
+    *p = v < o ? v : o;
+    return o;
+}
+inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 
 inline uint32_t min(uint32_t a, uint32_t b) { return a < b ? a : b; }
 inline uint64_t min(uint64_t a, uint64_t b) { return a < b ? a : b; }
 inline int min(int a, int b) { return a < b ? a : b; }
+inline uint32_t max(uint32_t a, uint32_t b) { return a > b ? a : b; }
 
 // ------------------------------------------------------------- runtime API
 
