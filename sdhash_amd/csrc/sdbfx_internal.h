@@ -20,7 +20,7 @@
 #define SDBFX_NT 256             // threads per digest workgroup
 #define SDBFX_TILE 16384         // owned score positions per stream tile
 #ifndef SDBFX_WARM
-#define SDBFX_WARM 128           // speculative warm-up steps
+#define SDBFX_WARM 64            // speculative warm-up steps
 #endif
 #define SDBFX_CAND_CAP 1024      // candidate slots per stream tile (>= (TILE+64)/17)
 #define SDBFX_PAD 256            // alignment / padding of inputs in device memory

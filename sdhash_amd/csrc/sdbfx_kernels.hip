@@ -45,7 +45,7 @@
 #include "sdbfx_internal.h"
 
 #define RANK_LANES 128
-#define LMAX (SDBFX_TILE + 64 + SDBFX_WARM + 64)
+#define LMAX (SDBFX_TILE + 64 + SDBFX_WARM + 64 + 8)
 #define BCAND_CAP 1152   // block mode: positions with score >= 15 in <= 16320 positions
 #define NWAVE (SDBFX_NT / 64)
 
@@ -239,52 +239,62 @@ __device__ __forceinline__ uint32_t wave_excl_sum(uint32_t v, int lane, uint32_t
     return x + add - v;
 }
 
-// Runs the score state machine over steps [i0, i1) from state m.  With CREDIT
-// each step adds SIGN to the score of the position it elects, when that
-// position is owned ([t0, t0+T)).  When step hand_i is run, the canonical state
-// before step hand_i+1 is stored to *hand.  Returns the state after i1-1.
-// The words a step reads do not depend on the state, so they are loaded four
-// steps at a time ahead of the (branch-free) transitions.
-template <int SIGN, bool CREDIT>
-__device__ uint32_t sm_run(TileSmem& sm, uint32_t base, uint32_t t0, uint32_t T, uint32_t i0,
-                           uint32_t i1, uint32_t m, uint32_t hand_i, uint32_t* hand)
+// Runs the score state machine over steps [i0, i1) from state m (i0 - base
+// is a multiple of 4).  With CREDIT each step adds SIGN to the score of the
+// position it elects, when that position is owned ([t0, t0+T)); consecutive
+// credits to one position are added at once.  With HAND, the canonical state
+// before step hand_i+1 is stored to sm.misc[2] when step hand_i runs.  Returns
+// the state after step i1-1.  The words a step reads do not depend on the
+// state, so four steps' words are loaded at once ahead of the branch-free
+// transitions.
+template <int SIGN, bool CREDIT, bool HAND>
+__device__ __forceinline__ uint32_t sm_run(TileSmem& sm, uint32_t base, uint32_t t0, uint32_t T, uint32_t i0,
+                                           uint32_t i1, uint32_t m, uint32_t hand_i)
 {
     const uint16_t* P = sm.P;
     uint32_t rm = m ? rank_at(P, base, m) : 0u;
+    uint32_t cpos = 0xFFFFFFFFu, ccnt = 0;
+    auto flush = [&]() {
+        const uint32_t rel = cpos - t0;
+        if (ccnt && cpos >= t0 && rel < T)
+            atomicAdd(&sm.u.s.score[rel >> 2], ((uint32_t)SIGN * ccnt) << (8 * (rel & 3)));
+    };
     for (uint32_t i = i0; i < i1; i += 4) {
-        uint32_t w[4], r64[4], rf[4];
+        const uint64_t W4 = *(const uint64_t*)(P + (i - base));
+        const uint64_t R4 = *(const uint64_t*)(P + (i + 64 - base));
+        uint32_t rf[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const uint32_t ii = min(i + j, i1 - 1);
-            w[j] = P[ii - base];
-            r64[j] = P[ii + 64 - base] & 0x3FFu;
-        }
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const uint32_t ii = min(i + j, i1 - 1);
-            rf[j] = P[ii + (w[j] >> 10) - base] & 0x3FFu;
+            const uint32_t w = (uint32_t)(W4 >> (16 * j)) & 0xFFFFu;
+            rf[j] = P[i + j + (w >> 10) - base] & 0x3FFu;
         }
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const uint32_t ii = i + j;
             if (ii < i1) {
                 // cheap slide (sdbf_core.cc:61-68) or full rescan (:69-81)
-                const bool cheap = ii > 0 && rm > 0 && r64[j] >= rm && ii < m;
-                const uint32_t mc = r64[j] == rm ? ii + 64 : m;
-                const uint32_t f = ii + (w[j] >> 10);
-                m = cheap ? mc : f;
+                const uint32_t w = (uint32_t)(W4 >> (16 * j)) & 0xFFFFu;
+                const uint32_t r64 = (uint32_t)(R4 >> (16 * j)) & 0x3FFu;
+                const bool cheap = ii > 0 && rm > 0 && r64 >= rm && ii < m;
+                const uint32_t mc = r64 == rm ? ii + 64 : m;
+                m = cheap ? mc : ii + (w >> 10);
                 const bool has = cheap || rf[j] != 0;
                 rm = cheap ? rm : rf[j];
                 if (CREDIT && has) {
-                    const uint32_t rel = m - t0;
-                    if (m >= t0 && rel < T)
-                        atomicAdd(&sm.u.s.score[rel >> 2], ((uint32_t)SIGN) << (8 * (rel & 3)));
+                    if (m != cpos) {
+                        flush();
+                        cpos = m;
+                        ccnt = 0;
+                    }
+                    ccnt++;
                 }
-                if (hand && ii == hand_i)
-                    *hand = canon(P, base, ii + 1, m);
+                if (HAND && ii == hand_i)
+                    sm.misc[2] = canon(P, base, ii + 1, m);
             }
         }
     }
+    if (CREDIT)
+        flush();
     return m;
 }
 
@@ -299,10 +309,11 @@ __device__ __forceinline__ uint32_t byte_of(const uint4& v, int j)
 // Computes ranks, F and the scores of owned positions [t0, t0+T) of a unit of
 // n bytes at u (n > 64, t0 + T <= n - 64, t0 == 0 or t0 >= 16384; u is 16-byte
 // aligned and 16 bytes past n are readable).  forced >= 0 gives the exact
-// canonical state before step t0-64.  Returns 1 when every score is exact
-// (certified), 0 when the incoming state could not be proven.
-__device__ uint32_t tile_core(TileSmem& sm, const uint8_t* __restrict__ u, uint32_t n, uint32_t t0,
-                              uint32_t T, int64_t forced, uint32_t hand_i, uint32_t* hand_out)
+// canonical state before step t0-64.  When hand_i is not ~0, the canonical
+// state before step hand_i+1 is left in sm.misc[2].  Returns 1 when every
+// score is exact (certified), 0 when the incoming state could not be proven.
+__device__ __forceinline__ uint32_t tile_core(TileSmem& sm, const uint8_t* __restrict__ u, uint32_t n,
+                                              uint32_t t0, uint32_t T, int64_t forced, uint32_t hand_i)
 {
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -382,40 +393,79 @@ __device__ uint32_t tile_core(TileSmem& sm, const uint8_t* __restrict__ u, uint3
     }
     __syncthreads();
 
-    // ---- 2. F (and clear the score words, which alias the histograms)
+    // ---- 2. F (and clear the score words, which alias the histograms).
+    // One lane per 64-aligned block b.  A descending pass keeps the suffix
+    // minima of (rank << 6 | offset) over block b; an ascending pass carries
+    // the prefix minimum over block b+1 and its run-start bits, so the window
+    // starting at offset o takes the smaller rank of suffix[o] and prefix[o-1]
+    // (the suffix on ties: it is the earlier position).  Run-start bits of
+    // block b+1 at offsets >= o lie past the window and are never needed.
     for (int q = tid; q < SDBFX_TILE / 4; q += SDBFX_NT)
         sm.u.s.score[q] = 0;
-    for (uint32_t blk = wave; blk * 64 < NI; blk += NWAVE) {
-        const uint32_t pa = blk * 64 + lane;
-        const uint32_t pb = pa + 64;
-        const uint32_t ra = pa < L ? (sm.P[pa] & 0x3FFu) : 0u;
-        const uint32_t rb = pb < L ? (sm.P[pb] & 0x3FFu) : 0u;
-        const uint32_t rpa = (pa >= 1 && pa - 1 < L) ? (sm.P[pa - 1] & 0x3FFu) : 0xFFFFu;
-        const uint32_t rpb = (pb - 1 < L) ? (sm.P[pb - 1] & 0x3FFu) : 0xFFFFu;
-        const uint32_t ka = ra ? ((ra << 16) | pa) : 0xFFFFFFFFu;
-        const uint32_t kb = rb ? ((rb << 16) | pb) : 0xFFFFFFFFu;
-        const uint32_t sa = wave_suffix_min(ka, lane);
-        const uint32_t pm = wave_prefix_min(kb, lane);
-        const uint32_t pm1 = (uint32_t)__builtin_amdgcn_update_dpp((int)0xFFFFFFFFu, (int)pm, 0x138, 0xF, 0xF, false);
-        const uint32_t key = min(sa, pm1);
-        const uint64_t da = __ballot(pa < L && ra != rpa);
-        const uint64_t db = __ballot(pb < L && rb != rpb);
-        if (pa < NI) {
-            uint32_t fr = 0;
-            if (ra) {
-                const uint32_t q = (key & 0xFFFFu) - blk * 64;   // in [lane, lane+63]
-                const uint32_t t = first_after(da, db, q);
-                const uint32_t end = min(t - 1, (uint32_t)lane + 63);
-                fr = end - lane;
+    for (uint32_t blk = tid; blk * 64 < NI; blk += SDBFX_NT) {
+        const uint32_t p0 = blk * 64;
+        uint64_t rw[16];
+        uint64_t da = 0;
+        uint32_t prev = p0 ? (sm.P[p0 - 1] & 0x3FFu) : 0xFFFFu;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            rw[q] = *(const uint64_t*)(sm.P + p0 + 4 * q);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t r = (uint32_t)(rw[q] >> (16 * j)) & 0x3FFu;
+                da |= (uint64_t)(r != prev) << (4 * q + j);
+                prev = r;
             }
-            sm.P[pa] = (uint16_t)((sm.P[pa] & 0x3FFu) | (fr << 10));
+        }
+        uint32_t suf2[32];
+        uint32_t sfx = 0xFFFFu;
+#pragma unroll
+        for (int o = 63; o >= 0; o--) {
+            const uint32_t r = (uint32_t)(rw[o >> 2] >> (16 * (o & 3))) & 0x3FFu;
+            sfx = min(sfx, r ? ((r << 6) | (uint32_t)o) : 0xFFFFu);
+            if (o & 1)
+                suf2[o >> 1] = sfx << 16;
+            else
+                suf2[o >> 1] |= sfx;
+        }
+        uint64_t db = 0;
+        uint32_t pfx = 0xFFFFu;
+        uint32_t prevb = prev;   // rank at offset 63 of block b
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const uint32_t pq = p0 + 64 + 4 * q;
+            const uint64_t vb = pq < L ? *(const uint64_t*)(sm.P + pq) : 0ull;
+            uint32_t out4[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int o = 4 * q + j;
+                const uint32_t r = (uint32_t)(rw[q] >> (16 * j)) & 0x3FFu;
+                uint32_t fr = 0;
+                if (r) {
+                    const uint32_t so = (o & 1) ? (suf2[o >> 1] >> 16) : (suf2[o >> 1] & 0xFFFFu);
+                    const uint32_t qq = (pfx >> 6) < (so >> 6) ? 64 + (pfx & 63u) : (so & 63u);
+                    const uint32_t t = first_after(da, db, qq);
+                    fr = min(t - 1, (uint32_t)o + 63) - o;
+                }
+                out4[j] = r | (fr << 10);
+                const uint32_t p = pq + j;
+                const uint32_t rb = p < L ? (uint32_t)(vb >> (16 * j)) & 0x3FFu : 0u;
+                pfx = min(pfx, rb ? ((rb << 6) | (uint32_t)o) : 0xFFFFu);
+                db |= (uint64_t)(p < L && rb != prevb) << o;
+                prevb = rb;
+            }
+            if (p0 + 4 * q < NI) {
+                const uint64_t v = (uint64_t)out4[0] | ((uint64_t)out4[1] << 16) | ((uint64_t)out4[2] << 32) |
+                                   ((uint64_t)out4[3] << 48);
+                *(uint64_t*)(sm.P + p0 + 4 * q) = v;
+            }
         }
     }
     __syncthreads();
 
     // ---- 3. scores
     const uint32_t NIter = t_end - a;
-    const uint32_t S = (NIter + SDBFX_NT - 1) / SDBFX_NT;
+    const uint32_t S = ((NIter + SDBFX_NT - 1) / SDBFX_NT + 3) & ~3u;
     const uint32_t s = a + tid * S;
     const uint32_t e = min(s + S, t_end);
     const bool active = s < t_end;
@@ -430,7 +480,7 @@ __device__ uint32_t tile_core(TileSmem& sm, const uint8_t* __restrict__ u, uint3
             uint32_t m = lane ? base + lane : 0u;
             if (m && rank_at(sm.P, base, m) == 0)
                 m = 0;
-            m = sm_run<0, false>(sm, base, t0, T, base, a, m, 0xFFFFFFFFu, nullptr);
+            m = sm_run<0, false, false>(sm, base, t0, T, base, a, m, 0);
             const uint32_t c = canon(sm.P, base, a, m);
             const uint32_t c0 = (uint32_t)__builtin_amdgcn_readlane((int)c, 0);
             cert = __ballot(c != c0) == 0 ? 1u : 0u;
@@ -448,11 +498,13 @@ __device__ uint32_t tile_core(TileSmem& sm, const uint8_t* __restrict__ u, uint3
         mstart = sm.misc[0];
     } else if (active) {
         const uint32_t w = (s > base + SDBFX_WARM) ? s - SDBFX_WARM : base;
-        const uint32_t m = sm_run<0, false>(sm, base, t0, T, w, s, 0u, 0xFFFFFFFFu, nullptr);
+        const uint32_t m = sm_run<0, false, false>(sm, base, t0, T, w, s, 0u, 0);
         mstart = canon(sm.P, base, s, m);
     }
     if (active) {
-        const uint32_t m = sm_run<1, true>(sm, base, t0, T, s, e, mstart, hand_i, hand_out);
+        const uint32_t m = hand_i != 0xFFFFFFFFu
+                               ? sm_run<1, true, true>(sm, base, t0, T, s, e, mstart, hand_i)
+                               : sm_run<1, true, false>(sm, base, t0, T, s, e, mstart, 0);
         sm.start_m[tid] = mstart;
         sm.end_m[tid] = canon(sm.P, base, e, m);
     }
@@ -466,8 +518,10 @@ __device__ uint32_t tile_core(TileSmem& sm, const uint8_t* __restrict__ u, uint3
         }
         __syncthreads();
         if (mis) {
-            sm_run<-1, true>(sm, base, t0, T, s, e, sm.start_m[tid], 0xFFFFFFFFu, nullptr);
-            const uint32_t m = sm_run<1, true>(sm, base, t0, T, s, e, pred, hand_i, hand_out);
+            sm_run<-1, true, false>(sm, base, t0, T, s, e, sm.start_m[tid], 0);
+            const uint32_t m = hand_i != 0xFFFFFFFFu
+                                   ? sm_run<1, true, true>(sm, base, t0, T, s, e, pred, hand_i)
+                                   : sm_run<1, true, false>(sm, base, t0, T, s, e, pred, 0);
             sm.start_m[tid] = pred;
             sm.end_m[tid] = canon(sm.P, base, e, m);
         }
@@ -481,7 +535,7 @@ __device__ uint32_t tile_core(TileSmem& sm, const uint8_t* __restrict__ u, uint3
 // into sm.u.s.cand as (position | score << 16).  256 positions per chunk,
 // four per lane, read one score word at a time.  Returns the count.
 template <typename Pred>
-__device__ uint32_t compact(TileSmem& sm, uint32_t cnt, Pred pred)
+__device__ __forceinline__ uint32_t compact(TileSmem& sm, uint32_t cnt, Pred pred)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t nchunk = (cnt + 255) / 256;
@@ -542,7 +596,7 @@ __global__ __launch_bounds__(SDBFX_NT) void block_digest_kernel(
     const uint32_t n = bd.n;
     const uint32_t T = n - 64;
 
-    tile_core(sm, u, n, 0, T, -1, 0xFFFFFFFFu, nullptr);
+    tile_core(sm, u, n, 0, T, -1, 0xFFFFFFFFu);
 
     // histogram of scores >= 16 over positions [0, n-64)
     if (tid < 68)
@@ -692,7 +746,7 @@ __global__ __launch_bounds__(SDBFX_NT) void block_digest_kernel(
 
 // ------------------------------------------------------------------ stream mode
 
-__device__ void stream_tile(TileSmem& sm, const uint8_t* __restrict__ data, const TileDesc& td,
+__device__ __forceinline__ void stream_tile(TileSmem& sm, const uint8_t* __restrict__ data, const TileDesc& td,
                             uint32_t tile, int64_t forced, uint32_t* __restrict__ cand,
                             TileState* __restrict__ tstate)
 {
@@ -702,7 +756,7 @@ __device__ void stream_tile(TileSmem& sm, const uint8_t* __restrict__ data, cons
     const uint32_t hand_i = (t_end < td.n - 64) ? t_end - 65 : 0xFFFFFFFFu;
     if (tid == 0)
         sm.misc[2] = 0;
-    const uint32_t cert = tile_core(sm, u, td.n, td.t0, td.T, forced, hand_i, &sm.misc[2]);
+    const uint32_t cert = tile_core(sm, u, td.n, td.t0, td.T, forced, hand_i);
     const uint32_t C = compact(sm, td.T, [](uint32_t sc) { return sc > SDBFX_THRESHOLD; });
     uint32_t* cw = cand + (uint64_t)tile * 5 * SDBFX_CAND_CAP;
     for (uint32_t c = tid; c < C; c += SDBFX_NT) {
