@@ -31,6 +31,12 @@ SEGMENT = 128 * MIB
 NFRAG = 4096
 FRAG = 4096
 HBM_PEAK_GBPS = 8000.0
+# MI355X: 256 CUs x 4 SIMDs at 2.4 GHz.  VALU: a wave64 instruction per 4
+# cycles per SIMD, 2 instructions (AND, popcount-accumulate) per 32-bit word,
+# 64 words per filter pair -> 128 lane-instructions per pair.  MFMA: 5.0e15
+# dense int8 ops/s = 2.5e15 MACs/s, 2048 MACs per pair.
+VALU_PEAK_PAIRS = 256 * 4 * 2.4e9 * 16 / 128
+MFMA_PEAK_PAIRS = 5.0e15 / 2 / 2048
 
 
 def log(msg: str) -> None:
@@ -117,13 +123,41 @@ def main() -> int:
     eng.profiling(False)
 
     k = {}
-    for name in ("block_digest_kernel", "stream_tile_kernel", "stream_insert_kernel", "cmp_kernel",
-                 "pair_sum_kernel", "gather_filters_kernel"):
+    for name in ("block_digest_kernel", "stream_tile_kernel", "stream_fixup_kernel", "stream_insert_kernel",
+                 "gather_filters_kernel", "cmp_kernel", "cmp_mfma_kernel", "pair_sum_kernel"):
         ms, n = eng.kernel_time(name)
-        k[name] = (ms, n)
+        if n:
+            k[name] = (ms, n)
     nfilt_t = tset.filter_count()
     pairs = sum(qset.info(i).bf_count for i in range(len(qset))) * nfilt_t
-    cmp_ms = (k["cmp_kernel"][0] + k["pair_sum_kernel"][0]) / args.steps
+    cmp_ms = sum(k.get(n, (0.0, 0))[0] for n in ("cmp_kernel", "cmp_mfma_kernel", "pair_sum_kernel")) / args.steps
+
+    # the two comparison kernels on the same sets: same scores, rate of each
+    # against the peak pair rate of its unit (2048 AND-popcount bit pairs per
+    # filter pair: VALU 2 instructions per 32-bit word per lane; MFMA 2048
+    # int8 MACs)
+    forms = {}
+    for label, method, kname, peak in (
+            ("valu", sdhash_amd.CMP_VALU, "cmp_kernel", VALU_PEAK_PAIRS),
+            ("mfma", sdhash_amd.CMP_MFMA, "cmp_mfma_kernel", MFMA_PEAK_PAIRS)):
+        eng.compare_method(method)
+        got = eng.compare_to(qset, tset)
+        if not np.array_equal(got, scores):
+            log("compare method %s disagrees with the step's scores" % label)
+            return 1
+        eng.sync()
+        eng.profiling(True)
+        eng.reset_kernel_times()
+        for _ in range(args.steps):
+            eng.compare_to(qset, tset)
+        eng.sync()
+        eng.profiling(False)
+        kms, kn = eng.kernel_time(kname)
+        sms, _ = eng.kernel_time("pair_sum_kernel")
+        rate = pairs / (kms / max(kn, 1) / 1e3)
+        forms[label] = {"kernel": kname, "kernel_ms": kms / max(kn, 1), "pair_sum_ms": sms / max(kn, 1),
+                        "pairs_per_s": rate, "peak_pairs_per_s": peak, "frac_of_unit_peak": rate / peak}
+    eng.compare_method(sdhash_amd.CMP_AUTO)
     blk_ms, blk_n = k["block_digest_kernel"]
     blk_avg = blk_ms / max(blk_n, 1)
     digest_ms = blk_ms / args.steps
@@ -142,6 +176,7 @@ def main() -> int:
                    "filter_pairs_per_step": pairs, "image_filters": nfilt_t},
         "digest_MBps": IMAGE / 1e6 / (digest_ms / 1e3),
         "kernel_ms_per_step": {n: v[0] / args.steps for n, v in k.items()},
+        "compare_forms": forms,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
                      "kernel": "block_digest_kernel",

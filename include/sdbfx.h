@@ -120,6 +120,22 @@ int sdbfx_compare_all(sdbfx_ctx *ctx, const sdbfx_set *set, int32_t *scores);
  */
 int sdbfx_compare_to(sdbfx_ctx *ctx, const sdbfx_set *q, const sdbfx_set *t, int32_t *scores);
 
+/*
+ * Selects the comparison kernel for later sdbfx_compare_* calls of the
+ * context.  Both give identical scores:
+ *   SDBFX_CMP_VALU  AND + popcount on the vector ALUs, one reference filter
+ *                   per lane;
+ *   SDBFX_CMP_MFMA  exact int8 matrix-core products of {0,1} bit-planes,
+ *                   128 x 128 filter pairs per wave;
+ *   SDBFX_CMP_AUTO  (default) picks one from the shape of the call.
+ * No counterpart in the reference, whose bf_bitcount_cut_256
+ * (sdbf/bf_utils.cc:115-206) is a single CPU loop.
+ */
+#define SDBFX_CMP_AUTO 0
+#define SDBFX_CMP_VALU 1
+#define SDBFX_CMP_MFMA 2
+int sdbfx_set_compare_method(sdbfx_ctx *ctx, int method);
+
 /* ---- device memory for device-resident inputs ------------------------ */
 
 /* Allocates `bytes` of device memory plus 256 readable padding bytes. */

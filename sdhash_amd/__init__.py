@@ -21,8 +21,11 @@ SYMBOLS = (
     "sdbfx_set_digest", "sdbfx_set_digest_device", "sdbfx_set_add", "sdbfx_set_info",
     "sdbfx_set_read", "sdbfx_compare_all", "sdbfx_compare_to", "sdbfx_device_alloc",
     "sdbfx_device_free", "sdbfx_upload", "sdbfx_set_profiling", "sdbfx_kernel_time",
-    "sdbfx_reset_kernel_times",
+    "sdbfx_reset_kernel_times", "sdbfx_set_compare_method",
 )
+
+# comparison kernels (sdbfx_set_compare_method)
+CMP_AUTO, CMP_VALU, CMP_MFMA = 0, 1, 2
 
 _lib = None
 
@@ -71,13 +74,14 @@ def load():
     L.sdbfx_set_profiling.argtypes = [P, C.c_int]
     L.sdbfx_kernel_time.argtypes = [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(U64)]
     L.sdbfx_reset_kernel_times.argtypes = [P]
+    L.sdbfx_set_compare_method.argtypes = [P, C.c_int]
     for name in SYMBOLS:
         getattr(L, name).restype = getattr(L, name).restype or I32
     for name in ("sdbfx_ctx_create", "sdbfx_sync", "sdbfx_set_create", "sdbfx_set_digest",
                  "sdbfx_set_digest_device", "sdbfx_set_add", "sdbfx_set_info", "sdbfx_set_read",
                  "sdbfx_compare_all", "sdbfx_compare_to", "sdbfx_device_alloc", "sdbfx_device_free",
                  "sdbfx_upload", "sdbfx_set_profiling", "sdbfx_kernel_time",
-                 "sdbfx_reset_kernel_times"):
+                 "sdbfx_reset_kernel_times", "sdbfx_set_compare_method"):
         getattr(L, name).restype = I32
     _lib = L
     return L
@@ -246,6 +250,11 @@ class Engine:
 
     def sync(self) -> None:
         self._check(self.lib.sdbfx_sync(self.ctx))
+
+    def compare_method(self, method: int) -> None:
+        """CMP_AUTO, CMP_VALU (popcount on the vector ALUs) or CMP_MFMA (int8
+        matrix cores on {0,1} bit-planes); the scores are the same."""
+        self._check(self.lib.sdbfx_set_compare_method(self.ctx, method))
 
     def profiling(self, on: bool) -> None:
         self._check(self.lib.sdbfx_set_profiling(self.ctx, 1 if on else 0))

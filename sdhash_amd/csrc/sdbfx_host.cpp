@@ -40,6 +40,7 @@ struct sdbfx_ctx {
     uint64_t budget = 0;
     std::string err;
     bool prof = false;
+    int cmp_method = SDBFX_CMP_AUTO;
     struct Ev {
         const char* name;
         hipEvent_t a, b;
@@ -414,6 +415,18 @@ struct PairRef {
     uint32_t idx;
 };
 
+// MFMA compare for AUTO: when the reference side and the target side both
+// fill most of a 128 x 128 tile on average.
+bool use_mfma(const sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, uint64_t npairs)
+{
+    if (ctx->cmp_method != SDBFX_CMP_AUTO)
+        return ctx->cmp_method == SDBFX_CMP_MFMA;
+    (void)A;
+    (void)B;
+    (void)npairs;
+    return false;
+}
+
 int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool all, int32_t* out)
 {
     const uint32_t na = (uint32_t)A->names.size();
@@ -458,6 +471,8 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
         for (uint64_t k = 0; k < npairs; k++)
             order[pos[key(tgt[k])]++] = (uint32_t)k;
     }
+    const bool mfma = use_mfma(ctx, A, S[1], npairs);
+    const uint32_t rows = mfma ? SDBFX_MROWS : SDBFX_CMP_ROWS;
     std::vector<CmpPair> pairs(npairs);
     std::vector<CmpGroup> groups;
     uint64_t inst = 0, wi = 0, partial_n = 0;
@@ -492,7 +507,7 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
         if (ic > 0xFFFFFFFFull)
             return fail(ctx, "too many filter instances for one target digest");
         g.inst_count = (uint32_t)ic;
-        g.nblk = (g.inst_count + 63) / 64;
+        g.nblk = (g.inst_count + rows - 1) / rows;
         g.wi_begin = wi;
         wi += (uint64_t)g.nblk * g.nch;
         partial_n += (uint64_t)g.inst_count * g.nch;
@@ -505,10 +520,17 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
         return rc;
     SetView va{(const uint32_t*)S[0]->filt.p, (const uint16_t*)S[0]->ecf.p, (const uint16_t*)S[0]->hamf.p};
     SetView vb{(const uint32_t*)S[1]->filt.p, (const uint16_t*)S[1]->ecf.p, (const uint16_t*)S[1]->hamf.p};
-    const uint32_t grid = (uint32_t)std::min<uint64_t>(wi, 16384);
-    if (wi)
+    if (wi && mfma) {
+        // one wave per SIMD (the 128 x 128 tile holds 256 accumulators per lane),
+        // a multiple of the 8 XCDs
+        const uint32_t grid = (uint32_t)std::min<uint64_t>((wi + 7) & ~7ull, 1024);
+        LAUNCH(ctx, cmp_mfma_kernel, grid, 64, va, vb, (const CmpGroup*)ctx->groups.p, (uint32_t)groups.size(),
+               (const CmpPair*)ctx->pairs.p, wi, (uint32_t*)ctx->partial.p);
+    } else if (wi) {
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(wi, 16384);
         LAUNCH(ctx, cmp_kernel, grid, 64, va, vb, (const CmpGroup*)ctx->groups.p, (uint32_t)groups.size(),
                (const CmpPair*)ctx->pairs.p, wi, (uint32_t*)ctx->partial.p);
+    }
     LAUNCH(ctx, pair_sum_kernel, (uint32_t)((npairs + 255) / 256), 256, (const CmpPair*)ctx->pairs.p,
            (uint32_t)npairs, va.ecf, vb.ecf, (const uint32_t*)ctx->partial.p, (int32_t*)ctx->scores.p);
     std::vector<int32_t> sorted(npairs);
@@ -752,6 +774,16 @@ int sdbfx_upload(sdbfx_ctx* ctx, void* dst, const void* src, uint64_t bytes)
         return -1;
     HIPC(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
     HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int sdbfx_set_compare_method(sdbfx_ctx* ctx, int method)
+{
+    if (!ctx)
+        return -1;
+    if (method != SDBFX_CMP_AUTO && method != SDBFX_CMP_VALU && method != SDBFX_CMP_MFMA)
+        return fail(ctx, "unknown compare method");
+    ctx->cmp_method = method;
     return 0;
 }
 

@@ -27,6 +27,8 @@
 
 // compare
 #define SDBFX_CMP_CHUNK 1024     // target filters per compare work item
+#define SDBFX_CMP_ROWS 64        // reference-filter instances per work item, VALU compare
+#define SDBFX_MROWS 128          // reference-filter instances per work item, MFMA compare
 
 // One dd-block of block mode (also the tail block of an input).
 struct BlockDesc {

@@ -1081,6 +1081,351 @@ __global__ __launch_bounds__(64) void cmp_kernel(
     }
 }
 
+// ------------------------------------------------------- compare on MFMA
+//
+// The same per-(instance, chunk) maxima as cmp_kernel, with the AND-popcount of
+// 128 x 128 filter pairs per wave computed on the matrix cores: every filter
+// is expanded into {0,1} bytes ("bit-planes") and multiplied as int8 with
+// int32 accumulation (v_mfma_i32_32x32x32_i8, a 4 x 4 grid of 32 x 32
+// blocks), so the 2048-element dot product of two expanded filters is their
+// exact AND-popcount.
+//
+// Bit-planes: plane j (0..7) of filter word w is (w >> j) & 0x01010101, the
+// bits j, 8+j, 16+j, 24+j of w as bytes.  A K-slice of one MFMA is 32 bytes
+// per row: lanes 0-31 give planes of one word and lanes 32-63 of another (16
+// bytes each).  Over 8 K-slices the two lane halves cover words 8q..8q+3 and
+// 8q+4..8q+7; the reference rows (A) and target columns (B) use the same
+// byte-to-bit assignment, so the product sums each bit position exactly once.
+//
+// Only the full 2048-bit count comes out of the matrix cores.  A pair can
+// score above 0 only if that count exceeds its cutoff; the cutoff grows with
+// s1 + s2, so a count at or below cut[s1min + s2min] (smallest qualifying
+// sizes among the tile's rows and the lane's columns) rules a pair out.  The
+// rare pairs above that bound are queued and recomputed exactly on the VALU
+// with the early-exit checkpoints of bf_bitcount_cut_256; their fractions are
+// merged with a 64-bit LDS max on floor(num * 2^22 / den) (strictly monotone
+// in num/den for denominators below 2^11) followed by num << 16 | den.
+// Result D of one 32x32x32 MFMA: lane l, register i holds row
+// 8 * (i / 4) + 4 * (l / 32) + i % 4, column l % 32.
+
+typedef int sdbfx_v4i __attribute__((vector_size(16)));
+typedef int sdbfx_v16i __attribute__((vector_size(64)));
+
+#define MF_B 4                                  // 32-wide blocks per tile side
+#define MF_CAND_CAP (MF_B * MF_B * 16)         // result registers per tile
+
+__device__ __forceinline__ uint32_t word_of(const uint4& v, int k)
+{
+    return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
+}
+
+// planes sh..sh+3 of w
+__device__ __forceinline__ sdbfx_v4i planes4(uint32_t w, int sh)
+{
+    sdbfx_v4i r;
+    r[0] = (int)((w >> sh) & 0x01010101u);
+    r[1] = (int)((w >> (sh + 1)) & 0x01010101u);
+    r[2] = (int)((w >> (sh + 2)) & 0x01010101u);
+    r[3] = (int)((w >> (sh + 3)) & 0x01010101u);
+    return r;
+}
+
+// Compiler controls for the MFMA loop; they change no result.  acc_read
+// copies one accumulator (an AGPR) to a VGPR where it is used, so the
+// compiler cannot copy all 256 out at once; opaque() hides a value's origin so
+// that a prefetch is not merged with the load it feeds.  The CPU emulation of
+// the kernels (tests/hipemu, which defines HIPEMU) uses the plain forms.
+#if defined(HIPEMU)
+__device__ __forceinline__ uint32_t acc_read(int x) { return (uint32_t)x; }
+__device__ __forceinline__ void opaque(uint32_t&) {}
+#else
+__device__ __forceinline__ uint32_t acc_read(int x)
+{
+    uint32_t v;
+    asm volatile("v_accvgpr_read_b32 %0, %1" : "=v"(v) : "a"(x));
+    return v;
+}
+__device__ __forceinline__ void opaque(uint32_t& x) { asm volatile("" : "+v"(x)); }
+#endif
+__device__ __forceinline__ void opaque(uint4& v)
+{
+    opaque(v.x);
+    opaque(v.y);
+    opaque(v.z);
+    opaque(v.w);
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1)
+        v = min(v, (uint32_t)__shfl_xor((int)v, o));
+    return v;
+}
+
+// bf_bitcount_cut_256 (sdbf/bf_utils.cc:115-206) and the score fraction of
+// sdbf_max_score (sdbf_core.cc:418-440) for one pair of qualifying filters:
+// true with num/den when the match exceeds the cutoff.
+__device__ __forceinline__ bool pair_fraction(const uint32_t* __restrict__ fa, const uint32_t* __restrict__ fb,
+                                              uint32_t s1, uint32_t e1, uint32_t s2, uint32_t e2,
+                                              const uint16_t* cut, uint32_t& num, uint32_t& den)
+{
+    uint32_t c[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const uint4 x = ((const uint4*)fa)[q];
+        const uint4 y = ((const uint4*)fb)[q];
+        const uint32_t p = __popc(x.x & y.x) + __popc(x.y & y.y) + __popc(x.z & y.z) + __popc(x.w & y.w);
+        c[q < 2 ? 0 : q < 4 ? 1 : q < 8 ? 2 : 3] += p;
+    }
+    const uint32_t p32 = c[0], p64 = p32 + c[1], p128 = p64 + c[2], p256 = p128 + c[3];
+    const uint32_t cutoff = cut[min(s1 + s2, 1025u)];
+    const bool pass = (8 * p32 + 48 >= cutoff) && (4 * p64 + 48 >= cutoff) && (2 * p128 + 48 >= cutoff);
+    const uint32_t match = pass ? p256 : 0u;
+    if (match <= cutoff)
+        return false;
+    num = match - cutoff;
+    den = min(e1, e2) - cutoff;
+    return true;
+}
+
+__global__ __launch_bounds__(64) void cmp_mfma_kernel(
+    SetView A, SetView B, const CmpGroup* __restrict__ groups, uint32_t ngroups,
+    const CmpPair* __restrict__ pairs, uint64_t nwork, uint32_t* __restrict__ partial)
+{
+    __shared__ uint16_t cut[1026];                  // cut[min(s, 1025)]
+    __shared__ uint32_t rinfo[SDBFX_MROWS];         // s1 | e1 << 16 (0 for rows past the end)
+    __shared__ const uint32_t* rptr[SDBFX_MROWS];   // reference filter of each row
+    __shared__ unsigned long long best[SDBFX_MROWS];
+    __shared__ uint32_t creg[MF_CAND_CAP];
+    __shared__ unsigned long long cmask[MF_CAND_CAP];
+    // the 128 reference filters, 16-byte chunk c of row r at r * 16 + (c ^ (r & 15))
+    // so that the 32 rows read by one instruction fall in different banks
+    __shared__ uint4 atile[SDBFX_MROWS * 16];
+
+    const int lane = threadIdx.x;
+    const int h = lane >> 5, r32 = lane & 31;
+    for (int s = lane; s < 1026; s += 64) {
+        const uint32_t idx = s >= 28 ? 4096u / (uint32_t)s : 0u;
+        cut[s] = s > 1024 ? (uint16_t)1250 : c_cut[idx];
+    }
+    __syncthreads();
+
+    // each XCD (workgroups are dealt round-robin over 8) takes a contiguous
+    // range of work items, so neighbouring items share its L2
+    const uint32_t xcd = blockIdx.x & 7, nslot = gridDim.x >> 3, slot = blockIdx.x >> 3;
+    const uint64_t w_lo = nwork * xcd / 8, w_hi = nwork * (xcd + 1) / 8;
+
+    for (uint64_t w = w_lo + slot; w < w_hi; w += nslot) {
+        uint32_t lo = 0, hi = ngroups;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (groups[mid].wi_begin <= w)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const CmpGroup g = groups[lo];
+        const uint64_t local = w - g.wi_begin;
+        const uint32_t b = (uint32_t)(local / g.nch);
+        const uint32_t c = (uint32_t)(local % g.nch);
+
+        // rows: instance b*128 + 32*bm + r32 (lanes r32 and r32+32 share it)
+        uint32_t s1min = 0xFFFFu;
+#pragma unroll
+        for (int bm = 0; bm < MF_B; bm++) {
+            const uint32_t row = 32 * bm + r32;
+            const uint32_t li = b * SDBFX_MROWS + row;
+            const bool valid = li < g.inst_count;
+            const uint64_t inst = g.inst_begin + (valid ? li : g.inst_count - 1);
+            uint32_t plo = g.pair_begin, phi = g.pair_end;
+            while (phi - plo > 1) {
+                const uint32_t mid = (plo + phi) >> 1;
+                if (pairs[mid].inst_begin <= inst)
+                    plo = mid;
+                else
+                    phi = mid;
+            }
+            const CmpPair pr = pairs[plo];
+            const SetView R = pr.ref_set ? B : A;
+            const uint64_t rf = pr.ref_first + (inst - pr.inst_begin);
+            const uint32_t* f = R.filt + rf * SDBFX_BFW;
+            const uint32_t s1 = valid ? R.ecf[rf] : 0u;
+            if (s1 >= SDBFX_MIN_ELEM)
+                s1min = min(s1min, s1);
+            if (h == 0) {
+                rinfo[row] = valid ? (s1 | (uint32_t)R.hamf[rf] << 16) : 0u;
+                rptr[row] = f;
+                best[row] = 0;
+            }
+        }
+        s1min = wave_min_u32(s1min);
+        __syncthreads();
+#pragma unroll 4
+        for (uint32_t k = lane; k < SDBFX_MROWS * 16; k += 64) {
+            const uint32_t r = k >> 4, ch = k & 15;
+            atile[r * 16 + (ch ^ (r & 15))] = ((const uint4*)rptr[r])[ch];
+        }
+        __syncthreads();
+
+        const uint64_t j0 = g.tgt_first + (uint64_t)c * SDBFX_CMP_CHUNK;
+        const uint64_t j1 = min(j0 + SDBFX_CMP_CHUNK, g.tgt_first + g.tgt_count);
+        const uint32_t* __restrict__ tf = g.tgt_set ? B.filt : A.filt;
+        const uint16_t* __restrict__ tec = g.tgt_set ? B.ecf : A.ecf;
+        const uint16_t* __restrict__ tham = g.tgt_set ? B.hamf : A.hamf;
+        bool anyq = false;
+
+        for (uint64_t nt = j0; nt < j1; nt += 32 * MF_B) {
+            // target chunk index of each lane's column, relative to column nt
+            const uint4* __restrict__ tb = (const uint4*)(tf + nt * SDBFX_BFW);
+            const uint32_t ncol = (uint32_t)min((uint64_t)(32 * MF_B), j1 - nt);
+            uint32_t boff[MF_B];
+            uint32_t s2min = 0xFFFFu;
+#pragma unroll
+            for (int bn = 0; bn < MF_B; bn++) {
+                const uint32_t col = 32 * bn + r32;
+                const bool ok = col < ncol;
+                const uint64_t j = nt + col;
+                boff[bn] = (ok ? col : ncol - 1) * 16 + h;
+                const uint32_t s2 = ok ? tec[j] : 0u;
+                if (s2 >= SDBFX_MIN_ELEM) {
+                    s2min = min(s2min, s2);
+                    anyq = true;
+                }
+            }
+            const uint32_t lb = (s1min == 0xFFFFu || s2min == 0xFFFFu) ? 0xFFFFFFFFu
+                                                                       : (uint32_t)cut[min(s1min + s2min, 1025u)];
+
+            sdbfx_v16i acc[MF_B][MF_B];
+#pragma unroll
+            for (int bm = 0; bm < MF_B; bm++)
+#pragma unroll
+                for (int bn = 0; bn < MF_B; bn++)
+#pragma unroll
+                    for (int i = 0; i < 16; i++)
+                        acc[bm][bn][i] = 0;
+
+            // words 8q + 4h .. 8q + 4h + 3 of a row or column are its chunk 2q + h
+            uint4 aw[MF_B], bw[MF_B];
+#pragma unroll
+            for (int k = 0; k < MF_B; k++) {
+                aw[k] = atile[(32 * k + r32) * 16 + (h ^ (r32 & 15))];
+                bw[k] = tb[boff[k]];
+                opaque(aw[k]);
+                opaque(bw[k]);
+            }
+#pragma unroll 1
+            for (int q = 0; q < 8; q++) {
+                const int qn = q < 7 ? q + 1 : 7;
+                // prefetch of the next 8 words, consumed one iteration later
+                uint32_t ao = (uint32_t)(r32 * 16 + ((2 * qn + h) ^ (r32 & 15)));
+                uint32_t bo = (uint32_t)(2 * qn);
+                opaque(ao);
+                opaque(bo);
+                uint4 an[MF_B], bnx[MF_B];
+#pragma unroll
+                for (int k = 0; k < MF_B; k++) {
+                    an[k] = atile[ao + 32 * 16 * k];
+                    bnx[k] = tb[boff[k] + bo];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int t = 0; t < 8; t++) {
+                    const int sh = 4 * (t & 1);
+                    sdbfx_v4i av[MF_B], bv[MF_B];
+#pragma unroll
+                    for (int k = 0; k < MF_B; k++) {
+                        av[k] = planes4(word_of(aw[k], t >> 1), sh);
+                        bv[k] = planes4(word_of(bw[k], t >> 1), sh);
+                    }
+#pragma unroll
+                    for (int bm = 0; bm < MF_B; bm++)
+#pragma unroll
+                        for (int bn = 0; bn < MF_B; bn++)
+                            acc[bm][bn] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[bm], bv[bn], acc[bm][bn], 0, 0, 0);
+                }
+#pragma unroll
+                for (int k = 0; k < MF_B; k++) {
+                    aw[k] = an[k];
+                    bw[k] = bnx[k];
+                }
+            }
+
+            // candidates: counts above the lane's lower bound of the cutoff,
+            // block maxima first
+            uint32_t bmax[MF_B * MF_B];
+            uint32_t mx = 0;
+#pragma unroll
+            for (int bm = 0; bm < MF_B; bm++)
+#pragma unroll
+                for (int bn = 0; bn < MF_B; bn++) {
+                    uint32_t x = 0;
+#pragma unroll
+                    for (int i = 0; i < 16; i++)
+                        x = max(x, acc_read(acc[bm][bn][i]));
+                    bmax[bm * MF_B + bn] = x;
+                    mx = max(mx, x);
+                }
+            if (__ballot(mx > lb) == 0)
+                continue;
+            uint32_t ncr = 0;
+#pragma unroll
+            for (int bm = 0; bm < MF_B; bm++)
+#pragma unroll
+                for (int bn = 0; bn < MF_B; bn++) {
+                    if (__ballot(bmax[bm * MF_B + bn] > lb) == 0)
+                        continue;
+#pragma unroll
+                    for (int i = 0; i < 16; i++) {
+                        const unsigned long long m = __ballot(acc_read(acc[bm][bn][i]) > lb);
+                        if (m) {
+                            if (lane == 0) {
+                                creg[ncr] = (uint32_t)((bm * MF_B + bn) * 16 + i);
+                                cmask[ncr] = m;
+                            }
+                            ncr++;
+                        }
+                    }
+                }
+            __syncthreads();
+            for (uint32_t k = 0; k < ncr; k++) {
+                const uint32_t r = creg[k];
+                if (!((cmask[k] >> lane) & 1))
+                    continue;
+                const uint32_t bm = r >> 6, bn = (r >> 4) & 3, i = r & 15;
+                const uint32_t row = 32 * bm + 8 * (i >> 2) + 4 * h + (i & 3);
+                const uint64_t j = nt + 32 * bn + r32;
+                if (j >= j1)
+                    continue;
+                const uint32_t ri = rinfo[row];
+                const uint32_t s1 = ri & 0xFFFFu, s2 = tec[j];
+                if (s1 < SDBFX_MIN_ELEM || s2 < SDBFX_MIN_ELEM)
+                    continue;
+                uint32_t num, den;
+                if (pair_fraction(rptr[row], tf + j * SDBFX_BFW, s1, ri >> 16, s2, tham[j], cut, num, den)) {
+                    const unsigned long long key = ((unsigned long long)num << 22) / den;
+                    atomicMax(&best[row], key << 32 | (unsigned long long)(num << 16 | den));
+                }
+            }
+            __syncthreads();
+        }
+
+        const bool hq = __ballot(anyq) != 0;
+        if (h == 0) {
+#pragma unroll
+            for (int bm = 0; bm < MF_B; bm++) {
+                const uint32_t row = 32 * bm + r32;
+                const uint32_t li = b * SDBFX_MROWS + row;
+                if (li < g.inst_count) {
+                    const unsigned long long v = best[row];
+                    partial[g.partial_begin + (uint64_t)li * g.nch + c] = !hq ? 0xFFFFFFFFu : v ? (uint32_t)v : 1u;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // sdbf_score (sdbf_core.cc:331-389) for each oriented pair: ordered double
 // sum of the per-filter maxima with the reference's -1 sentinel handling,
 // sparse-filter denominator and floor(x + 0.5) rounding.

@@ -18,3 +18,12 @@ def engine():
     e = sdhash_amd.Engine(0)
     yield e
     e.close()
+
+
+@pytest.fixture(params=["valu", "mfma"])
+def cmp_engine(request, engine):
+    """The engine with one of its two comparison kernels selected."""
+    import sdhash_amd
+    engine.compare_method(sdhash_amd.CMP_VALU if request.param == "valu" else sdhash_amd.CMP_MFMA)
+    yield engine
+    engine.compare_method(sdhash_amd.CMP_AUTO)

@@ -88,6 +88,23 @@ def main(lib):
     assert np.array_equal(eng.compare_to(gq, gt), want), (eng.compare_to(gq, gt), want)
     assert np.array_equal(eng.compare_to(gt, gq), oracle.compare_to(t, q))
     assert (want > 0).sum() >= 3
+    # more than one 128-instance row block against a two-chunk target
+    q2 = [oracle.digest_stream(big[o:o + 30000], b"r%d" % k)
+          for k, o in enumerate(range(0, 17000000, 170000))]
+    gq2 = eng.new_set()
+    for d in q2:
+        gq2.add(d.name, d.orig_size, d.bf_count, d.last_count, d.dd_block_size, d.max_elem, d.filters,
+                d.elem_counts)
+    assert sum(d.bf_count for d in q2) > 128
+    want2 = oracle.compare_to(q2, t)
+    assert (want2 > 0).sum() >= 10
+    want_all = oracle.compare_all(ods)
+    for method in (sdhash_amd.CMP_MFMA, sdhash_amd.CMP_VALU):
+        eng.compare_method(method)
+        assert np.array_equal(eng.compare_all(s), want_all), method
+        assert np.array_equal(eng.compare_to(gq, gt), want), method
+        assert np.array_equal(eng.compare_to(gq2, gt), want2), method
+        print("compare method %d ok" % method, flush=True)
     print("compare ok", flush=True)
     eng.close()
 

@@ -6,6 +6,7 @@
 // run one after another.  Test infrastructure only.
 #ifndef HIPEMU_HIP_RUNTIME_H
 #define HIPEMU_HIP_RUNTIME_H
+#define HIPEMU 1
 
 #include <stdint.h>
 #include <stdio.h>
@@ -16,6 +17,7 @@
 #include <atomic>
 #include <barrier>
 #include <cmath>
+#include <array>
 #include <functional>
 #include <memory>
 #include <thread>
@@ -95,6 +97,7 @@ struct Block {
     std::vector<uint32_t> wcount;
     std::vector<uint64_t> wgen;
     std::vector<uint64_t> slot;
+    std::vector<std::array<int32_t, 8>> mslot;   // matrix-core operands of each lane
     int orv = 0;
 };
 
@@ -214,6 +217,7 @@ void launch(const dim3& grid, const dim3& block, F&& body)
         b.wcount.assign(nw, 0);
         b.wgen.assign(nw, 0);
         b.slot.assign(nt, 0);
+        b.mslot.assign(nt, {});
         tl_block = &b;
         tl_body = &fn;
         tl_blockIdx = dim3(bx);
@@ -290,6 +294,7 @@ inline int __syncthreads_or(int p)
 inline void __threadfence() {}
 inline void __threadfence_block() {}
 inline void __builtin_amdgcn_wave_barrier() { hipemu::wave_sync(); }
+inline void __builtin_amdgcn_sched_barrier(int) {}
 
 template <typename T>
 inline T __shfl_down(T v, unsigned off)
@@ -337,6 +342,44 @@ inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int row_mask,
         return bound_ctrl ? 0 : old;
     return (int)(uint32_t)v;
 }
+template <typename T>
+inline T __shfl_xor(T v, int mask)
+{
+    return (T)hipemu::exchange((uint64_t)v, hipemu::lane() ^ mask);
+}
+
+// v_mfma_i32_32x32x32_i8: A lane l = row l%32, 16 int8 of K-half l/32; B lane
+// l = column l%32, same K assignment; D lane l register i = row
+// 8*(i/4) + 4*(l/32) + i%4, column l%32.
+typedef int hipemu_v4i __attribute__((vector_size(16)));
+typedef int hipemu_v16i __attribute__((vector_size(64)));
+inline hipemu_v16i __builtin_amdgcn_mfma_i32_32x32x32_i8(hipemu_v4i a, hipemu_v4i b, hipemu_v16i c, int, int, int)
+{
+    hipemu::Block* bk = hipemu::tl_block;
+    const int base = hipemu::wave() * hipemu::kWave, l = hipemu::lane();
+    auto& me = bk->mslot[threadIdx.x];
+    for (int k = 0; k < 4; k++) {
+        me[k] = a[k];
+        me[4 + k] = b[k];
+    }
+    hipemu::wave_sync();
+    hipemu_v16i d;
+    const int n = l % 32;
+    for (int i = 0; i < 16; i++) {
+        const int m = 8 * (i / 4) + 4 * (l / 32) + i % 4;
+        int32_t s = c[i];
+        for (int hh = 0; hh < 2; hh++) {
+            const int8_t* pa = (const int8_t*)bk->mslot[base + 32 * hh + m].data();
+            const int8_t* pb = (const int8_t*)(bk->mslot[base + 32 * hh + n].data() + 4);
+            for (int j = 0; j < 16; j++)
+                s += (int32_t)pa[j] * (int32_t)pb[j];
+        }
+        d[i] = s;
+    }
+    hipemu::wave_sync();
+    return d;
+}
+
 inline uint32_t __lane_id() { return (uint32_t)hipemu::lane(); }
 inline int __builtin_amdgcn_readlane(int v, int l) { return (int)hipemu::exchange((uint32_t)v, l); }
 inline uint32_t __builtin_amdgcn_alignbyte(uint32_t hi, uint32_t lo, uint32_t s)
@@ -354,6 +397,13 @@ inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v)
 
 inline uint32_t atomicAdd(uint32_t* p, uint32_t v) { const uint32_t o = *p; *p = o + v; return o; }
 inline uint32_t atomicOr(uint32_t* p, uint32_t v) { const uint32_t o = *p; *p = o | v; return o; }
+inline unsigned long long atomicMax(unsigned long long* p, unsigned long long v)
+{
+    const unsigned long long o = *p;
+    if (v > o)
+        *p = v;
+    return o;
+}
 inline uint32_t atomicMin(uint32_t* p, uint32_t v)
 {
     const uint32_t o = *p;

@@ -1,5 +1,6 @@
 """Comparison parity on the GPU: every 0-100 score (and -1) identical with
-the CPU oracle, including the reference's orientation rule."""
+the CPU oracle, including the reference's orientation rule, for both
+comparison kernels (VALU popcount and int8 MFMA on bit-planes)."""
 import numpy as np
 import pytest
 
@@ -20,48 +21,48 @@ def upload_oracle(engine, ds):
     return s
 
 
-def test_compare_all_stream(engine):
+def test_compare_all_stream(cmp_engine):
     files = synth.file_set(31, 24, 256 * KIB)
     names = [b"f%02d" % i for i in range(len(files))]
     ods = [oracle.digest_stream(f, n) for f, n in zip(files, names)]
     want = oracle.compare_all(ods)
-    s = engine.new_set()
+    s = cmp_engine.new_set()
     assert np.all(s.digest(files, names) == 0)
-    got = engine.compare_all(s)
+    got = cmp_engine.compare_all(s)
     assert np.array_equal(got, want), np.flatnonzero(got != want)[:10]
     assert (want > 0).sum() > 10
 
 
-def test_compare_all_mixed_modes_and_names(engine):
+def test_compare_all_mixed_modes_and_names(cmp_engine):
     files = synth.file_set(32, 10, 200 * KIB)
     # names chosen so that the strcmp part of the orientation rule matters
     names = [b"z", b"a", b"m", b"b", b"y", b"c", b"x", b"d", b"w", b"e"]
     ods = []
     for i, (f, n) in enumerate(zip(files, names)):
         ods.append(oracle.digest_block(f, 16384, n) if i % 3 == 0 else oracle.digest_stream(f, n))
-    s = upload_oracle(engine, ods)
+    s = upload_oracle(cmp_engine, ods)
     want = oracle.compare_all(ods)
-    got = engine.compare_all(s)
+    got = cmp_engine.compare_all(s)
     assert np.array_equal(got, want)
 
 
-def test_compare_to_fragments_vs_image(engine):
+def test_compare_to_fragments_vs_image(cmp_engine):
     img = synth.disk_image(33, 2 * MIB)
     frags = synth.fragments(34, img, 256)
     tgt = [oracle.digest_block(img[:MIB], 16384, b"img.0000M"),
            oracle.digest_block(img[MIB:], 16384, b"img.0001M")]
     q = [oracle.digest_stream(f, b"frag%04d" % i) for i, f in enumerate(frags)]
     want = oracle.compare_to(q, tgt)
-    gq = engine.new_set()
+    gq = cmp_engine.new_set()
     assert np.all(gq.digest(frags, [d.name for d in q]) == 0)
-    gt = engine.new_set()
+    gt = cmp_engine.new_set()
     assert np.all(gt.digest([img[:MIB], img[MIB:]], [b"img.0000M", b"img.0001M"], dd_block=16384) == 0)
-    got = engine.compare_to(gq, gt)
+    got = cmp_engine.compare_to(gq, gt)
     assert np.array_equal(got, want)
     assert (want[:128] > 0).sum() > 50
 
 
-def test_compare_hand_cases(engine):
+def test_compare_hand_cases(cmp_engine):
     def bits(*ranges):
         v = np.zeros(2048, np.uint8)
         for lo, hi in ranges:
@@ -74,24 +75,56 @@ def test_compare_hand_cases(engine):
     e = oracle.Digest(b"e", np.concatenate([bits((0, 600)), bits((0, 600))]), 2, 0,
                       np.array([5, 5], np.uint16), 1000, 16384)
     ds = [a, b, c, d, e]
-    s = upload_oracle(engine, ds)
-    got = engine.compare_all(s)
+    s = upload_oracle(cmp_engine, ds)
+    got = cmp_engine.compare_all(s)
     want = oracle.compare_all(ds)
     assert np.array_equal(got, want)
     assert got[0] == 72          # a|b
     assert got[3] == -1          # a|e: every target filter sparse
 
 
-def test_compare_large_target_chunks(engine):
+def test_compare_large_target_chunks(cmp_engine):
     # a target digest of > 1024 filters is split over several work items
     img = synth.disk_image(35, 20 * MIB)
     frags = synth.fragments(36, img, 64, size=64 * KIB)
     t = [oracle.digest_block(img, 16384, b"img")]
     q = [oracle.digest_stream(f, b"q%02d" % i) for i, f in enumerate(frags)]
     want = oracle.compare_to(q, t)
-    gq = upload_oracle(engine, q)
-    gt = upload_oracle(engine, t)
-    got = engine.compare_to(gq, gt)
+    gq = upload_oracle(cmp_engine, q)
+    gt = upload_oracle(cmp_engine, t)
+    got = cmp_engine.compare_to(gq, gt)
     assert np.array_equal(got, want)
     # and the transposed call exercises the other orientation
-    assert np.array_equal(engine.compare_to(gt, gq), oracle.compare_to(t, q))
+    assert np.array_equal(cmp_engine.compare_to(gt, gq), oracle.compare_to(t, q))
+
+
+def test_compare_identical_digests(cmp_engine):
+    # every filter of every digest has exact duplicates: the MFMA path's
+    # candidate queue sees a match in every row of every tile
+    f = synth.file_set(37, 1, 2 * MIB)[0]
+    names = [b"dup%02d" % i for i in range(12)]
+    d = oracle.digest_stream(f, b"x")
+    ods = [oracle.Digest(n, d.filters, d.bf_count, d.last_count, d.elem_counts, d.orig_size,
+                         d.dd_block_size) for n in names]
+    s = upload_oracle(cmp_engine, ods)
+    got = cmp_engine.compare_all(s)
+    want = oracle.compare_all(ods)
+    assert np.array_equal(got, want)
+    assert np.all(want == 100)
+
+
+def test_compare_to_many_rows(cmp_engine):
+    # several 128-instance row blocks and 1024-filter chunks in one group,
+    # with reference rows of both sets (orientation flips on size)
+    img = synth.disk_image(38, 24 * MIB)
+    frags = synth.fragments(39, img, 600, size=8 * KIB)
+    frags += [img[o:o + 4 * MIB] for o in (0, 9 * MIB)]
+    t = [oracle.digest_block(img[:16 * MIB], 16384, b"img.0000M"),
+         oracle.digest_block(img[16 * MIB:], 16384, b"img.0016M"),
+         oracle.digest_block(img[9 * MIB:10 * MIB], 16384, b"img.small")]
+    q = [oracle.digest_stream(x, b"q%03d" % i) for i, x in enumerate(frags)]
+    q = [d for d in q if d is not None]
+    want = oracle.compare_to(q, t)
+    got = cmp_engine.compare_to(upload_oracle(cmp_engine, q), upload_oracle(cmp_engine, t))
+    assert np.array_equal(got, want), np.argwhere(got != want)[:10]
+    assert (want > 0).sum() > 100

@@ -6,12 +6,15 @@ popularity state machine (sdbf/sdbf_core.cc:51-89), hand-computed pair scores
 and brute force.  CPU only."""
 import base64
 import hashlib
+import os
 
 import numpy as np
 import pytest
 
 import oracle
 from sdhash_amd import synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def fingerprint(values):
@@ -347,3 +350,18 @@ def test_block_digest_elem_counts_bounded():
     assert d.bf_count == 9
     assert np.all(d.elem_counts <= 192)
     assert oracle.digest_block(f[:16384 * 8 + 300], 16384).bf_count == 8   # tail < 512 dropped
+
+
+def test_cutoff_table_monotone_in_size():
+    # The MFMA comparison rules a pair out when its full AND-popcount is at
+    # most cut(s1min + s2min); that is a lower bound of every cut(s1 + s2) in
+    # the tile only if cut(s) = CUTOFFS256[4096 // s] never decreases for
+    # s >= 32 (both filters have >= 16 elements) and sums above 1024 map to
+    # 1250, the table's largest entry.
+    import re
+    text = open(os.path.join(ROOT, "sdhash_amd", "csrc", "sdbf_tables.h")).read()
+    body = text.split("#define SDBFX_CUTOFFS256_INIT", 1)[1].split("\n\n", 1)[0]
+    table = [int(x) for x in re.findall(r"\d+", body)]
+    cut = [table[4096 // s] for s in range(28, 1025)]
+    assert all(a <= b for a, b in zip(cut, cut[1:]))
+    assert max(table) == 1250 and cut[-1] <= 1250
