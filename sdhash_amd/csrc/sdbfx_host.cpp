@@ -521,10 +521,10 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
     SetView va{(const uint32_t*)S[0]->filt.p, (const uint16_t*)S[0]->ecf.p, (const uint16_t*)S[0]->hamf.p};
     SetView vb{(const uint32_t*)S[1]->filt.p, (const uint16_t*)S[1]->ecf.p, (const uint16_t*)S[1]->hamf.p};
     if (wi && mfma) {
-        // one wave per SIMD (the 128 x 128 tile holds 256 accumulators per lane),
-        // a multiple of the 8 XCDs
-        const uint32_t grid = (uint32_t)std::min<uint64_t>((wi + 7) & ~7ull, 1024);
-        LAUNCH(ctx, cmp_mfma_kernel, grid, 64, va, vb, (const CmpGroup*)ctx->groups.p, (uint32_t)groups.size(),
+        // one wave per SIMD (the 128 x 128 tile holds 256 accumulators per
+        // lane): a 4-wave workgroup per CU, a multiple of the 8 XCDs
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(((wi + 3) / 4 + 7) & ~7ull, 256);
+        LAUNCH(ctx, cmp_mfma_kernel, grid, 256, va, vb, (const CmpGroup*)ctx->groups.p, (uint32_t)groups.size(),
                (const CmpPair*)ctx->pairs.p, wi, (uint32_t*)ctx->partial.p);
     } else if (wi) {
         const uint32_t grid = (uint32_t)std::min<uint64_t>(wi, 16384);

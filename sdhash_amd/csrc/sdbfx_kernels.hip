@@ -1098,13 +1098,14 @@ __global__ __launch_bounds__(64) void cmp_kernel(
 // byte-to-bit assignment, so the product sums each bit position exactly once.
 //
 // Only the full 2048-bit count comes out of the matrix cores.  A pair can
-// score above 0 only if that count exceeds its cutoff; the cutoff grows with
-// s1 + s2, so a count at or below cut[s1min + s2min] (smallest qualifying
-// sizes among the tile's rows and the lane's columns) rules a pair out.  The
-// rare pairs above that bound are queued and recomputed exactly on the VALU
-// with the early-exit checkpoints of bf_bitcount_cut_256; their fractions are
-// merged with a 64-bit LDS max on floor(num * 2^22 / den) (strictly monotone
-// in num/den for denominators below 2^11) followed by num << 16 | den.
+// score above 0 only if that count exceeds its cutoff cut(s1 + s2), so every
+// accumulator starts at minus its pair's cutoff (read from an LDS table
+// indexed by the sum of the two filters' size codes, size_code below) and
+// ends at count - cutoff.  The rare pairs above 0 are queued and recomputed
+// exactly on the VALU with the early-exit checkpoints of bf_bitcount_cut_256;
+// their fractions are merged with a 64-bit LDS max on floor(num * 2^22 / den)
+// (strictly monotone in num/den for denominators below 2^11) followed by
+// num << 16 | den.
 // Result D of one 32x32x32 MFMA: lane l, register i holds row
 // 8 * (i / 4) + 4 * (l / 32) + i % 4, column l % 32.
 
@@ -1135,9 +1136,12 @@ __device__ __forceinline__ sdbfx_v4i planes4(uint32_t w, int sh)
 // compiler cannot copy all 256 out at once; opaque() hides a value's origin so
 // that a prefetch is not merged with the load it feeds.  The CPU emulation of
 // the kernels (tests/hipemu, which defines HIPEMU) uses the plain forms.
+// in_agprs() places a 32x32 block of accumulators in AGPRs at that point,
+// so that its initial values are not held in VGPRs up to the loop.
 #if defined(HIPEMU)
 __device__ __forceinline__ uint32_t acc_read(int x) { return (uint32_t)x; }
 __device__ __forceinline__ void opaque(uint32_t&) {}
+__device__ __forceinline__ void in_agprs(sdbfx_v16i&) {}
 #else
 __device__ __forceinline__ uint32_t acc_read(int x)
 {
@@ -1146,6 +1150,7 @@ __device__ __forceinline__ uint32_t acc_read(int x)
     return v;
 }
 __device__ __forceinline__ void opaque(uint32_t& x) { asm volatile("" : "+v"(x)); }
+__device__ __forceinline__ void in_agprs(sdbfx_v16i& v) { asm volatile("" : "+a"(v)); }
 #endif
 __device__ __forceinline__ void opaque(uint4& v)
 {
@@ -1155,31 +1160,32 @@ __device__ __forceinline__ void opaque(uint4& v)
     opaque(v.w);
 }
 
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
+// Index part of a filter with s elements in the cutoff table cutx: cutx[a + b]
+// is the reference's cutoff for two qualifying filters (>= 16 elements), 1250
+// for sums above 1024, and 0xFFFF (never exceeded) when either filter has
+// fewer than 16 elements.
+__device__ __forceinline__ uint32_t size_code(uint32_t s)
 {
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1)
-        v = min(v, (uint32_t)__shfl_xor((int)v, o));
-    return v;
+    return s >= SDBFX_MIN_ELEM ? min(s, 1025u) : 2048u;
 }
 
 // bf_bitcount_cut_256 (sdbf/bf_utils.cc:115-206) and the score fraction of
-// sdbf_max_score (sdbf_core.cc:418-440) for one pair of qualifying filters:
-// true with num/den when the match exceeds the cutoff.
-__device__ __forceinline__ bool pair_fraction(const uint32_t* __restrict__ fa, const uint32_t* __restrict__ fb,
-                                              uint32_t s1, uint32_t e1, uint32_t s2, uint32_t e2,
-                                              const uint16_t* cut, uint32_t& num, uint32_t& den)
+// sdbf_max_score (sdbf_core.cc:418-440) for one pair of qualifying filters
+// with the given cutoff: true with num/den when the match exceeds it.  The
+// reference filter's 16-byte chunk q is ra[q ^ sw] (the swizzled LDS tile).
+__device__ __forceinline__ bool pair_fraction(const uint4* ra, uint32_t sw, const uint4* __restrict__ fb,
+                                              uint32_t cutoff, uint32_t e1, uint32_t e2, uint32_t& num,
+                                              uint32_t& den)
 {
     uint32_t c[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int q = 0; q < 16; q++) {
-        const uint4 x = ((const uint4*)fa)[q];
-        const uint4 y = ((const uint4*)fb)[q];
-        const uint32_t p = __popc(x.x & y.x) + __popc(x.y & y.y) + __popc(x.z & y.z) + __popc(x.w & y.w);
-        c[q < 2 ? 0 : q < 4 ? 1 : q < 8 ? 2 : 3] += p;
+        const uint4 x = ra[q ^ sw];
+        const uint4 y = fb[q];
+        c[q < 2 ? 0 : q < 4 ? 1 : q < 8 ? 2 : 3] +=
+            __popc(x.x & y.x) + __popc(x.y & y.y) + __popc(x.z & y.z) + __popc(x.w & y.w);
     }
     const uint32_t p32 = c[0], p64 = p32 + c[1], p128 = p64 + c[2], p256 = p128 + c[3];
-    const uint32_t cutoff = cut[min(s1 + s2, 1025u)];
     const bool pass = (8 * p32 + 48 >= cutoff) && (4 * p64 + 48 >= cutoff) && (2 * p128 + 48 >= cutoff);
     const uint32_t match = pass ? p256 : 0u;
     if (match <= cutoff)
@@ -1189,31 +1195,74 @@ __device__ __forceinline__ bool pair_fraction(const uint32_t* __restrict__ fa, c
     return true;
 }
 
-__global__ __launch_bounds__(64) void cmp_mfma_kernel(
+// position of the r-th (from 0) set bit of m; m has more than r set bits
+__device__ __forceinline__ uint32_t select_bit(unsigned long long m, uint32_t r)
+{
+    uint32_t pos = 0, x = (uint32_t)m;
+    uint32_t c = __popc(x);
+    if (r >= c) {
+        r -= c;
+        x = (uint32_t)(m >> 32);
+        pos = 32;
+    }
+#pragma unroll
+    for (int w = 16; w >= 1; w >>= 1) {
+        c = __popc(x & ((1u << w) - 1));
+        if (r >= c) {
+            r -= c;
+            x >>= w;
+            pos += w;
+        }
+    }
+    return pos;
+}
+
+// Per-wave LDS of cmp_mfma_kernel.
+struct MfmaWaveSmem {
+    // the 128 reference filters, 16-byte chunk c of row r at r * 16 + (c ^ (r & 15))
+    // so that the 32 rows read by one instruction fall in different banks
+    uint4 atile[SDBFX_MROWS * 16];
+    uint32_t rinfo[SDBFX_MROWS];                    // size_code(s1) | e1 << 16
+    const uint32_t* rptr[SDBFX_MROWS];              // reference filter of each row
+    unsigned long long best[SDBFX_MROWS];
+    uint32_t creg[MF_CAND_CAP];
+    unsigned long long cmask[MF_CAND_CAP];
+};
+
+// LDS written by some lanes of a wave, then read by others
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Four waves per workgroup, one per SIMD, each taking work items on its own;
+// they share only the cutoff table.
+__global__ __launch_bounds__(256) void cmp_mfma_kernel(
     SetView A, SetView B, const CmpGroup* __restrict__ groups, uint32_t ngroups,
     const CmpPair* __restrict__ pairs, uint64_t nwork, uint32_t* __restrict__ partial)
 {
-    __shared__ uint16_t cut[1026];                  // cut[min(s, 1025)]
-    __shared__ uint32_t rinfo[SDBFX_MROWS];         // s1 | e1 << 16 (0 for rows past the end)
-    __shared__ const uint32_t* rptr[SDBFX_MROWS];   // reference filter of each row
-    __shared__ unsigned long long best[SDBFX_MROWS];
-    __shared__ uint32_t creg[MF_CAND_CAP];
-    __shared__ unsigned long long cmask[MF_CAND_CAP];
-    // the 128 reference filters, 16-byte chunk c of row r at r * 16 + (c ^ (r & 15))
-    // so that the 32 rows read by one instruction fall in different banks
-    __shared__ uint4 atile[SDBFX_MROWS * 16];
+    __shared__ int16_t ncut[4097];                  // -cutoff by size_code(s1) + size_code(s2)
+    __shared__ MfmaWaveSmem wsm[4];
 
-    const int lane = threadIdx.x;
-    const int h = lane >> 5, r32 = lane & 31;
-    for (int s = lane; s < 1026; s += 64) {
+    for (int s = threadIdx.x; s < 4097; s += 256) {
         const uint32_t idx = s >= 28 ? 4096u / (uint32_t)s : 0u;
-        cut[s] = s > 1024 ? (uint16_t)1250 : c_cut[idx];
+        ncut[s] = s > 2050 ? (int16_t)-32768 : s > 1024 ? (int16_t)-1250 : (int16_t)-(int)c_cut[idx];
     }
     __syncthreads();
 
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int h = lane >> 5, r32 = lane & 31;
+    uint4* const atile = wsm[wv].atile;
+    uint32_t* const rinfo = wsm[wv].rinfo;
+    const uint32_t** const rptr = wsm[wv].rptr;
+    unsigned long long* const best = wsm[wv].best;
+    uint32_t* const creg = wsm[wv].creg;
+    unsigned long long* const cmask = wsm[wv].cmask;
+
     // each XCD (workgroups are dealt round-robin over 8) takes a contiguous
     // range of work items, so neighbouring items share its L2
-    const uint32_t xcd = blockIdx.x & 7, nslot = gridDim.x >> 3, slot = blockIdx.x >> 3;
+    const uint32_t xcd = blockIdx.x & 7, nslot = (gridDim.x >> 3) * 4, slot = (blockIdx.x >> 3) * 4 + wv;
     const uint64_t w_lo = nwork * xcd / 8, w_hi = nwork * (xcd + 1) / 8;
 
     for (uint64_t w = w_lo + slot; w < w_hi; w += nslot) {
@@ -1231,7 +1280,6 @@ __global__ __launch_bounds__(64) void cmp_mfma_kernel(
         const uint32_t c = (uint32_t)(local % g.nch);
 
         // rows: instance b*128 + 32*bm + r32 (lanes r32 and r32+32 share it)
-        uint32_t s1min = 0xFFFFu;
 #pragma unroll
         for (int bm = 0; bm < MF_B; bm++) {
             const uint32_t row = 32 * bm + r32;
@@ -1250,23 +1298,19 @@ __global__ __launch_bounds__(64) void cmp_mfma_kernel(
             const SetView R = pr.ref_set ? B : A;
             const uint64_t rf = pr.ref_first + (inst - pr.inst_begin);
             const uint32_t* f = R.filt + rf * SDBFX_BFW;
-            const uint32_t s1 = valid ? R.ecf[rf] : 0u;
-            if (s1 >= SDBFX_MIN_ELEM)
-                s1min = min(s1min, s1);
             if (h == 0) {
-                rinfo[row] = valid ? (s1 | (uint32_t)R.hamf[rf] << 16) : 0u;
+                rinfo[row] = valid ? (size_code(R.ecf[rf]) | (uint32_t)R.hamf[rf] << 16) : size_code(0);
                 rptr[row] = f;
                 best[row] = 0;
             }
         }
-        s1min = wave_min_u32(s1min);
-        __syncthreads();
+        wave_lds_sync();
 #pragma unroll 4
         for (uint32_t k = lane; k < SDBFX_MROWS * 16; k += 64) {
             const uint32_t r = k >> 4, ch = k & 15;
             atile[r * 16 + (ch ^ (r & 15))] = ((const uint4*)rptr[r])[ch];
         }
-        __syncthreads();
+        wave_lds_sync();
 
         const uint64_t j0 = g.tgt_first + (uint64_t)c * SDBFX_CMP_CHUNK;
         const uint64_t j1 = min(j0 + SDBFX_CMP_CHUNK, g.tgt_first + g.tgt_count);
@@ -1279,8 +1323,7 @@ __global__ __launch_bounds__(64) void cmp_mfma_kernel(
             // target chunk index of each lane's column, relative to column nt
             const uint4* __restrict__ tb = (const uint4*)(tf + nt * SDBFX_BFW);
             const uint32_t ncol = (uint32_t)min((uint64_t)(32 * MF_B), j1 - nt);
-            uint32_t boff[MF_B];
-            uint32_t s2min = 0xFFFFu;
+            uint32_t boff[MF_B], ccode[MF_B];
 #pragma unroll
             for (int bn = 0; bn < MF_B; bn++) {
                 const uint32_t col = 32 * bn + r32;
@@ -1288,23 +1331,31 @@ __global__ __launch_bounds__(64) void cmp_mfma_kernel(
                 const uint64_t j = nt + col;
                 boff[bn] = (ok ? col : ncol - 1) * 16 + h;
                 const uint32_t s2 = ok ? tec[j] : 0u;
-                if (s2 >= SDBFX_MIN_ELEM) {
-                    s2min = min(s2min, s2);
-                    anyq = true;
-                }
+                anyq |= s2 >= SDBFX_MIN_ELEM;
+                ccode[bn] = 2 * size_code(s2);
             }
-            const uint32_t lb = (s1min == 0xFFFFu || s2min == 0xFFFFu) ? 0xFFFFFFFFu
-                                                                       : (uint32_t)cut[min(s1min + s2min, 1025u)];
+            const char* ncutb = (const char*)ncut;
 
+            // accumulators start at -cutoff of their pair, so that after the
+            // 2048 bits they hold count - cutoff
             sdbfx_v16i acc[MF_B][MF_B];
+            uint32_t rb = 8 * h;
+            opaque(rb);
 #pragma unroll
-            for (int bm = 0; bm < MF_B; bm++)
+            for (int bm = 0; bm < MF_B; bm++) {
+                uint32_t rc[16];
 #pragma unroll
-                for (int bn = 0; bn < MF_B; bn++)
+                for (int i = 0; i < 16; i++)
+                    rc[i] = 2 * ((const uint16_t*)rinfo)[rb + 64 * bm + 16 * (i >> 2) + 2 * (i & 3)];
+#pragma unroll
+                for (int bn = 0; bn < MF_B; bn++) {
 #pragma unroll
                     for (int i = 0; i < 16; i++)
-                        acc[bm][bn][i] = 0;
-
+                        acc[bm][bn][i] = *(const int16_t*)(ncutb + rc[i] + ccode[bn]);
+                    in_agprs(acc[bm][bn]);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
             // words 8q + 4h .. 8q + 4h + 3 of a row or column are its chunk 2q + h
             uint4 aw[MF_B], bw[MF_B];
 #pragma unroll
@@ -1351,33 +1402,32 @@ __global__ __launch_bounds__(64) void cmp_mfma_kernel(
                 }
             }
 
-            // candidates: counts above the lane's lower bound of the cutoff,
-            // block maxima first
-            uint32_t bmax[MF_B * MF_B];
-            uint32_t mx = 0;
+            // candidates: count - cutoff > 0, block maxima first
+            int bmax[MF_B * MF_B];
+            int mx = -1;
 #pragma unroll
             for (int bm = 0; bm < MF_B; bm++)
 #pragma unroll
                 for (int bn = 0; bn < MF_B; bn++) {
-                    uint32_t x = 0;
+                    int x = -1;
 #pragma unroll
                     for (int i = 0; i < 16; i++)
-                        x = max(x, acc_read(acc[bm][bn][i]));
+                        x = max(x, acc[bm][bn][i]);
                     bmax[bm * MF_B + bn] = x;
                     mx = max(mx, x);
                 }
-            if (__ballot(mx > lb) == 0)
+            if (__ballot(mx > 0) == 0)
                 continue;
             uint32_t ncr = 0;
 #pragma unroll
             for (int bm = 0; bm < MF_B; bm++)
 #pragma unroll
                 for (int bn = 0; bn < MF_B; bn++) {
-                    if (__ballot(bmax[bm * MF_B + bn] > lb) == 0)
+                    if (__ballot(bmax[bm * MF_B + bn] > 0) == 0)
                         continue;
 #pragma unroll
                     for (int i = 0; i < 16; i++) {
-                        const unsigned long long m = __ballot(acc_read(acc[bm][bn][i]) > lb);
+                        const unsigned long long m = __ballot((int)acc_read(acc[bm][bn][i]) > 0);
                         if (m) {
                             if (lane == 0) {
                                 creg[ncr] = (uint32_t)((bm * MF_B + bn) * 16 + i);
@@ -1387,27 +1437,54 @@ __global__ __launch_bounds__(64) void cmp_mfma_kernel(
                         }
                     }
                 }
-            __syncthreads();
-            for (uint32_t k = 0; k < ncr; k++) {
-                const uint32_t r = creg[k];
-                if (!((cmask[k] >> lane) & 1))
-                    continue;
-                const uint32_t bm = r >> 6, bn = (r >> 4) & 3, i = r & 15;
-                const uint32_t row = 32 * bm + 8 * (i >> 2) + 4 * h + (i & 3);
-                const uint64_t j = nt + 32 * bn + r32;
-                if (j >= j1)
-                    continue;
-                const uint32_t ri = rinfo[row];
-                const uint32_t s1 = ri & 0xFFFFu, s2 = tec[j];
-                if (s1 < SDBFX_MIN_ELEM || s2 < SDBFX_MIN_ELEM)
-                    continue;
-                uint32_t num, den;
-                if (pair_fraction(rptr[row], tf + j * SDBFX_BFW, s1, ri >> 16, s2, tham[j], cut, num, den)) {
-                    const unsigned long long key = ((unsigned long long)num << 22) / den;
-                    atomicMax(&best[row], key << 32 | (unsigned long long)(num << 16 | den));
+            wave_lds_sync();
+            // expand the (register, lane mask) entries into one candidate per
+            // lane, 64 at a time: creg[k] gets the entry's first candidate
+            // index in its upper bits (ncr <= 256, 4 entries per lane)
+            uint32_t pc[4], own = 0;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const uint32_t k = 4 * lane + e;
+                pc[e] = k < ncr ? (uint32_t)__popcll(cmask[k]) : 0u;
+                own += pc[e];
+            }
+            uint32_t total;
+            uint32_t first = wave_excl_sum(own, lane, &total);
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const uint32_t k = 4 * lane + e;
+                if (k < ncr)
+                    creg[k] |= first << 8;
+                first += pc[e];
+            }
+            wave_lds_sync();
+            for (uint32_t base = 0; base < total; base += 64) {
+                const uint32_t idx = base + lane;
+                if (idx < total) {
+                    uint32_t lo = 0, hi = ncr;
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if ((creg[mid] >> 8) <= idx)
+                            lo = mid;
+                        else
+                            hi = mid;
+                    }
+                    const uint32_t r = creg[lo] & 0xFFu;
+                    const uint32_t src = select_bit(cmask[lo], idx - (creg[lo] >> 8));
+                    const uint32_t bm = r >> 6, bn = (r >> 4) & 3, i = r & 15;
+                    const uint32_t row = 32 * bm + 8 * (i >> 2) + 4 * (src >> 5) + (i & 3);
+                    const uint64_t j = nt + 32 * bn + (src & 31);
+                    const uint32_t ri = rinfo[row];
+                    const uint32_t cutoff = (uint32_t)-(int)ncut[(ri & 0xFFFFu) + size_code(tec[j])];
+                    uint32_t num, den;
+                    if (pair_fraction(atile + row * 16, row & 15, (const uint4*)(tf + j * SDBFX_BFW), cutoff,
+                                      ri >> 16, tham[j], num, den)) {
+                        const unsigned long long key = ((unsigned long long)num << 22) / den;
+                        atomicMax(&best[row], key << 32 | (unsigned long long)(num << 16 | den));
+                    }
                 }
             }
-            __syncthreads();
+            wave_lds_sync();
         }
 
         const bool hq = __ballot(anyq) != 0;
@@ -1422,7 +1499,7 @@ __global__ __launch_bounds__(64) void cmp_mfma_kernel(
                 }
             }
         }
-        __syncthreads();
+        wave_lds_sync();
     }
 }
 

@@ -99,11 +99,21 @@ def main(lib):
     want2 = oracle.compare_to(q2, t)
     assert (want2 > 0).sum() >= 10
     want_all = oracle.compare_all(ods)
+    # identical digests: a candidate in every row of a tile (more than 64)
+    d = oracle.digest_stream(synth.file_set(9, 1, 400 * KIB)[0], b"x")
+    dups = [oracle.Digest(b"d%d" % i, d.filters, d.bf_count, d.last_count, None, d.orig_size, 0)
+            for i in range(4)]
+    gd = eng.new_set()
+    for x in dups:
+        gd.add(x.name, x.orig_size, x.bf_count, x.last_count, 0, 160, x.filters, None)
+    want_dup = oracle.compare_all(dups)
+    assert d.bf_count > 20 and np.all(want_dup == 100)
     for method in (sdhash_amd.CMP_MFMA, sdhash_amd.CMP_VALU):
         eng.compare_method(method)
         assert np.array_equal(eng.compare_all(s), want_all), method
         assert np.array_equal(eng.compare_to(gq, gt), want), method
         assert np.array_equal(eng.compare_to(gq2, gt), want2), method
+        assert np.array_equal(eng.compare_all(gd), want_dup), method
         print("compare method %d ok" % method, flush=True)
     print("compare ok", flush=True)
     eng.close()

@@ -295,6 +295,7 @@ inline void __threadfence() {}
 inline void __threadfence_block() {}
 inline void __builtin_amdgcn_wave_barrier() { hipemu::wave_sync(); }
 inline void __builtin_amdgcn_sched_barrier(int) {}
+#define __builtin_amdgcn_fence(order, scope) ((void)0)
 
 template <typename T>
 inline T __shfl_down(T v, unsigned off)
@@ -382,6 +383,7 @@ inline hipemu_v16i __builtin_amdgcn_mfma_i32_32x32x32_i8(hipemu_v4i a, hipemu_v4
 
 inline uint32_t __lane_id() { return (uint32_t)hipemu::lane(); }
 inline int __builtin_amdgcn_readlane(int v, int l) { return (int)hipemu::exchange((uint32_t)v, l); }
+inline int __builtin_amdgcn_readfirstlane(int v) { return (int)hipemu::exchange((uint32_t)v, 0); }
 inline uint32_t __builtin_amdgcn_alignbyte(uint32_t hi, uint32_t lo, uint32_t s)
 {
     return (uint32_t)((((uint64_t)hi << 32) | lo) >> (8 * (s & 3)));
@@ -415,6 +417,7 @@ inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 inline uint32_t min(uint32_t a, uint32_t b) { return a < b ? a : b; }
 inline uint64_t min(uint64_t a, uint64_t b) { return a < b ? a : b; }
 inline int min(int a, int b) { return a < b ? a : b; }
+inline int max(int a, int b) { return a > b ? a : b; }
 inline uint32_t max(uint32_t a, uint32_t b) { return a > b ? a : b; }
 
 // ------------------------------------------------------------- runtime API
