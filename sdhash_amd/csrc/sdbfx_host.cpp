@@ -415,16 +415,22 @@ struct PairRef {
     uint32_t idx;
 };
 
-// MFMA compare for AUTO: when the reference side and the target side both
-// fill most of a 128 x 128 tile on average.
-bool use_mfma(const sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, uint64_t npairs)
+// Kernel choice for SDBFX_CMP_AUTO from the padded work of each tiling, per
+// target digest with I reference-filter instances and T target filters:
+// VALU ceil(I/64)*64 * T pair slots at about 8 SIMD cycles each (128
+// instructions per lane-pair), MFMA ceil(I/128)*128 * ceil(T/128)*128 slots
+// at about 2.75 cycles each (measured on MI355X, candidate re-checks
+// included).
+bool use_mfma(const sdbfx_ctx* ctx, const std::vector<uint64_t>& inst, const std::vector<uint64_t>& tgt)
 {
     if (ctx->cmp_method != SDBFX_CMP_AUTO)
         return ctx->cmp_method == SDBFX_CMP_MFMA;
-    (void)A;
-    (void)B;
-    (void)npairs;
-    return false;
+    double valu = 0, mfma = 0;
+    for (size_t g = 0; g < inst.size(); g++) {
+        valu += 8.0 * (double)((inst[g] + 63) / 64 * 64) * (double)tgt[g];
+        mfma += 2.75 * (double)((inst[g] + 127) / 128 * 128) * (double)((tgt[g] + 127) / 128 * 128);
+    }
+    return mfma < valu;
 }
 
 int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool all, int32_t* out)
@@ -471,7 +477,20 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
         for (uint64_t k = 0; k < npairs; k++)
             order[pos[key(tgt[k])]++] = (uint32_t)k;
     }
-    const bool mfma = use_mfma(ctx, A, S[1], npairs);
+    std::vector<uint64_t> ginst, gtgt;
+    for (uint32_t q = 0; q < nkeys; q++) {
+        if (kc[q] == kc[q + 1])
+            continue;
+        const sdbfx_set* T = S[q >= na ? 1 : 0];
+        uint64_t ic = 0;
+        for (uint32_t s = kc[q]; s < kc[q + 1]; s++) {
+            const PairRef r = ref[order[s]];
+            ic += S[r.set]->bfc[r.idx];
+        }
+        ginst.push_back(ic);
+        gtgt.push_back(T->bfc[q >= na ? q - na : q]);
+    }
+    const bool mfma = use_mfma(ctx, ginst, gtgt);
     const uint32_t rows = mfma ? SDBFX_MROWS : SDBFX_CMP_ROWS;
     std::vector<CmpPair> pairs(npairs);
     std::vector<CmpGroup> groups;

@@ -1099,9 +1099,9 @@ __global__ __launch_bounds__(64) void cmp_kernel(
 //
 // Only the full 2048-bit count comes out of the matrix cores.  A pair can
 // score above 0 only if that count exceeds its cutoff cut(s1 + s2), so every
-// accumulator starts at minus its pair's cutoff (read from an LDS table
+// accumulator starts at -(cutoff + 1) of its pair (read from an LDS table
 // indexed by the sum of the two filters' size codes, size_code below) and
-// ends at count - cutoff.  The rare pairs above 0 are queued and recomputed
+// ends >= 0 exactly for those pairs.  These rare pairs are listed and recomputed
 // exactly on the VALU with the early-exit checkpoints of bf_bitcount_cut_256;
 // their fractions are merged with a 64-bit LDS max on floor(num * 2^22 / den)
 // (strictly monotone in num/den for denominators below 2^11) followed by
@@ -1113,7 +1113,7 @@ typedef int sdbfx_v4i __attribute__((vector_size(16)));
 typedef int sdbfx_v16i __attribute__((vector_size(64)));
 
 #define MF_B 4                                  // 32-wide blocks per tile side
-#define MF_CAND_CAP (MF_B * MF_B * 16)         // result registers per tile
+#define MF_CAND_CAP 768                         // candidates listed per tile
 
 __device__ __forceinline__ uint32_t word_of(const uint4& v, int k)
 {
@@ -1131,24 +1131,16 @@ __device__ __forceinline__ sdbfx_v4i planes4(uint32_t w, int sh)
     return r;
 }
 
-// Compiler controls for the MFMA loop; they change no result.  acc_read
-// copies one accumulator (an AGPR) to a VGPR where it is used, so the
-// compiler cannot copy all 256 out at once; opaque() hides a value's origin so
-// that a prefetch is not merged with the load it feeds.  The CPU emulation of
-// the kernels (tests/hipemu, which defines HIPEMU) uses the plain forms.
-// in_agprs() places a 32x32 block of accumulators in AGPRs at that point,
-// so that its initial values are not held in VGPRs up to the loop.
+// Compiler controls for the MFMA loop; they change no result.  opaque()
+// hides a value's origin so that a prefetch is not merged with the load it
+// feeds; in_agprs() places a 32x32 block of accumulators in AGPRs at that
+// point, so that its initial values are not held in VGPRs up to the loop.
+// The CPU emulation of the kernels (tests/hipemu, which defines HIPEMU) uses
+// the plain forms.
 #if defined(HIPEMU)
-__device__ __forceinline__ uint32_t acc_read(int x) { return (uint32_t)x; }
 __device__ __forceinline__ void opaque(uint32_t&) {}
 __device__ __forceinline__ void in_agprs(sdbfx_v16i&) {}
 #else
-__device__ __forceinline__ uint32_t acc_read(int x)
-{
-    uint32_t v;
-    asm volatile("v_accvgpr_read_b32 %0, %1" : "=v"(v) : "a"(x));
-    return v;
-}
 __device__ __forceinline__ void opaque(uint32_t& x) { asm volatile("" : "+v"(x)); }
 __device__ __forceinline__ void in_agprs(sdbfx_v16i& v) { asm volatile("" : "+a"(v)); }
 #endif
@@ -1169,13 +1161,12 @@ __device__ __forceinline__ uint32_t size_code(uint32_t s)
     return s >= SDBFX_MIN_ELEM ? min(s, 1025u) : 2048u;
 }
 
-// bf_bitcount_cut_256 (sdbf/bf_utils.cc:115-206) and the score fraction of
-// sdbf_max_score (sdbf_core.cc:418-440) for one pair of qualifying filters
-// with the given cutoff: true with num/den when the match exceeds it.  The
-// reference filter's 16-byte chunk q is ra[q ^ sw] (the swizzled LDS tile).
-__device__ __forceinline__ bool pair_fraction(const uint4* ra, uint32_t sw, const uint4* __restrict__ fb,
-                                              uint32_t cutoff, uint32_t e1, uint32_t e2, uint32_t& num,
-                                              uint32_t& den)
+// AND-popcounts of the first 32, 64, 128 and all 256 bytes of a reference
+// filter and a target filter, as bf_bitcount_cut_256 (sdbf/bf_utils.cc:115-206)
+// takes them.  The reference filter's 16-byte chunk q is ra[q ^ sw] (the
+// swizzled LDS tile).
+__device__ __forceinline__ void prefix_counts(const uint4* ra, uint32_t sw, const uint4* __restrict__ fb,
+                                              uint32_t& p32, uint32_t& p64, uint32_t& p128, uint32_t& p256)
 {
     uint32_t c[4] = {0, 0, 0, 0};
 #pragma unroll
@@ -1185,36 +1176,31 @@ __device__ __forceinline__ bool pair_fraction(const uint4* ra, uint32_t sw, cons
         c[q < 2 ? 0 : q < 4 ? 1 : q < 8 ? 2 : 3] +=
             __popc(x.x & y.x) + __popc(x.y & y.y) + __popc(x.z & y.z) + __popc(x.w & y.w);
     }
-    const uint32_t p32 = c[0], p64 = p32 + c[1], p128 = p64 + c[2], p256 = p128 + c[3];
+    p32 = c[0];
+    p64 = p32 + c[1];
+    p128 = p64 + c[2];
+    p256 = p128 + c[3];
+}
+
+// The early-exit checks of bf_bitcount_cut_256 and the fraction of
+// sdbf_max_score (sdbf_core.cc:418-440): true with num/den when the pair
+// passes and its match exceeds the cutoff, merged into *best as a 64-bit
+// key floor(num * 2^22 / den) << 32 | num << 16 | den.
+__device__ __forceinline__ void merge_pair(uint32_t p32, uint32_t p64, uint32_t p128, uint32_t p256,
+                                           uint32_t cutoff, uint32_t e1, uint32_t e2, unsigned long long* best)
+{
     const bool pass = (8 * p32 + 48 >= cutoff) && (4 * p64 + 48 >= cutoff) && (2 * p128 + 48 >= cutoff);
     const uint32_t match = pass ? p256 : 0u;
     if (match <= cutoff)
-        return false;
-    num = match - cutoff;
-    den = min(e1, e2) - cutoff;
-    return true;
+        return;
+    const uint32_t num = match - cutoff, den = min(e1, e2) - cutoff;
+    const unsigned long long key = ((unsigned long long)num << 22) / den;
+    atomicMax(best, key << 32 | (unsigned long long)(num << 16 | den));
 }
 
-// position of the r-th (from 0) set bit of m; m has more than r set bits
-__device__ __forceinline__ uint32_t select_bit(unsigned long long m, uint32_t r)
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long m)
 {
-    uint32_t pos = 0, x = (uint32_t)m;
-    uint32_t c = __popc(x);
-    if (r >= c) {
-        r -= c;
-        x = (uint32_t)(m >> 32);
-        pos = 32;
-    }
-#pragma unroll
-    for (int w = 16; w >= 1; w >>= 1) {
-        c = __popc(x & ((1u << w) - 1));
-        if (r >= c) {
-            r -= c;
-            x >>= w;
-            pos += w;
-        }
-    }
-    return pos;
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
 // Per-wave LDS of cmp_mfma_kernel.
@@ -1225,8 +1211,7 @@ struct MfmaWaveSmem {
     uint32_t rinfo[SDBFX_MROWS];                    // size_code(s1) | e1 << 16
     const uint32_t* rptr[SDBFX_MROWS];              // reference filter of each row
     unsigned long long best[SDBFX_MROWS];
-    uint32_t creg[MF_CAND_CAP];
-    unsigned long long cmask[MF_CAND_CAP];
+    uint32_t clist[MF_CAND_CAP];    // row | column << 7
 };
 
 // LDS written by some lanes of a wave, then read by others
@@ -1242,12 +1227,12 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
     SetView A, SetView B, const CmpGroup* __restrict__ groups, uint32_t ngroups,
     const CmpPair* __restrict__ pairs, uint64_t nwork, uint32_t* __restrict__ partial)
 {
-    __shared__ int16_t ncut[4097];                  // -cutoff by size_code(s1) + size_code(s2)
+    __shared__ int16_t ncut[4097];                  // -(cutoff + 1) by size_code(s1) + size_code(s2)
     __shared__ MfmaWaveSmem wsm[4];
 
     for (int s = threadIdx.x; s < 4097; s += 256) {
         const uint32_t idx = s >= 28 ? 4096u / (uint32_t)s : 0u;
-        ncut[s] = s > 2050 ? (int16_t)-32768 : s > 1024 ? (int16_t)-1250 : (int16_t)-(int)c_cut[idx];
+        ncut[s] = s > 2050 ? (int16_t)-32768 : s > 1024 ? (int16_t)-1251 : (int16_t)(-1 - (int)c_cut[idx]);
     }
     __syncthreads();
 
@@ -1257,8 +1242,7 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
     uint32_t* const rinfo = wsm[wv].rinfo;
     const uint32_t** const rptr = wsm[wv].rptr;
     unsigned long long* const best = wsm[wv].best;
-    uint32_t* const creg = wsm[wv].creg;
-    unsigned long long* const cmask = wsm[wv].cmask;
+    uint32_t* const clist = wsm[wv].clist;
 
     // each XCD (workgroups are dealt round-robin over 8) takes a contiguous
     // range of work items, so neighbouring items share its L2
@@ -1336,8 +1320,8 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
             }
             const char* ncutb = (const char*)ncut;
 
-            // accumulators start at -cutoff of their pair, so that after the
-            // 2048 bits they hold count - cutoff
+            // accumulators start at -(cutoff + 1) of their pair, so that
+            // after the 2048 bits they are >= 0 exactly when count > cutoff
             sdbfx_v16i acc[MF_B][MF_B];
             uint32_t rb = 8 * h;
             opaque(rb);
@@ -1402,85 +1386,77 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
                 }
             }
 
-            // candidates: count - cutoff > 0, block maxima first
-            int bmax[MF_B * MF_B];
-            int mx = -1;
+            // candidates: accumulators >= 0, one bit per accumulator, from
+            // the sign bits (shifted in by v_alignbit).  Word w holds blocks
+            // 2w (bits 31..16, register i at bit 31 - i) and 2w + 1 (bits
+            // 15..0, register i at bit 15 - i).
+            uint32_t cbits[MF_B * MF_B / 2];
+            uint32_t anyc = 0;
 #pragma unroll
-            for (int bm = 0; bm < MF_B; bm++)
+            for (int w = 0; w < MF_B * MF_B / 2; w++) {
+                uint32_t x = 0;
 #pragma unroll
-                for (int bn = 0; bn < MF_B; bn++) {
-                    int x = -1;
+                for (int k = 0; k < 2; k++) {
+                    const int bm = (2 * w + k) / MF_B, bn = (2 * w + k) % MF_B;
 #pragma unroll
                     for (int i = 0; i < 16; i++)
-                        x = max(x, acc[bm][bn][i]);
-                    bmax[bm * MF_B + bn] = x;
-                    mx = max(mx, x);
+                        x = __builtin_amdgcn_alignbit(x, (uint32_t)acc[bm][bn][i], 31);
                 }
-            if (__ballot(mx > 0) == 0)
+                cbits[w] = ~x;
+                anyc |= ~x;
+            }
+            if (__ballot(anyc != 0) == 0)
                 continue;
-            uint32_t ncr = 0;
+            // each lane lists its own candidates after those of lower lanes
+            uint32_t own = 0;
 #pragma unroll
-            for (int bm = 0; bm < MF_B; bm++)
+            for (int w = 0; w < MF_B * MF_B / 2; w++)
+                own += (uint32_t)__popc(cbits[w]);
+            uint32_t ncand;
+            uint32_t pos = wave_excl_sum(own, lane, &ncand);
+            if (ncand <= MF_CAND_CAP) {
 #pragma unroll
-                for (int bn = 0; bn < MF_B; bn++) {
-                    if (__ballot(bmax[bm * MF_B + bn] > 0) == 0)
-                        continue;
-#pragma unroll
-                    for (int i = 0; i < 16; i++) {
-                        const unsigned long long m = __ballot((int)acc_read(acc[bm][bn][i]) > 0);
-                        if (m) {
-                            if (lane == 0) {
-                                creg[ncr] = (uint32_t)((bm * MF_B + bn) * 16 + i);
-                                cmask[ncr] = m;
-                            }
-                            ncr++;
-                        }
+                for (int w = 0; w < MF_B * MF_B / 2; w++) {
+                    uint32_t x = cbits[w];
+                    while (x) {
+                        const uint32_t bit = 31u - (uint32_t)__clz(x);
+                        x ^= 1u << bit;
+                        const uint32_t blk = 2 * w + (bit < 16 ? 1 : 0), i = (bit < 16 ? 15u : 31u) - bit;
+                        const uint32_t row = 32 * (blk / MF_B) + 8 * (i >> 2) + 4 * h + (i & 3);
+                        clist[pos++] = row | (32 * (blk % MF_B) + r32) << 7;
                     }
                 }
-            wave_lds_sync();
-            // expand the (register, lane mask) entries into one candidate per
-            // lane, 64 at a time: creg[k] gets the entry's first candidate
-            // index in its upper bits (ncr <= 256, 4 entries per lane)
-            uint32_t pc[4], own = 0;
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const uint32_t k = 4 * lane + e;
-                pc[e] = k < ncr ? (uint32_t)__popcll(cmask[k]) : 0u;
-                own += pc[e];
-            }
-            uint32_t total;
-            uint32_t first = wave_excl_sum(own, lane, &total);
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const uint32_t k = 4 * lane + e;
-                if (k < ncr)
-                    creg[k] |= first << 8;
-                first += pc[e];
             }
             wave_lds_sync();
-            for (uint32_t base = 0; base < total; base += 64) {
-                const uint32_t idx = base + lane;
-                if (idx < total) {
-                    uint32_t lo = 0, hi = ncr;
-                    while (hi - lo > 1) {
-                        const uint32_t mid = (lo + hi) >> 1;
-                        if ((creg[mid] >> 8) <= idx)
-                            lo = mid;
-                        else
-                            hi = mid;
+            if (ncand <= MF_CAND_CAP) {
+                // one candidate per lane
+                for (uint32_t base = 0; base < ncand; base += 64) {
+                    if (base + lane < ncand) {
+                        const uint32_t e = clist[base + lane];
+                        const uint32_t row = e & 127u, col = e >> 7;
+                        const uint64_t j = nt + col;
+                        const uint32_t ri = rinfo[row];
+                        const uint32_t cutoff = (uint32_t)(-1 - (int)ncut[(ri & 0xFFFFu) + size_code(tec[j])]);
+                        uint32_t p32, p64, p128, p256;
+                        prefix_counts(atile + row * 16, row & 15, (const uint4*)(tf + j * SDBFX_BFW), p32,
+                                            p64, p128, p256);
+                        merge_pair(p32, p64, p128, p256, cutoff, ri >> 16, tham[j], &best[row]);
                     }
-                    const uint32_t r = creg[lo] & 0xFFu;
-                    const uint32_t src = select_bit(cmask[lo], idx - (creg[lo] >> 8));
-                    const uint32_t bm = r >> 6, bn = (r >> 4) & 3, i = r & 15;
-                    const uint32_t row = 32 * bm + 8 * (i >> 2) + 4 * (src >> 5) + (i & 3);
-                    const uint64_t j = nt + 32 * bn + (src & 31);
+                }
+            } else {
+                // too many to list: every qualifying pair of the tile, exactly
+                for (uint32_t base = 0; base < 128u * 128u; base += 64) {
+                    const uint32_t row = (base + lane) >> 7, col = (base + lane) & 127u;
                     const uint32_t ri = rinfo[row];
-                    const uint32_t cutoff = (uint32_t)-(int)ncut[(ri & 0xFFFFu) + size_code(tec[j])];
-                    uint32_t num, den;
-                    if (pair_fraction(atile + row * 16, row & 15, (const uint4*)(tf + j * SDBFX_BFW), cutoff,
-                                      ri >> 16, tham[j], num, den)) {
-                        const unsigned long long key = ((unsigned long long)num << 22) / den;
-                        atomicMax(&best[row], key << 32 | (unsigned long long)(num << 16 | den));
+                    if (col < ncol) {
+                        const uint64_t j = nt + col;
+                        const uint32_t t = (uint32_t)(-1 - (int)ncut[(ri & 0xFFFFu) + size_code(tec[j])]);
+                        if (t < 0x7FFFu) {
+                            uint32_t p32, p64, p128, p256;
+                            prefix_counts(atile + row * 16, row & 15, (const uint4*)(tf + j * SDBFX_BFW), p32,
+                                                p64, p128, p256);
+                            merge_pair(p32, p64, p128, p256, t, ri >> 16, tham[j], &best[row]);
+                        }
                     }
                 }
             }

@@ -108,12 +108,20 @@ def main(lib):
         gd.add(x.name, x.orig_size, x.bf_count, x.last_count, 0, 160, x.filters, None)
     want_dup = oracle.compare_all(dups)
     assert d.bf_count > 20 and np.all(want_dup == 100)
+    # one filter repeated: every pair of a tile is a candidate (listing overflows)
+    one = d.filters[:256]
+    reps = [oracle.Digest(b"r%d" % i, np.tile(one, n), n, 160, None, n * 40000, 0) for i, n in enumerate((150, 140))]
+    gr = eng.new_set()
+    for x in reps:
+        gr.add(x.name, x.orig_size, x.bf_count, x.last_count, 0, 160, x.filters, None)
+    want_rep = oracle.compare_all(reps)
     for method in (sdhash_amd.CMP_MFMA, sdhash_amd.CMP_VALU):
         eng.compare_method(method)
         assert np.array_equal(eng.compare_all(s), want_all), method
         assert np.array_equal(eng.compare_to(gq, gt), want), method
         assert np.array_equal(eng.compare_to(gq2, gt), want2), method
         assert np.array_equal(eng.compare_all(gd), want_dup), method
+        assert np.array_equal(eng.compare_all(gr), want_rep), method
         print("compare method %d ok" % method, flush=True)
     print("compare ok", flush=True)
     eng.close()

@@ -381,14 +381,32 @@ inline hipemu_v16i __builtin_amdgcn_mfma_i32_32x32x32_i8(hipemu_v4i a, hipemu_v4
     return d;
 }
 
+inline uint32_t __builtin_amdgcn_mbcnt_lo(uint32_t m, uint32_t base)
+{
+    const int l = hipemu::lane();
+    const uint32_t below = l >= 32 ? 0xFFFFFFFFu : ((1u << l) - 1u);
+    return base + (uint32_t)__builtin_popcount(m & below);
+}
+inline uint32_t __builtin_amdgcn_mbcnt_hi(uint32_t m, uint32_t base)
+{
+    const int l = hipemu::lane();
+    const uint32_t below = l < 32 ? 0u : ((1u << (l - 32)) - 1u);
+    return base + (uint32_t)__builtin_popcount(m & below);
+}
+
 inline uint32_t __lane_id() { return (uint32_t)hipemu::lane(); }
 inline int __builtin_amdgcn_readlane(int v, int l) { return (int)hipemu::exchange((uint32_t)v, l); }
 inline int __builtin_amdgcn_readfirstlane(int v) { return (int)hipemu::exchange((uint32_t)v, 0); }
+inline uint32_t __builtin_amdgcn_alignbit(uint32_t hi, uint32_t lo, uint32_t s)
+{
+    return (uint32_t)((((uint64_t)hi << 32) | lo) >> (s & 31));
+}
 inline uint32_t __builtin_amdgcn_alignbyte(uint32_t hi, uint32_t lo, uint32_t s)
 {
     return (uint32_t)((((uint64_t)hi << 32) | lo) >> (8 * (s & 3)));
 }
 inline int __popc(uint32_t x) { return __builtin_popcount(x); }
+inline int __clz(uint32_t x) { return x ? __builtin_clz(x) : 32; }
 inline uint64_t __builtin_amdgcn_s_memrealtime() { return 0; }
 inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v)
 {

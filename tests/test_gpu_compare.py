@@ -128,3 +128,16 @@ def test_compare_to_many_rows(cmp_engine):
     got = cmp_engine.compare_to(upload_oracle(cmp_engine, q), upload_oracle(cmp_engine, t))
     assert np.array_equal(got, want), np.argwhere(got != want)[:10]
     assert (want > 0).sum() > 100
+
+
+def test_compare_repeated_filter(cmp_engine):
+    # one filter repeated in two digests: every pair of a 128 x 128 tile
+    # qualifies (the MFMA kernel then re-checks the whole tile exactly)
+    d = oracle.digest_stream(synth.file_set(40, 1, 256 * KIB)[0], b"x")
+    one = d.filters[:256]
+    reps = [oracle.Digest(b"r%d" % i, np.tile(one, n), n, 160, None, n * 40000, 0)
+            for i, n in enumerate((300, 260, 129))]
+    s = upload_oracle(cmp_engine, reps)
+    want = oracle.compare_all(reps)
+    assert np.array_equal(cmp_engine.compare_all(s), want)
+    assert np.all(want == 100)
