@@ -68,13 +68,47 @@ def cpu_baseline(img, frags, seg_names):
     }
 
 
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "bench_block_search_1GiB_hbm_traffic.md")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by
+    the copy calibration) from the committed rocprofv3 --pmc summary, or None."""
+    try:
+        with open(TRAFFIC_TABLE) as f:
+            for line in f:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) >= 6 and cells[0] == kernel:
+                    return float(cells[5])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
+def copy_probe() -> int:
+    """A plain 256 MiB device-to-device copy, three times: the reference
+    point for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    a = torch.ones(256 * MIB, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print("copied 3 x %d bytes" % a.numel())
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 256 MiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X "
               "(multi-GPU sharding is a design note, not a measured path)", file=sys.stderr)
@@ -178,9 +212,15 @@ def main() -> int:
         "kernel_ms_per_step": {n: v[0] / args.steps for n, v in k.items()},
         "compare_forms": forms,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic("block_digest_kernel"),
                      "kernel": "block_digest_kernel",
                      "algorithmic_bytes_per_launch": IMAGE * args.steps // max(blk_n, 1)},
+        # the kernel behind `value`: 2048 int8 MACs (4096 ops) per filter pair
+        "roofline_compare": {"bound": "mfma", "kernel": "cmp_mfma_kernel", "unit": "TOPS",
+                             "achieved": forms["mfma"]["pairs_per_s"] * 4096 / 1e12, "peak": 5000.0,
+                             "frac": forms["mfma"]["pairs_per_s"] * 4096 / 1e12 / 5000.0,
+                             "traffic": measured_traffic("cmp_mfma_kernel"),
+                             "algorithmic_ops_per_launch": pairs * 4096},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(img, frags, seg_names)

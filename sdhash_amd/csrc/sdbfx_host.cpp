@@ -365,7 +365,7 @@ int digest_common(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t n, const uint8_t* d_b
     if (!ctx || !set || (n && (!lens || !status)))
         return fail(ctx, "null argument");
     for (uint32_t i = 0; i < n; i++) {
-        if (dd && dd[i] > SDBFX_TILE + 64)
+        if (dd && dd[i] > SDBFX_TILE)
             return fail(ctx, "dd block size above 16384");
         if (dd && dd[i] && dd[i] < SDBFX_MIN_FILE)
             return fail(ctx, "dd block size below 512");
