@@ -136,6 +136,41 @@ int sdbfx_compare_to(sdbfx_ctx *ctx, const sdbfx_set *q, const sdbfx_set *t, int
 #define SDBFX_CMP_MFMA 2
 int sdbfx_set_compare_method(sdbfx_ctx *ctx, int method);
 
+/* ---- serving -------------------------------------------------------- */
+
+/*
+ * A query server over a resident reference set (continuous batching).
+ * Queries submitted while earlier ones are on the GPU are queued and digested
+ * together as the next batch (arrival order, at most batch_bytes of input and
+ * batch_max queries per batch; a larger query goes alone); batch k+1 is
+ * digested on one HIP stream while batch k is compared with the reference
+ * set on another.  The server owns two contexts on `device`; `ref` must stay
+ * alive and unchanged while the server exists.  Each result is
+ * q.compare(ref[j]) for every digest j of `ref`, the same whether the query
+ * was batched with others or alone.  Replaces the per-request hashString /
+ * compareTwo path of sdhash-server (sdhash-srv-main.cc:224-402).
+ */
+typedef struct sdbfx_server sdbfx_server;
+int sdbfx_serve_create(int device, const sdbfx_set *ref, uint64_t batch_bytes, uint32_t batch_max,
+                       sdbfx_server **out);
+/* Waits for the batches already past the queue; queued requests are dropped. */
+void sdbfx_serve_destroy(sdbfx_server *s);
+const char *sdbfx_serve_error(sdbfx_server *s);
+uint32_t sdbfx_serve_ref_size(const sdbfx_server *s);
+/* Copies the query; dd_block as in sdbfx_set_digest (0 = stream mode). */
+int sdbfx_serve_submit(sdbfx_server *s, const uint8_t *buf, uint64_t len, const char *name, uint32_t dd_block,
+                       uint64_t *ticket);
+/* *done = 1 when the result is ready; then scores (sdbfx_serve_ref_size
+ * ints), status (0, or -3 for a query under 512 bytes, -1 on a device
+ * error) and the submit-to-result latency are written and the ticket is
+ * released.  Any output pointer may be NULL. */
+int sdbfx_serve_poll(sdbfx_server *s, uint64_t ticket, int32_t *scores, int32_t *status, uint64_t *latency_ns,
+                     int *done);
+/* As poll, blocking until the result is ready. */
+int sdbfx_serve_wait(sdbfx_server *s, uint64_t ticket, int32_t *scores, int32_t *status, uint64_t *latency_ns);
+/* Batches and queries completed so far. */
+int sdbfx_serve_stats(sdbfx_server *s, uint64_t *batches, uint64_t *queries);
+
 /* ---- device memory for device-resident inputs ------------------------ */
 
 /* Allocates `bytes` of device memory plus 256 readable padding bytes. */

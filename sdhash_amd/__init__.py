@@ -22,6 +22,8 @@ SYMBOLS = (
     "sdbfx_set_read", "sdbfx_compare_all", "sdbfx_compare_to", "sdbfx_device_alloc",
     "sdbfx_device_free", "sdbfx_upload", "sdbfx_set_profiling", "sdbfx_kernel_time",
     "sdbfx_reset_kernel_times", "sdbfx_set_compare_method",
+    "sdbfx_serve_create", "sdbfx_serve_destroy", "sdbfx_serve_error", "sdbfx_serve_ref_size",
+    "sdbfx_serve_submit", "sdbfx_serve_poll", "sdbfx_serve_wait", "sdbfx_serve_stats",
 )
 
 # comparison kernels (sdbfx_set_compare_method)
@@ -75,13 +77,25 @@ def load():
     L.sdbfx_kernel_time.argtypes = [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(U64)]
     L.sdbfx_reset_kernel_times.argtypes = [P]
     L.sdbfx_set_compare_method.argtypes = [P, C.c_int]
+    L.sdbfx_serve_create.argtypes = [C.c_int, P, U64, U32, C.POINTER(P)]
+    L.sdbfx_serve_destroy.argtypes = [P]
+    L.sdbfx_serve_destroy.restype = None
+    L.sdbfx_serve_error.argtypes = [P]
+    L.sdbfx_serve_error.restype = C.c_char_p
+    L.sdbfx_serve_ref_size.argtypes = [P]
+    L.sdbfx_serve_ref_size.restype = U32
+    L.sdbfx_serve_submit.argtypes = [P, P, U64, C.c_char_p, U32, C.POINTER(U64)]
+    L.sdbfx_serve_poll.argtypes = [P, U64, P, C.POINTER(I32), C.POINTER(U64), C.POINTER(C.c_int)]
+    L.sdbfx_serve_wait.argtypes = [P, U64, P, C.POINTER(I32), C.POINTER(U64)]
+    L.sdbfx_serve_stats.argtypes = [P, C.POINTER(U64), C.POINTER(U64)]
     for name in SYMBOLS:
         getattr(L, name).restype = getattr(L, name).restype or I32
     for name in ("sdbfx_ctx_create", "sdbfx_sync", "sdbfx_set_create", "sdbfx_set_digest",
                  "sdbfx_set_digest_device", "sdbfx_set_add", "sdbfx_set_info", "sdbfx_set_read",
                  "sdbfx_compare_all", "sdbfx_compare_to", "sdbfx_device_alloc", "sdbfx_device_free",
                  "sdbfx_upload", "sdbfx_set_profiling", "sdbfx_kernel_time",
-                 "sdbfx_reset_kernel_times", "sdbfx_set_compare_method"):
+                 "sdbfx_reset_kernel_times", "sdbfx_set_compare_method", "sdbfx_serve_create",
+                 "sdbfx_serve_submit", "sdbfx_serve_poll", "sdbfx_serve_wait", "sdbfx_serve_stats"):
         getattr(L, name).restype = I32
     _lib = L
     return L
@@ -272,3 +286,70 @@ class Engine:
         if self.ctx:
             self.lib.sdbfx_ctx_destroy(self.ctx)
             self.ctx = None
+
+
+@dataclass
+class ServeResult:
+    status: int                 # 0, -3 (query under 512 bytes) or -1 (device error)
+    scores: np.ndarray          # int32 per reference digest (empty unless status == 0)
+    latency_s: float            # submit to result
+
+
+class Server:
+    """Continuous-batching query server over a resident reference set
+    (sdbfx_serve_*): submit() returns a ticket at once, wait()/poll() give
+    the query's scores against every reference digest."""
+
+    def __init__(self, ref: DigestSet, device: int = 0, batch_bytes: int = 256 << 20,
+                 batch_max: int = 1024):
+        self.lib = load()
+        self.ref = ref                      # must outlive the server
+        p = C.c_void_p()
+        if self.lib.sdbfx_serve_create(device, ref.handle, batch_bytes, batch_max, C.byref(p)) != 0:
+            raise SdbfxError("cannot create the server")
+        self.handle = p
+        self.nref = int(self.lib.sdbfx_serve_ref_size(p))
+
+    def submit(self, data, name: bytes = b"query", dd_block: int = 0) -> int:
+        a = np.ascontiguousarray(np.frombuffer(data, np.uint8) if isinstance(data, (bytes, bytearray))
+                                 else data, dtype=np.uint8)
+        t = C.c_uint64()
+        if self.lib.sdbfx_serve_submit(self.handle, _ptr(a), a.size, name, dd_block, C.byref(t)) != 0:
+            raise SdbfxError(self.lib.sdbfx_serve_error(self.handle).decode() or "submit failed")
+        return t.value
+
+    def _result(self, fn, ticket, *extra):
+        sc = np.zeros(max(self.nref, 1), np.int32)
+        st = C.c_int32()
+        lat = C.c_uint64()
+        rc = fn(self.handle, ticket, _ptr(sc), C.byref(st), C.byref(lat), *extra)
+        if rc != 0:
+            raise SdbfxError("unknown ticket %d" % ticket)
+        return ServeResult(st.value, sc[: self.nref] if st.value == 0 else sc[:0], lat.value / 1e9)
+
+    def wait(self, ticket: int) -> ServeResult:
+        return self._result(self.lib.sdbfx_serve_wait, ticket)
+
+    def poll(self, ticket: int):
+        done = C.c_int()
+        r = self._result(self.lib.sdbfx_serve_poll, ticket, C.byref(done))
+        return r if done.value else None
+
+    def stats(self) -> tuple[int, int]:
+        b, q = C.c_uint64(), C.c_uint64()
+        self.lib.sdbfx_serve_stats(self.handle, C.byref(b), C.byref(q))
+        return b.value, q.value
+
+    def error(self) -> str:
+        return self.lib.sdbfx_serve_error(self.handle).decode()
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.sdbfx_serve_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

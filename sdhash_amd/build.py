@@ -38,8 +38,8 @@ def _run(cmd, verbose):
 
 def build(verbose: bool = False) -> str:
     # the engine: kernels + C ABI
-    _run([HIPCC, *FLAGS, "-I", INCLUDE, "-x", "hip", os.path.join(CSRC, "sdbfx_host.cpp"), "-o",
-          LIB + ".tmp"], verbose)
+    _run([HIPCC, *FLAGS, "-I", INCLUDE, "-x", "hip", os.path.join(CSRC, "sdbfx_host.cpp"),
+          os.path.join(CSRC, "sdbfx_serve.cpp"), "-o", LIB + ".tmp"], verbose)
     os.replace(LIB + ".tmp", LIB)
     # the C++ sdbf / sdbf_set classes over the C ABI, and the sdhash CLI
     cxx = ["g++", "-std=c++17", "-O2", "-fPIC", "-Wall", "-I", INCLUDE]

@@ -1,0 +1,311 @@
+// Query serving over a resident reference set: continuous batching on two
+// HIP streams.  Built on the engine's C ABI only.
+//
+// Requests are queued by sdbfx_serve_submit.  A digest thread takes whatever
+// is queued (in arrival order, up to batch_bytes of input and batch_max
+// queries; a larger query goes alone) and digests it as one batch on its own
+// context; the compare thread compares each digested batch with the
+// reference set on a second context.  While batch k is compared, batch k+1
+// is digested, and requests that arrive meanwhile form batch k+2.  At most
+// two batches are past the queue at a time, which bounds device memory to
+// about two batches plus the reference set.
+//
+// The reference counterpart is sdhash-server's handler (sdhash-srv-main.cc:
+// 224-402), which hashes and compares each request on its own.
+#include <stdint.h>
+#include <string.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "sdbfx.h"
+
+namespace {
+
+uint64_t now_ns()
+{
+    return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+struct Request {
+    uint64_t ticket = 0;
+    std::vector<uint8_t> data;
+    std::string name;
+    uint32_t dd = 0;
+    uint64_t t_submit = 0, t_done = 0;
+    bool done = false;
+    int32_t status = 0;
+    std::vector<int32_t> scores;
+};
+
+struct Batch {
+    std::vector<std::shared_ptr<Request>> reqs;
+    sdbfx_set* set = nullptr;      // digests of the requests with status 0, in order
+    std::vector<int32_t> status;
+};
+
+}  // namespace
+
+struct sdbfx_server {
+    const sdbfx_set* ref = nullptr;
+    uint32_t nref = 0;
+    uint64_t batch_bytes = 0;
+    uint32_t batch_max = 0;
+    sdbfx_ctx* cd = nullptr;   // digest context
+    sdbfx_ctx* cc = nullptr;   // compare context
+
+    std::mutex mu;
+    std::condition_variable cv_queue, cv_handoff, cv_done;
+    std::deque<std::shared_ptr<Request>> queue;
+    std::deque<Batch> handoff;
+    bool handoff_busy = false;  // the compare thread holds a batch
+    std::unordered_map<uint64_t, std::shared_ptr<Request>> live;
+    uint64_t next_ticket = 1;
+    bool stop = false;
+    std::string err;
+    uint64_t nbatches = 0, nqueries = 0;
+    std::thread td, tc;
+
+    void set_error(const std::string& e)
+    {
+        std::lock_guard<std::mutex> g(mu);
+        if (err.empty())
+            err = e;
+    }
+
+    void digest_loop()
+    {
+        for (;;) {
+            Batch b;
+            {
+                std::unique_lock<std::mutex> g(mu);
+                // a new batch only when the compare stage has room for it
+                cv_queue.wait(g, [&] { return stop || (!queue.empty() && handoff.size() < 1); });
+                if (stop)
+                    return;
+                uint64_t bytes = 0;
+                while (!queue.empty() && b.reqs.size() < batch_max) {
+                    const auto& r = queue.front();
+                    if (!b.reqs.empty() && bytes + r->data.size() > batch_bytes)
+                        break;
+                    bytes += r->data.size();
+                    b.reqs.push_back(r);
+                    queue.pop_front();
+                }
+            }
+            const uint32_t n = (uint32_t)b.reqs.size();
+            std::vector<const uint8_t*> bufs(n);
+            std::vector<uint64_t> lens(n);
+            std::vector<const char*> names(n);
+            std::vector<uint32_t> dd(n);
+            for (uint32_t i = 0; i < n; i++) {
+                bufs[i] = b.reqs[i]->data.data();
+                lens[i] = b.reqs[i]->data.size();
+                names[i] = b.reqs[i]->name.c_str();
+                dd[i] = b.reqs[i]->dd;
+            }
+            b.status.assign(n, 0);
+            int rc = sdbfx_set_create(cd, &b.set);
+            if (rc == 0)
+                rc = sdbfx_set_digest(cd, b.set, n, bufs.data(), lens.data(), names.data(), dd.data(),
+                                      b.status.data());
+            if (rc != 0) {
+                set_error(std::string("digest: ") + sdbfx_last_error(cd));
+                for (uint32_t i = 0; i < n; i++)
+                    b.status[i] = -1;
+            }
+            for (auto& r : b.reqs)
+                std::vector<uint8_t>().swap(r->data);   // input no longer needed
+            std::lock_guard<std::mutex> g(mu);
+            handoff.push_back(std::move(b));
+            cv_handoff.notify_one();
+        }
+    }
+
+    void compare_loop()
+    {
+        for (;;) {
+            Batch b;
+            {
+                std::unique_lock<std::mutex> g(mu);
+                cv_handoff.wait(g, [&] { return stop || !handoff.empty(); });
+                if (stop && handoff.empty())
+                    return;
+                b = std::move(handoff.front());
+                handoff.pop_front();
+                cv_queue.notify_one();   // the digest stage may start the next batch
+            }
+            const uint32_t n = (uint32_t)b.reqs.size();
+            const uint32_t nd = b.set ? sdbfx_set_size(b.set) : 0;
+            std::vector<int32_t> sc((size_t)nd * nref);
+            int rc = 0;
+            if (nd && nref)
+                rc = sdbfx_compare_to(cc, b.set, ref, sc.data());
+            if (rc != 0)
+                set_error(std::string("compare: ") + sdbfx_last_error(cc));
+            const uint64_t t = now_ns();
+            {
+                std::lock_guard<std::mutex> g(mu);
+                uint32_t k = 0;   // row of the next digested request
+                for (uint32_t i = 0; i < n; i++) {
+                    Request& r = *b.reqs[i];
+                    r.status = rc != 0 ? -1 : b.status[i];
+                    if (b.status[i] == 0) {
+                        if (rc == 0)
+                            r.scores.assign(sc.begin() + (size_t)k * nref, sc.begin() + (size_t)(k + 1) * nref);
+                        k++;
+                    }
+                    r.t_done = t;
+                    r.done = true;
+                }
+                nbatches++;
+                nqueries += n;
+            }
+            cv_done.notify_all();
+            if (b.set)
+                sdbfx_set_destroy(b.set);
+        }
+    }
+};
+
+extern "C" {
+
+int sdbfx_serve_create(int device, const sdbfx_set* ref, uint64_t batch_bytes, uint32_t batch_max,
+                       sdbfx_server** out)
+{
+    if (!out || !ref || batch_bytes == 0 || batch_max == 0)
+        return -1;
+    *out = nullptr;
+    auto* s = new sdbfx_server();
+    s->ref = ref;
+    s->nref = sdbfx_set_size(ref);
+    s->batch_bytes = batch_bytes;
+    s->batch_max = batch_max;
+    if (sdbfx_ctx_create(device, 0, &s->cd) != 0 || sdbfx_ctx_create(device, 0, &s->cc) != 0) {
+        if (s->cd)
+            sdbfx_ctx_destroy(s->cd);
+        delete s;
+        return -1;
+    }
+    s->td = std::thread([s] { s->digest_loop(); });
+    s->tc = std::thread([s] { s->compare_loop(); });
+    *out = s;
+    return 0;
+}
+
+void sdbfx_serve_destroy(sdbfx_server* s)
+{
+    if (!s)
+        return;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stop = true;
+    }
+    s->cv_queue.notify_all();
+    s->cv_handoff.notify_all();
+    s->td.join();
+    s->tc.join();
+    for (auto& b : s->handoff)
+        if (b.set)
+            sdbfx_set_destroy(b.set);
+    sdbfx_ctx_destroy(s->cd);
+    sdbfx_ctx_destroy(s->cc);
+    delete s;
+}
+
+const char* sdbfx_serve_error(sdbfx_server* s)
+{
+    if (!s)
+        return "null server";
+    std::lock_guard<std::mutex> g(s->mu);
+    return s->err.c_str();
+}
+
+uint32_t sdbfx_serve_ref_size(const sdbfx_server* s) { return s ? s->nref : 0; }
+
+int sdbfx_serve_submit(sdbfx_server* s, const uint8_t* buf, uint64_t len, const char* name, uint32_t dd_block,
+                       uint64_t* ticket)
+{
+    if (!s || (!buf && len) || !name || !ticket)
+        return -1;
+    auto r = std::make_shared<Request>();
+    r->data.assign(buf, buf + len);
+    r->name = name;
+    r->dd = dd_block;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->stop)
+            return -1;
+        r->ticket = s->next_ticket++;
+        r->t_submit = now_ns();
+        s->live[r->ticket] = r;
+        s->queue.push_back(r);
+        *ticket = r->ticket;
+    }
+    s->cv_queue.notify_one();
+    return 0;
+}
+
+// Shared by poll and wait: hands the result over and forgets the ticket.
+static int take(sdbfx_server* s, std::unordered_map<uint64_t, std::shared_ptr<Request>>::iterator it,
+                int32_t* scores, int32_t* status, uint64_t* latency_ns)
+{
+    Request& r = *it->second;
+    if (status)
+        *status = r.status;
+    if (latency_ns)
+        *latency_ns = r.t_done - r.t_submit;
+    if (scores && r.status == 0)
+        memcpy(scores, r.scores.data(), r.scores.size() * sizeof(int32_t));
+    s->live.erase(it);
+    return 0;
+}
+
+int sdbfx_serve_poll(sdbfx_server* s, uint64_t ticket, int32_t* scores, int32_t* status, uint64_t* latency_ns,
+                     int* done)
+{
+    if (!s || !done)
+        return -1;
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->live.find(ticket);
+    if (it == s->live.end())
+        return -1;
+    *done = it->second->done ? 1 : 0;
+    return *done ? take(s, it, scores, status, latency_ns) : 0;
+}
+
+int sdbfx_serve_wait(sdbfx_server* s, uint64_t ticket, int32_t* scores, int32_t* status, uint64_t* latency_ns)
+{
+    if (!s)
+        return -1;
+    std::unique_lock<std::mutex> g(s->mu);
+    auto it = s->live.find(ticket);
+    if (it == s->live.end())
+        return -1;
+    std::shared_ptr<Request> r = it->second;
+    s->cv_done.wait(g, [&] { return r->done; });
+    return take(s, s->live.find(ticket), scores, status, latency_ns);
+}
+
+int sdbfx_serve_stats(sdbfx_server* s, uint64_t* batches, uint64_t* queries)
+{
+    if (!s)
+        return -1;
+    std::lock_guard<std::mutex> g(s->mu);
+    if (batches)
+        *batches = s->nbatches;
+    if (queries)
+        *queries = s->nqueries;
+    return 0;
+}
+
+}  // extern "C"

@@ -124,6 +124,28 @@ def main(lib):
         assert np.array_equal(eng.compare_all(gr), want_rep), method
         print("compare method %d ok" % method, flush=True)
     print("compare ok", flush=True)
+
+    # serving: queries batched with others and alone give the oracle's scores
+    queries = [files[0][:30000], files[3], np.zeros(100, np.uint8), big[:20000], files[5][1000:21000]]
+    qdd = [0, 16384, 0, 0, 0]
+    want_q = [None if q.size < 512 else
+              oracle.compare_to([oracle.digest_block(q, d, b"q") if d else oracle.digest_stream(q, b"q")], ods)[0]
+              for q, d in zip(queries, qdd)]
+    srv = sdhash_amd.Server(s, batch_bytes=64 * KIB, batch_max=2)
+    tickets = [srv.submit(q, b"q%d" % i, d) for i, (q, d) in enumerate(zip(queries, qdd))]
+    mixed = [srv.wait(t) for t in tickets]
+    alone = []
+    for i, (q, d) in enumerate(zip(queries, qdd)):
+        alone.append(srv.wait(srv.submit(q, b"q%d" % i, d)))
+    nb, nq = srv.stats()
+    srv.close()
+    assert nq == 2 * len(queries) and nb < nq, (nb, nq)
+    for w, m, a in zip(want_q, mixed, alone):
+        if w is None:
+            assert m.status == -3 and a.status == -3
+        else:
+            assert m.status == 0 and np.array_equal(m.scores, w) and np.array_equal(a.scores, w), (m, a, w)
+    print("serve ok", flush=True)
     eng.close()
 
 

@@ -20,6 +20,7 @@
 #include <array>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -204,9 +205,18 @@ inline uint64_t ballot(bool p)
     __builtin_unreachable();
 }
 
+// One kernel at a time in the process: __shared__ arrays are function
+// statics here, so launches from several host threads are serialized.
+inline std::mutex& launch_mutex()
+{
+    static std::mutex m;
+    return m;
+}
+
 template <typename F>
 void launch(const dim3& grid, const dim3& block, F&& body)
 {
+    std::lock_guard<std::mutex> serialize(launch_mutex());
     const uint32_t nt = block.x;
     std::function<void()> fn = body;
     for (uint32_t bx = 0; bx < grid.x; bx++) {

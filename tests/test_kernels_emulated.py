@@ -20,7 +20,8 @@ def build(tmp, extra):
     subprocess.run(["g++", "-std=c++20", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer",
                     "-fPIC", "-shared", "-I", os.path.join(ROOT, "tests", "hipemu"),
                     "-I", os.path.join(ROOT, "include"), *extra, "-x", "c++",
-                    os.path.join(ROOT, "sdhash_amd", "csrc", "sdbfx_host.cpp"), "-o", out], check=True)
+                    os.path.join(ROOT, "sdhash_amd", "csrc", "sdbfx_host.cpp"),
+                    os.path.join(ROOT, "sdhash_amd", "csrc", "sdbfx_serve.cpp"), "-o", out], check=True)
     return out
 
 
@@ -32,6 +33,7 @@ def run(lib):
                        capture_output=True, text=True, timeout=900)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
     assert "compare ok" in r.stdout
+    assert "serve ok" in r.stdout
 
 
 @pytest.mark.parametrize("warm", [64, 12])
