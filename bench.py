@@ -85,6 +85,188 @@ def measured_traffic(kernel: str):
     return None
 
 
+def _hip():
+    import ctypes as C
+    h = C.CDLL("libamdhip64.so")
+    h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    h.hipMemcpy.restype = C.c_int
+    return h
+
+
+def download(hip, dbuf, offset: int, nbytes: int) -> np.ndarray:
+    """Copies nbytes of a device buffer to the host (hipMemcpyDeviceToHost)."""
+    out = np.empty(nbytes, np.uint8)
+    if hip.hipMemcpy(out.ctypes.data, dbuf.ptr.value + offset, nbytes, 2) != 0:
+        raise RuntimeError("hipMemcpy failed")
+    return out
+
+
+def pct(xs, q):
+    return float(np.percentile(np.asarray(xs) * 1e3, q))
+
+
+def serving(eng, args) -> dict:
+    """Serving: a reference set digested on the device from ref_gb of
+    device-generated segment mix (1 GiB inputs hashed with -b 10, about one
+    filter per 10 KiB), kept resident; queries of 4 KiB-16 MiB (log-uniform),
+    half cut from the reference data and half fresh, stream-digested.
+    plain: one query at a time through digest + compare_to.  batched: all
+    queries submitted to the server at once.  poisson: arrivals at a fixed
+    rate (70 % of the batched rate), latency from submit to result."""
+    import threading
+    import sdhash_amd
+    from sdhash_amd import synth
+    hip = _hip()
+    GIB = 1 << 30
+    nfile = max(1, int(round(args.serve_ref_gb * 1e9 / GIB)))
+    dref = eng.alloc(nfile * GIB)
+    synth.device_fill(dref.ptr.value, nfile * GIB, seed=11)
+    ref = eng.new_set()
+    t = time.perf_counter()
+    st = ref.digest_device(dref, [i * GIB for i in range(nfile)], [GIB] * nfile,
+                           [b"ref%04d" % i for i in range(nfile)], dd_block=10240)
+    eng.sync()
+    t_ref = time.perf_counter() - t
+    assert np.all(st == 0)
+    rng = np.random.Generator(np.random.PCG64(12))
+    nq = args.serve_queries
+    sizes = np.minimum((2.0 ** rng.uniform(12, 24, nq)).astype(np.int64), 16 * MIB - 16)
+    qs = []
+    for k, n in enumerate(sizes):
+        if k % 2 == 0:                   # cut from the reference data
+            o = int(rng.integers(0, nfile * GIB - n)) & ~15
+            qs.append(download(hip, dref, o, int(n)))
+        else:                            # fresh bytes of another stream
+            qs.append(None)
+    dref.free()
+    fresh_bytes = int(sum(int(n + 15) & ~15 for k, n in enumerate(sizes) if k % 2))
+    dq = eng.alloc(max(fresh_bytes, 16))
+    synth.device_fill(dq.ptr.value, max(fresh_bytes, 16) + 15 & ~15, seed=13)
+    o = 0
+    for k, n in enumerate(sizes):
+        if k % 2:
+            qs[k] = download(hip, dq, o, int(n))
+            o += int(n + 15) & ~15
+    dq.free()
+    names = [b"q%05d" % k for k in range(nq)]
+
+    # plain: one query at a time
+    nplain = min(nq, args.serve_plain)
+    plain_lat, plain_scores = [], []
+    t0 = time.perf_counter()
+    for k in range(nplain):
+        t = time.perf_counter()
+        qset = eng.new_set()
+        qset.digest([qs[k]], [names[k]])
+        plain_scores.append(eng.compare_to(qset, ref)[0].copy())
+        qset.close()
+        plain_lat.append(time.perf_counter() - t)
+    plain_s = time.perf_counter() - t0
+
+    srv = sdhash_amd.Server(ref, batch_bytes=args.serve_batch_mib * MIB, batch_max=1024)
+    # batched: everything at once
+    t0 = time.perf_counter()
+    tickets = [srv.submit(qs[k], names[k]) for k in range(nq)]
+    res = [srv.wait(tk) for tk in tickets]
+    batched_s = time.perf_counter() - t0
+    nb_closed, _ = srv.stats()
+    for k in range(nplain):
+        if res[k].status != 0 or not np.array_equal(res[k].scores, plain_scores[k]):
+            raise RuntimeError("served result of query %d differs from the plain one" % k)
+    # poisson arrivals at 70 % of the batched rate
+    rate = 0.7 * nq / batched_s
+    gaps = rng.exponential(1.0 / rate, nq)
+    tick = [None] * nq
+
+    def submitter():
+        tnext = time.perf_counter()
+        for k in range(nq):
+            tnext += gaps[k]
+            d = tnext - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tick[k] = srv.submit(qs[k], names[k])
+    th = threading.Thread(target=submitter)
+    t0 = time.perf_counter()
+    th.start()
+    lat = []
+    for k in range(nq):
+        while tick[k] is None:
+            time.sleep(0.0002)
+        r = srv.wait(tick[k])
+        lat.append(r.latency_s)
+        if k < nplain and not np.array_equal(r.scores, plain_scores[k]):
+            raise RuntimeError("served result of query %d differs from the plain one" % k)
+    th.join()
+    poisson_s = time.perf_counter() - t0
+    nb_total, _, nb_mfma = srv.stats3()
+    srv.close()
+    qfilters = sum(int(x.size) for x in qs) / 1e6
+    return {
+        "ref_filters": ref.filter_count(), "ref_digests": len(ref), "ref_bytes": nfile * GIB,
+        "ref_digest_s": t_ref, "queries": nq, "query_MB": qfilters,
+        "plain_queries": nplain, "plain_qps": nplain / plain_s,
+        "plain_p50_ms": pct(plain_lat, 50), "plain_p99_ms": pct(plain_lat, 99),
+        "batched_qps": nq / batched_s, "batched_batches": nb_closed,
+        "batched_vs_plain": (nq / batched_s) / (nplain / plain_s),
+        "poisson_target_qps": rate, "poisson_qps": nq / poisson_s,
+        "poisson_batches": nb_total - nb_closed, "mfma_batches": nb_mfma,
+        "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+        "identical_results": True,
+        "data": "device-generated segment mix (sdbfx_synth_fill); reference -b 10 over 1 GiB inputs",
+    }
+
+
+def batched_digest(eng, args) -> dict:
+    """Batched digesting: files with log-normal sizes (median 256 KiB,
+    sigma 1.5, clamped to 4 KiB-64 MiB) of device-generated segment mix,
+    resident in HBM; >= 16 MiB hashed with -b 16, the rest stream mode (the
+    sdhash CLI policy).  batched: one call for all files; plain: one call per
+    file over the first `plain` files.  Digests compared on a sample."""
+    from sdhash_amd import synth
+    rng = np.random.Generator(np.random.PCG64(21))
+    n = args.batch_files
+    sizes = np.clip(np.exp(rng.normal(np.log(256 * 1024), 1.5, n)), 4096, 64 * MIB).astype(np.int64)
+    offs = np.zeros(n, np.int64)
+    offs[1:] = np.cumsum((sizes[:-1] + 255) // 256 * 256)
+    total = int(offs[-1] + (sizes[-1] + 255) // 256 * 256)
+    dd = np.where(sizes >= 16 * MIB, 16384, 0).astype(np.uint32)
+    dbuf = eng.alloc(total)
+    synth.device_fill(dbuf.ptr.value, total, seed=31)
+    names = [b"file%05d" % i for i in range(n)]
+    s = eng.new_set()
+    eng.sync()
+    t = time.perf_counter()
+    st = s.digest_device(dbuf, offs, sizes, names, dd_block=dd)
+    eng.sync()
+    t_batched = time.perf_counter() - t
+    nplain = min(n, args.batch_plain)
+    plain = []
+    t = time.perf_counter()
+    for i in range(nplain):
+        p = eng.new_set()
+        p.digest_device(dbuf, offs[i:i + 1], sizes[i:i + 1], names[i:i + 1], dd_block=dd[i:i + 1])
+        plain.append(p)
+    eng.sync()
+    t_plain = time.perf_counter() - t
+    for i in range(0, nplain, max(1, nplain // 200)):
+        a, b = s.read(i), plain[i].read(0)
+        if not all(np.array_equal(x, y) for x, y in zip(a, b)):
+            raise RuntimeError("batched digest of file %d differs from the per-file one" % i)
+    plain_bytes = int(sizes[:nplain].sum())
+    for p in plain:
+        p.close()
+    dbuf.free()
+    return {
+        "files": n, "bytes": int(sizes.sum()), "block_mode_files": int((dd > 0).sum()),
+        "batched_s": t_batched, "batched_GBps": float(sizes.sum()) / t_batched / 1e9,
+        "plain_files": nplain, "plain_s": t_plain, "plain_GBps": plain_bytes / t_plain / 1e9,
+        "batched_vs_plain": (float(sizes.sum()) / t_batched) / (plain_bytes / t_plain),
+        "too_small": int((st == -3).sum()), "identical_sample": True,
+        "data": "device-generated segment mix (sdbfx_synth_fill)",
+    }
+
+
 def copy_probe() -> int:
     """A plain 256 MiB device-to-device copy, three times: the reference
     point for scaling the FETCH_SIZE / WRITE_SIZE counters."""
@@ -104,6 +286,14 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-digest measurement")
+    ap.add_argument("--serve-ref-gb", type=float, default=100.0)
+    ap.add_argument("--serve-queries", type=int, default=256)
+    ap.add_argument("--serve-plain", type=int, default=256)
+    ap.add_argument("--serve-batch-mib", type=int, default=512)
+    ap.add_argument("--batch-files", type=int, default=20000)
+    ap.add_argument("--batch-plain", type=int, default=4000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -158,13 +348,14 @@ def main() -> int:
 
     k = {}
     for name in ("block_digest_kernel", "stream_tile_kernel", "stream_fixup_kernel", "stream_insert_kernel",
-                 "gather_filters_kernel", "cmp_kernel", "cmp_mfma_kernel", "pair_sum_kernel"):
+                 "gather_filters_kernel", "cmp_kernel", "cmp_mfma_kernel", "chunk_max_kernel", "pair_sum_kernel"):
         ms, n = eng.kernel_time(name)
         if n:
             k[name] = (ms, n)
     nfilt_t = tset.filter_count()
     pairs = sum(qset.info(i).bf_count for i in range(len(qset))) * nfilt_t
-    cmp_ms = sum(k.get(n, (0.0, 0))[0] for n in ("cmp_kernel", "cmp_mfma_kernel", "pair_sum_kernel")) / args.steps
+    cmp_ms = sum(k.get(n, (0.0, 0))[0] for n in ("cmp_kernel", "cmp_mfma_kernel", "chunk_max_kernel",
+                                                 "pair_sum_kernel")) / args.steps
 
     # the two comparison kernels on the same sets: same scores, rate of each
     # against the peak pair rate of its unit (2048 AND-popcount bit pairs per
@@ -222,6 +413,13 @@ def main() -> int:
                              "traffic": measured_traffic("cmp_mfma_kernel"),
                              "algorithmic_ops_per_launch": pairs * 4096},
     }
+    del tset, qset
+    if not args.no_batched:
+        log("batched digest ...")
+        out["batched_digest"] = batched_digest(eng, args)
+    if not args.no_serving:
+        log("serving ...")
+        out["serving"] = serving(eng, args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(img, frags, seg_names)
     print(json.dumps(out))

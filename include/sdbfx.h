@@ -168,8 +168,10 @@ int sdbfx_serve_poll(sdbfx_server *s, uint64_t ticket, int32_t *scores, int32_t 
                      int *done);
 /* As poll, blocking until the result is ready. */
 int sdbfx_serve_wait(sdbfx_server *s, uint64_t ticket, int32_t *scores, int32_t *status, uint64_t *latency_ns);
-/* Batches and queries completed so far. */
-int sdbfx_serve_stats(sdbfx_server *s, uint64_t *batches, uint64_t *queries);
+/* Batches and queries completed so far, and the batches compared with the
+ * MFMA kernel.  The server picks the comparison kernel per batch size class
+ * by timing both on the first batch of each class. */
+int sdbfx_serve_stats(sdbfx_server *s, uint64_t *batches, uint64_t *queries, uint64_t *mfma_batches);
 
 /* ---- device memory for device-resident inputs ------------------------ */
 

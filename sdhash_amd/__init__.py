@@ -87,7 +87,7 @@ def load():
     L.sdbfx_serve_submit.argtypes = [P, P, U64, C.c_char_p, U32, C.POINTER(U64)]
     L.sdbfx_serve_poll.argtypes = [P, U64, P, C.POINTER(I32), C.POINTER(U64), C.POINTER(C.c_int)]
     L.sdbfx_serve_wait.argtypes = [P, U64, P, C.POINTER(I32), C.POINTER(U64)]
-    L.sdbfx_serve_stats.argtypes = [P, C.POINTER(U64), C.POINTER(U64)]
+    L.sdbfx_serve_stats.argtypes = [P, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64)]
     for name in SYMBOLS:
         getattr(L, name).restype = getattr(L, name).restype or I32
     for name in ("sdbfx_ctx_create", "sdbfx_sync", "sdbfx_set_create", "sdbfx_set_digest",
@@ -336,9 +336,14 @@ class Server:
         return r if done.value else None
 
     def stats(self) -> tuple[int, int]:
-        b, q = C.c_uint64(), C.c_uint64()
-        self.lib.sdbfx_serve_stats(self.handle, C.byref(b), C.byref(q))
-        return b.value, q.value
+        """(batches, queries) completed so far."""
+        return self.stats3()[:2]
+
+    def stats3(self) -> tuple[int, int, int]:
+        """(batches, queries, batches compared with the MFMA kernel)."""
+        b, q, m = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self.lib.sdbfx_serve_stats(self.handle, C.byref(b), C.byref(q), C.byref(m))
+        return b.value, q.value, m.value
 
     def error(self) -> str:
         return self.lib.sdbfx_serve_error(self.handle).decode()

@@ -13,6 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsdbfx.so")
 CXXLIB = os.path.join(HERE, "libsdbf_amd.so")
 CLI = os.path.join(HERE, "sdhash-amd")
+SYNTH_LIB = os.path.join(HERE, "libsdbfx_synth.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -41,6 +42,9 @@ def build(verbose: bool = False) -> str:
     _run([HIPCC, *FLAGS, "-I", INCLUDE, "-x", "hip", os.path.join(CSRC, "sdbfx_host.cpp"),
           os.path.join(CSRC, "sdbfx_serve.cpp"), "-o", LIB + ".tmp"], verbose)
     os.replace(LIB + ".tmp", LIB)
+    # the device-side synthetic input generator used by bench.py
+    _run([HIPCC, *FLAGS, "-x", "hip", os.path.join(CSRC, "synth_fill.hip"), "-o", SYNTH_LIB + ".tmp"], verbose)
+    os.replace(SYNTH_LIB + ".tmp", SYNTH_LIB)
     # the C++ sdbf / sdbf_set classes over the C ABI, and the sdhash CLI
     cxx = ["g++", "-std=c++17", "-O2", "-fPIC", "-Wall", "-I", INCLUDE]
     _run([*cxx, "-shared", os.path.join(CSRC, "sdbf_api.cpp"), "-o", CXXLIB + ".tmp", "-L", HERE,

@@ -49,7 +49,7 @@ struct sdbfx_ctx {
     std::map<std::string, std::pair<double, uint64_t>> times;
     std::vector<hipEvent_t> evpool;
     DevBuf stage, blocks, tiles, tstate, units, files, cand, sfilt, sham, counts, gsrc, gdst, gcnt,
-        glast, groups, pairs, partial, scores;
+        glast, groups, pairs, partial, imax, scores;
 };
 
 struct sdbfx_set {
@@ -535,7 +535,8 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
 
     int rc;
     if ((rc = upload(ctx, ctx->groups, groups)) || (rc = upload(ctx, ctx->pairs, pairs)) ||
-        (rc = ensure(ctx, ctx->partial, partial_n * 4 + 256)) || (rc = ensure(ctx, ctx->scores, npairs * 4 + 256)))
+        (rc = ensure(ctx, ctx->partial, partial_n * 4 + 256)) || (rc = ensure(ctx, ctx->imax, inst * 4 + 256)) ||
+        (rc = ensure(ctx, ctx->scores, npairs * 4 + 256)))
         return rc;
     SetView va{(const uint32_t*)S[0]->filt.p, (const uint16_t*)S[0]->ecf.p, (const uint16_t*)S[0]->hamf.p};
     SetView vb{(const uint32_t*)S[1]->filt.p, (const uint16_t*)S[1]->ecf.p, (const uint16_t*)S[1]->hamf.p};
@@ -550,8 +551,11 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
         LAUNCH(ctx, cmp_kernel, grid, 64, va, vb, (const CmpGroup*)ctx->groups.p, (uint32_t)groups.size(),
                (const CmpPair*)ctx->pairs.p, wi, (uint32_t*)ctx->partial.p);
     }
+    if (inst)
+        LAUNCH(ctx, chunk_max_kernel, (uint32_t)((inst + 255) / 256), 256, (const CmpPair*)ctx->pairs.p,
+               (uint32_t)npairs, inst, (const uint32_t*)ctx->partial.p, (uint32_t*)ctx->imax.p);
     LAUNCH(ctx, pair_sum_kernel, (uint32_t)((npairs + 255) / 256), 256, (const CmpPair*)ctx->pairs.p,
-           (uint32_t)npairs, va.ecf, vb.ecf, (const uint32_t*)ctx->partial.p, (int32_t*)ctx->scores.p);
+           (uint32_t)npairs, va.ecf, vb.ecf, (const uint32_t*)ctx->imax.p, (int32_t*)ctx->scores.p);
     std::vector<int32_t> sorted(npairs);
     HIPC(ctx, hipMemcpyAsync(sorted.data(), ctx->scores.p, npairs * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPC(ctx, hipStreamSynchronize(ctx->stream));
@@ -595,7 +599,7 @@ void sdbfx_ctx_destroy(sdbfx_ctx* ctx)
     (void)hipStreamSynchronize(ctx->stream);
     DevBuf* bufs[] = {&ctx->stage, &ctx->blocks, &ctx->tiles, &ctx->tstate, &ctx->units, &ctx->files,
                       &ctx->cand, &ctx->sfilt, &ctx->sham, &ctx->counts, &ctx->gsrc, &ctx->gdst,
-                      &ctx->gcnt, &ctx->glast, &ctx->groups, &ctx->pairs, &ctx->partial, &ctx->scores};
+                      &ctx->gcnt, &ctx->glast, &ctx->groups, &ctx->pairs, &ctx->partial, &ctx->imax, &ctx->scores};
     for (DevBuf* b : bufs)
         if (b->p)
             (void)hipFree(b->p);

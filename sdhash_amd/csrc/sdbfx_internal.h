@@ -19,6 +19,12 @@
 // digest workgroups
 #define SDBFX_NT 256             // threads per digest workgroup
 #define SDBFX_TILE 16384         // owned score positions per stream tile
+// 1: stream insertion replays every batch one candidate at a time (a test
+// setting: the replay is otherwise taken only when a batch decides a
+// rejection internally)
+#ifndef SDBFX_INSERT_REPLAY
+#define SDBFX_INSERT_REPLAY 0
+#endif
 #ifndef SDBFX_WARM
 #define SDBFX_WARM 64            // speculative warm-up steps
 #endif

@@ -239,6 +239,35 @@ __device__ __forceinline__ uint32_t wave_excl_sum(uint32_t v, int lane, uint32_t
     return x + add - v;
 }
 
+// LDS written by some lanes of a wave, then read by others
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// position of the r-th (from 0) set bit of m; m has more than r set bits
+__device__ __forceinline__ uint32_t select_bit(unsigned long long m, uint32_t r)
+{
+    uint32_t pos = 0, x = (uint32_t)m;
+    uint32_t c = __popc(x);
+    if (r >= c) {
+        r -= c;
+        x = (uint32_t)(m >> 32);
+        pos = 32;
+    }
+#pragma unroll
+    for (int w = 16; w >= 1; w >>= 1) {
+        c = __popc(x & ((1u << w) - 1));
+        if (r >= c) {
+            r -= c;
+            x >>= w;
+            pos += w;
+        }
+    }
+    return pos;
+}
+
 // Runs the score state machine over steps [i0, i1) from state m (i0 - base
 // is a multiple of 4).  With CREDIT each step adds SIGN to the score of the
 // position it elects, when that position is owned ([t0, t0+T)); consecutive
@@ -820,32 +849,83 @@ __global__ __launch_bounds__(SDBFX_NT) void stream_fixup_kernel(
 // Ordered insertion of one stream-mode input (gen_chunk_hash per unit,
 // sdbf_core.cc:94-136, with the de-duplication filter of sdbf_class.cc:378-380
 // and bloom_filter.cc:334-359), then the final trim (sdbf_core.cc:235-238).
-// One wave per input; lane w holds word w of the current 256-byte filter.
+// One wave per input; the current 256-byte filter and the 16 KiB
+// de-duplication filter are in LDS.
+//
+// Candidates are taken 64 at a time in order (lane l = the l-th).  Every one
+// sets its 5 bits in the current filter (bf_sha1_insert); its count of new
+// bits is the number of its distinct positions that were clear before the
+// batch and that no lower lane of the batch also sets (first-lane ownership
+// through atomicMin).  Those with new bits reach the de-duplication filter,
+// where one is rejected when all its positions are set.  A position clear at
+// the start of the batch and set by another lane of the batch is detected
+// with two scratch bitmaps; when such positions could decide a rejection
+// (all of a lane's clear positions are shared), the batch is replayed one
+// candidate at a time.  The accepted candidates are counted in order; the
+// batch is committed up to the one that completes a filter (160 elements)
+// or a de-duplication filter (8738), and the rest is taken again with the
+// new state.
+struct InsertSmem {
+    uint32_t big[SDBFX_BIG_BYTES / 4];      // de-duplication filter
+    uint32_t seen[SDBFX_BIG_BYTES / 4];     // positions of the batch
+    uint32_t twice[SDBFX_BIG_BYTES / 4];    // positions of the batch set by two lanes
+    uint32_t own[SDBFX_BF * 8];             // lowest lane setting each filter bit
+    uint32_t fw[SDBFX_BFW];                 // current filter
+};
+
 __global__ __launch_bounds__(64) void stream_insert_kernel(
     const StreamFile* __restrict__ files, const TileDesc* __restrict__ tiles,
     const TileState* __restrict__ tstate, const uint32_t* __restrict__ cand, uint8_t* __restrict__ stage,
     uint16_t* __restrict__ stage_ham, uint32_t* __restrict__ out_counts)
 {
-    __shared__ uint32_t big[SDBFX_BIG_BYTES / 4];
+    __shared__ InsertSmem ism;
+    uint32_t* const big = ism.big;
+    uint32_t* const fw = ism.fw;
     const StreamFile sf = files[blockIdx.x];
     const int lane = threadIdx.x;
-    for (int q = lane; q < SDBFX_BIG_BYTES / 4; q += 64)
+    for (int q = lane; q < SDBFX_BIG_BYTES / 4; q += 64) {
         big[q] = 0;
+        ism.seen[q] = 0;
+        ism.twice[q] = 0;
+    }
+    for (int q = lane; q < SDBFX_BF * 8; q += 64)
+        ism.own[q] = 0xFFFFFFFFu;
+    fw[lane] = 0;
     __syncthreads();
-    uint32_t fword = 0, bfc = 1, last = 0, bigfi = 0, overflow = 0;
+    uint32_t bfc = 1, last = 0, bigfi = 0, overflow = 0;
 
     auto flush = [&](uint32_t idx) {
-        uint32_t pc = __popc(fword);
+        const uint32_t word = fw[lane];
+        uint32_t pc = __popc(word);
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1)
             pc += __shfl_down(pc, off);
         if (idx < sf.filt_cap) {
-            ((uint32_t*)(stage + (sf.filt_base + idx) * SDBFX_BF))[lane] = fword;
+            ((uint32_t*)(stage + (sf.filt_base + idx) * SDBFX_BF))[lane] = word;
             if (lane == 0)
                 stage_ham[sf.filt_base + idx] = (uint16_t)pc;
         } else {
             overflow = 1;
         }
+    };
+    // after an accepted candidate: rollover of the filter and of the
+    // de-duplication filter (sdbf_core.cc:121-132)
+    auto counted = [&](uint32_t k) {
+        last += k;
+        bigfi += k;
+        if (last == SDBFX_MAX_ELEM) {
+            flush(bfc - 1);
+            wave_lds_sync();
+            fw[lane] = 0;
+            bfc++;
+            last = 0;
+        }
+        if (bigfi == SDBFX_BIG_ELEM) {
+            for (int q = lane; q < SDBFX_BIG_BYTES / 4; q += 64)
+                big[q] = 0;
+            bigfi = 0;
+        }
+        wave_lds_sync();
     };
 
     for (uint32_t t = sf.tile_begin; t < sf.tile_end; t++) {
@@ -859,68 +939,135 @@ __global__ __launch_bounds__(64) void stream_insert_kernel(
 #pragma unroll
             for (int j = 0; j < 5; j++)
                 myh[j] = (c0 + lane < C) ? cw[j * SDBFX_CAND_CAP + c0 + lane] : 0u;
-            for (uint32_t cc = 0; cc < cnt; cc++) {
-                uint32_t h[5];
+            // positions and which of them repeat an earlier one of the lane
+            uint32_t p[5], q[5], pdup = 0, qdup = 0;
 #pragma unroll
-                for (int j = 0; j < 5; j++)
-                    h[j] = (uint32_t)__builtin_amdgcn_readlane((int)myh[j], (int)cc);
-                uint32_t nb = 0;
+            for (int j = 0; j < 5; j++) {
+                p[j] = myh[j] & 0x7FFu;
+                q[j] = myh[j] & 0x1FFFFu;
 #pragma unroll
-                for (int j = 0; j < 5; j++) {
-                    const uint32_t pos = h[j] & 0x7FFu;
-                    const uint32_t wi = pos >> 5;
-                    const uint32_t bit = 1u << (pos & 31);
-                    const uint32_t cur = (uint32_t)__builtin_amdgcn_readlane((int)fword, (int)wi);
-                    nb += (cur & bit) ? 0u : 1u;
-                    if ((uint32_t)lane == wi)
-                        fword |= bit;
+                for (int k = 0; k < j; k++) {
+                    pdup |= (p[k] == p[j] ? 1u : 0u) << j;
+                    qdup |= (q[k] == q[j] ? 1u : 0u) << j;
                 }
-                if (nb == 0)
-                    continue;
-                // de-duplication filter: inserted unless all 5 bits were set
-                uint32_t bw[5];
+            }
+            uint32_t s = 0;   // first candidate of the chunk not yet committed
+            while (s < cnt) {
+                const bool act = (uint32_t)lane >= s && (uint32_t)lane < cnt;
+                // 1. new bits in the current filter
+                if (act) {
 #pragma unroll
-                for (int j = 0; j < 5; j++)
-                    bw[j] = big[(h[j] & 0x1FFFFu) >> 5];
-                uint32_t setc = 0;
+                    for (int j = 0; j < 5; j++)
+                        if (!((pdup >> j) & 1))
+                            atomicMin(&ism.own[p[j]], (uint32_t)lane);
+                }
+                wave_lds_sync();
+                uint32_t nnew = 0;
+                if (act) {
 #pragma unroll
-                for (int j = 0; j < 5; j++) {
-                    const uint32_t bp = h[j] & 0x1FFFFu;
-                    uint32_t cur = bw[j];
+                    for (int j = 0; j < 5; j++)
+                        if (!((pdup >> j) & 1) && !((fw[p[j] >> 5] >> (p[j] & 31)) & 1u) &&
+                            ism.own[p[j]] == (uint32_t)lane)
+                            nnew++;
+                }
+                // 2. the de-duplication filter for those with new bits
+                const bool reach = act && nnew > 0;
+                uint32_t clr = 0;   // positions clear at the start of the batch
+                if (reach) {
 #pragma unroll
-                    for (int jj = 0; jj < j; jj++) {
-                        const uint32_t bq = h[jj] & 0x1FFFFu;
-                        if ((bq >> 5) == (bp >> 5))
-                            cur |= 1u << (bq & 31);
+                    for (int j = 0; j < 5; j++)
+                        if (!((qdup >> j) & 1) && !((big[q[j] >> 5] >> (q[j] & 31)) & 1u)) {
+                            clr |= 1u << j;
+                            const uint32_t bit = 1u << (q[j] & 31);
+                            if (atomicOr(&ism.seen[q[j] >> 5], bit) & bit)
+                                atomicOr(&ism.twice[q[j] >> 5], bit);
+                        }
+                }
+                wave_lds_sync();
+                uint32_t shared = 0;
+                if (reach) {
+#pragma unroll
+                    for (int j = 0; j < 5; j++)
+                        if (((clr >> j) & 1) && ((ism.twice[q[j] >> 5] >> (q[j] & 31)) & 1u))
+                            shared |= 1u << j;
+                }
+                // a rejection decided inside the batch: replay it in order
+                const bool replay = SDBFX_INSERT_REPLAY || __ballot(reach && clr != 0 && shared == clr) != 0;
+                const bool acc = reach && clr != 0;
+                wave_lds_sync();
+                if (reach) {
+#pragma unroll
+                    for (int j = 0; j < 5; j++)
+                        if ((clr >> j) & 1) {
+                            atomicAnd(&ism.seen[q[j] >> 5], ~(1u << (q[j] & 31)));
+                            atomicAnd(&ism.twice[q[j] >> 5], ~(1u << (q[j] & 31)));
+                        }
+                }
+                if (act) {
+#pragma unroll
+                    for (int j = 0; j < 5; j++)
+                        if (!((pdup >> j) & 1))
+                            ism.own[p[j]] = 0xFFFFFFFFu;
+                }
+                wave_lds_sync();
+                if (replay) {
+                    for (uint32_t cc = s; cc < cnt; cc++) {
+                        uint32_t h[5];
+#pragma unroll
+                        for (int j = 0; j < 5; j++)
+                            h[j] = (uint32_t)__builtin_amdgcn_readlane((int)myh[j], (int)cc);
+                        uint32_t nb = 0;
+#pragma unroll
+                        for (int j = 0; j < 5; j++) {
+                            const uint32_t pos = h[j] & 0x7FFu;
+                            nb += (fw[pos >> 5] >> (pos & 31)) & 1u ? 0u : 1u;
+                            wave_lds_sync();
+                            if (lane == 0)
+                                fw[pos >> 5] |= 1u << (pos & 31);
+                            wave_lds_sync();
+                        }
+                        if (nb == 0)
+                            continue;
+                        uint32_t setc = 0;
+#pragma unroll
+                        for (int j = 0; j < 5; j++) {
+                            const uint32_t bp = h[j] & 0x1FFFFu;
+                            setc += (big[bp >> 5] >> (bp & 31)) & 1u;
+                            wave_lds_sync();
+                            if (lane == 0)
+                                big[bp >> 5] |= 1u << (bp & 31);
+                            wave_lds_sync();
+                        }
+                        if (setc == 5)
+                            continue;
+                        counted(1);
                     }
-                    setc += (cur >> (bp & 31)) & 1u;
+                    s = cnt;
+                    continue;
                 }
-                // every lane has read the words before lane 0 updates them
-                __builtin_amdgcn_wave_barrier();
-                if (lane == 0) {
+                // 3. commit up to the candidate that completes a filter
+                const unsigned long long am = __ballot(acc);
+                const uint32_t need = min(SDBFX_MAX_ELEM - last, SDBFX_BIG_ELEM - bigfi);
+                uint32_t cut = cnt - 1;
+                if ((uint32_t)__popcll(am) >= need)
+                    cut = select_bit(am, need - 1);
+                const bool com = act && (uint32_t)lane <= cut;
+                if (com) {
 #pragma unroll
-                    for (int j = 0; j < 5; j++) {
-                        const uint32_t bp = h[j] & 0x1FFFFu;
-                        atomicOr(&big[bp >> 5], 1u << (bp & 31));
+                    for (int j = 0; j < 5; j++)
+                        atomicOr(&fw[p[j] >> 5], 1u << (p[j] & 31));
+                    if (acc) {
+#pragma unroll
+                        for (int j = 0; j < 5; j++)
+                            atomicOr(&big[q[j] >> 5], 1u << (q[j] & 31));
                     }
                 }
-                __builtin_amdgcn_wave_barrier();
-                if (setc == 5)
-                    continue;
-                last++;
-                bigfi++;
-                if (last == SDBFX_MAX_ELEM) {
-                    flush(bfc - 1);
-                    fword = 0;
-                    bfc++;
-                    last = 0;
-                }
-                if (bigfi == SDBFX_BIG_ELEM) {
-                    for (int q = lane; q < SDBFX_BIG_BYTES / 4; q += 64)
-                        big[q] = 0;
-                    __builtin_amdgcn_wave_barrier();
-                    bigfi = 0;
-                }
+                wave_lds_sync();
+                const unsigned long long upto = cut >= 63 ? ~0ull : ((2ull << cut) - 1);
+                const uint32_t k = (uint32_t)__popcll(am & upto);
+                if (k)
+                    counted(k);
+                s = cut + 1;
             }
         }
     }
@@ -1101,8 +1248,9 @@ __global__ __launch_bounds__(64) void cmp_kernel(
 // score above 0 only if that count exceeds its cutoff cut(s1 + s2), so every
 // accumulator starts at -(cutoff + 1) of its pair (read from an LDS table
 // indexed by the sum of the two filters' size codes, size_code below) and
-// ends >= 0 exactly for those pairs.  These rare pairs are listed and recomputed
-// exactly on the VALU with the early-exit checkpoints of bf_bitcount_cut_256;
+// ends >= 0 exactly for those pairs.  These pairs are listed (768 per round)
+// and recomputed exactly on the VALU with the early-exit checkpoints of
+// bf_bitcount_cut_256;
 // their fractions are merged with a 64-bit LDS max on floor(num * 2^22 / den)
 // (strictly monotone in num/den for denominators below 2^11) followed by
 // num << 16 | den.
@@ -1213,13 +1361,6 @@ struct MfmaWaveSmem {
     unsigned long long best[SDBFX_MROWS];
     uint32_t clist[MF_CAND_CAP];    // row | column << 7
 };
-
-// LDS written by some lanes of a wave, then read by others
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-}
 
 // Four waves per workgroup, one per SIMD, each taking work items on its own;
 // they share only the cutoff table.
@@ -1407,60 +1548,50 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
             }
             if (__ballot(anyc != 0) == 0)
                 continue;
-            // each lane lists its own candidates after those of lower lanes
+            // each lane lists its own candidates after those of lower lanes;
+            // MF_CAND_CAP of them per round
             uint32_t own = 0;
 #pragma unroll
             for (int w = 0; w < MF_B * MF_B / 2; w++)
                 own += (uint32_t)__popc(cbits[w]);
             uint32_t ncand;
-            uint32_t pos = wave_excl_sum(own, lane, &ncand);
-            if (ncand <= MF_CAND_CAP) {
+            const uint32_t first = wave_excl_sum(own, lane, &ncand);
+            for (uint32_t round = 0; round < ncand; round += MF_CAND_CAP) {
+                uint32_t pos = first;
+                if (pos < round + MF_CAND_CAP && pos + own > round) {
 #pragma unroll
-                for (int w = 0; w < MF_B * MF_B / 2; w++) {
-                    uint32_t x = cbits[w];
-                    while (x) {
-                        const uint32_t bit = 31u - (uint32_t)__clz(x);
-                        x ^= 1u << bit;
-                        const uint32_t blk = 2 * w + (bit < 16 ? 1 : 0), i = (bit < 16 ? 15u : 31u) - bit;
-                        const uint32_t row = 32 * (blk / MF_B) + 8 * (i >> 2) + 4 * h + (i & 3);
-                        clist[pos++] = row | (32 * (blk % MF_B) + r32) << 7;
+                    for (int w = 0; w < MF_B * MF_B / 2; w++) {
+                        uint32_t x = cbits[w];
+                        while (x) {
+                            const uint32_t bit = 31u - (uint32_t)__clz(x);
+                            x ^= 1u << bit;
+                            if (pos >= round && pos < round + MF_CAND_CAP) {
+                                const uint32_t blk = 2 * w + (bit < 16 ? 1 : 0), i = (bit < 16 ? 15u : 31u) - bit;
+                                const uint32_t row = 32 * (blk / MF_B) + 8 * (i >> 2) + 4 * h + (i & 3);
+                                clist[pos - round] = row | (32 * (blk % MF_B) + r32) << 7;
+                            }
+                            pos++;
+                        }
                     }
                 }
-            }
-            wave_lds_sync();
-            if (ncand <= MF_CAND_CAP) {
+                wave_lds_sync();
                 // one candidate per lane
-                for (uint32_t base = 0; base < ncand; base += 64) {
-                    if (base + lane < ncand) {
+                const uint32_t nlist = min(ncand - round, (uint32_t)MF_CAND_CAP);
+                for (uint32_t base = 0; base < nlist; base += 64) {
+                    if (base + lane < nlist) {
                         const uint32_t e = clist[base + lane];
                         const uint32_t row = e & 127u, col = e >> 7;
                         const uint64_t j = nt + col;
                         const uint32_t ri = rinfo[row];
                         const uint32_t cutoff = (uint32_t)(-1 - (int)ncut[(ri & 0xFFFFu) + size_code(tec[j])]);
                         uint32_t p32, p64, p128, p256;
-                        prefix_counts(atile + row * 16, row & 15, (const uint4*)(tf + j * SDBFX_BFW), p32,
-                                            p64, p128, p256);
+                        prefix_counts(atile + row * 16, row & 15, (const uint4*)(tf + j * SDBFX_BFW), p32, p64,
+                                      p128, p256);
                         merge_pair(p32, p64, p128, p256, cutoff, ri >> 16, tham[j], &best[row]);
                     }
                 }
-            } else {
-                // too many to list: every qualifying pair of the tile, exactly
-                for (uint32_t base = 0; base < 128u * 128u; base += 64) {
-                    const uint32_t row = (base + lane) >> 7, col = (base + lane) & 127u;
-                    const uint32_t ri = rinfo[row];
-                    if (col < ncol) {
-                        const uint64_t j = nt + col;
-                        const uint32_t t = (uint32_t)(-1 - (int)ncut[(ri & 0xFFFFu) + size_code(tec[j])]);
-                        if (t < 0x7FFFu) {
-                            uint32_t p32, p64, p128, p256;
-                            prefix_counts(atile + row * 16, row & 15, (const uint4*)(tf + j * SDBFX_BFW), p32,
-                                                p64, p128, p256);
-                            merge_pair(p32, p64, p128, p256, t, ri >> 16, tham[j], &best[row]);
-                        }
-                    }
-                }
+                wave_lds_sync();
             }
-            wave_lds_sync();
         }
 
         const bool hq = __ballot(anyq) != 0;
@@ -1479,12 +1610,46 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
     }
 }
 
+// Per reference-filter instance: the largest fraction over its target
+// chunks (sdbf_max_score's maximum, sdbf_core.cc:396-455, over all of the
+// target digest), or 0xFFFFFFFF when no chunk had a qualifying target.  One
+// thread per instance; maxima are order-free, so this part is parallel.
+__global__ void chunk_max_kernel(const CmpPair* __restrict__ pairs, uint32_t npairs, uint64_t ninst,
+                                 const uint32_t* __restrict__ partial, uint32_t* __restrict__ imax)
+{
+    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (t >= ninst)
+        return;
+    uint32_t lo = 0, hi = npairs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pairs[mid].inst_begin <= t)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const CmpPair P = pairs[lo];
+    const uint32_t* v = partial + P.partial_begin + (t - P.inst_begin) * P.nch;
+    uint32_t bn = 0, bd = 0;
+    for (uint32_t c = 0; c < P.nch; c++) {
+        const uint32_t x = v[c];
+        if (x == 0xFFFFFFFFu)
+            continue;
+        const uint32_t n = x >> 16, d = x & 0xFFFFu;
+        if (bd == 0 || n * bd > bn * d) {
+            bn = n;
+            bd = d;
+        }
+    }
+    imax[t] = bd ? (bn << 16 | bd) : 0xFFFFFFFFu;
+}
+
 // sdbf_score (sdbf_core.cc:331-389) for each oriented pair: ordered double
 // sum of the per-filter maxima with the reference's -1 sentinel handling,
 // sparse-filter denominator and floor(x + 0.5) rounding.
 __global__ void pair_sum_kernel(const CmpPair* __restrict__ pairs, uint32_t npairs,
                                 const uint16_t* __restrict__ ecfA, const uint16_t* __restrict__ ecfB,
-                                const uint32_t* __restrict__ partial, int32_t* __restrict__ scores)
+                                const uint32_t* __restrict__ imax, int32_t* __restrict__ scores)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npairs)
@@ -1500,19 +1665,8 @@ __global__ void pair_sum_kernel(const CmpPair* __restrict__ pairs, uint32_t npai
             m = 0.0;
             sparse++;
         } else {
-            uint32_t bn = 0, bd = 0;
-            const uint32_t* v = partial + P.partial_begin + (uint64_t)i * P.nch;
-            for (uint32_t c = 0; c < P.nch; c++) {
-                const uint32_t x = v[c];
-                if (x == 0xFFFFFFFFu)
-                    continue;
-                const uint32_t n = x >> 16, d = x & 0xFFFFu;
-                if (bd == 0 || n * bd > bn * d) {
-                    bn = n;
-                    bd = d;
-                }
-            }
-            m = bd ? (double)bn / (double)bd : -1.0;
+            const uint32_t x = imax[P.inst_begin + i];
+            m = x == 0xFFFFFFFFu ? -1.0 : (double)(x >> 16) / (double)(x & 0xFFFFu);
         }
         sum = (sum < 0) ? m : sum + m;
     }

@@ -10,6 +10,13 @@
 // two batches are past the queue at a time, which bounds device memory to
 // about two batches plus the reference set.
 //
+// Autotuning: the two comparison kernels give identical scores at different
+// speeds depending on the shape of the batch (query filters per reference
+// digest) and on how many filter pairs come close to their cutoff.  The first
+// batch of each size class (log2 of its filter count, up to 2048 filters) is
+// compared with both, and the faster one is used for that class from then
+// on; a larger class takes the choice of the largest tuned class below it.
+//
 // The reference counterpart is sdhash-server's handler (sdhash-srv-main.cc:
 // 224-402), which hashes and compares each request on its own.
 #include <stdint.h>
@@ -28,6 +35,9 @@
 #include "sdbfx.h"
 
 namespace {
+
+// batches of at most this many query filters are timed with both kernels
+constexpr uint64_t SERVE_TUNE_MAX_FILTERS = 2048;
 
 uint64_t now_ns()
 {
@@ -67,12 +77,12 @@ struct sdbfx_server {
     std::condition_variable cv_queue, cv_handoff, cv_done;
     std::deque<std::shared_ptr<Request>> queue;
     std::deque<Batch> handoff;
-    bool handoff_busy = false;  // the compare thread holds a batch
     std::unordered_map<uint64_t, std::shared_ptr<Request>> live;
     uint64_t next_ticket = 1;
     bool stop = false;
     std::string err;
-    uint64_t nbatches = 0, nqueries = 0;
+    uint64_t nbatches = 0, nqueries = 0, nmfma = 0;
+    int8_t tuned[64];   // per size class: 0 untuned, SDBFX_CMP_VALU or SDBFX_CMP_MFMA
     std::thread td, tc;
 
     void set_error(const std::string& e)
@@ -131,6 +141,16 @@ struct sdbfx_server {
         }
     }
 
+    int timed_compare(const sdbfx_set* q, int method, int32_t* out, double* seconds)
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        int rc = sdbfx_set_compare_method(cc, method);
+        if (rc == 0)
+            rc = sdbfx_compare_to(cc, q, ref, out);
+        *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return rc;
+    }
+
     void compare_loop()
     {
         for (;;) {
@@ -147,9 +167,38 @@ struct sdbfx_server {
             const uint32_t n = (uint32_t)b.reqs.size();
             const uint32_t nd = b.set ? sdbfx_set_size(b.set) : 0;
             std::vector<int32_t> sc((size_t)nd * nref);
-            int rc = 0;
-            if (nd && nref)
-                rc = sdbfx_compare_to(cc, b.set, ref, sc.data());
+            int rc = 0, method = SDBFX_CMP_AUTO;
+            if (nd && nref) {
+                const uint64_t nf = sdbfx_set_filter_count(b.set);
+                const int cls = nf ? 63 - __builtin_clzll(nf) : 0;
+                method = tuned[cls];
+                if (method == 0 && nf > SERVE_TUNE_MAX_FILTERS) {
+                    // too large to time twice: the choice of the largest
+                    // tuned class below, else the engine's own estimate
+                    method = SDBFX_CMP_AUTO;
+                    for (int c = cls - 1; c >= 0; c--)
+                        if (tuned[c]) {
+                            method = tuned[c];
+                            break;
+                        }
+                    double t;
+                    rc = timed_compare(b.set, method, sc.data(), &t);
+                } else if (method == 0) {
+                    // time both kernels on this batch; keep the faster
+                    std::vector<int32_t> sv(sc.size());
+                    double tv = 0, tm = 0;
+                    rc = timed_compare(b.set, SDBFX_CMP_VALU, sv.data(), &tv);
+                    if (rc == 0)
+                        rc = timed_compare(b.set, SDBFX_CMP_MFMA, sc.data(), &tm);
+                    if (rc == 0 && sv != sc)
+                        set_error("compare: the two kernels disagree");
+                    method = tm <= tv ? SDBFX_CMP_MFMA : SDBFX_CMP_VALU;
+                    tuned[cls] = (int8_t)method;
+                } else {
+                    double t;
+                    rc = timed_compare(b.set, method, sc.data(), &t);
+                }
+            }
             if (rc != 0)
                 set_error(std::string("compare: ") + sdbfx_last_error(cc));
             const uint64_t t = now_ns();
@@ -169,6 +218,7 @@ struct sdbfx_server {
                 }
                 nbatches++;
                 nqueries += n;
+                nmfma += method == SDBFX_CMP_MFMA ? 1 : 0;   // AUTO counts as not tuned
             }
             cv_done.notify_all();
             if (b.set)
@@ -186,6 +236,7 @@ int sdbfx_serve_create(int device, const sdbfx_set* ref, uint64_t batch_bytes, u
         return -1;
     *out = nullptr;
     auto* s = new sdbfx_server();
+    memset(s->tuned, 0, sizeof(s->tuned));
     s->ref = ref;
     s->nref = sdbfx_set_size(ref);
     s->batch_bytes = batch_bytes;
@@ -296,7 +347,7 @@ int sdbfx_serve_wait(sdbfx_server* s, uint64_t ticket, int32_t* scores, int32_t*
     return take(s, s->live.find(ticket), scores, status, latency_ns);
 }
 
-int sdbfx_serve_stats(sdbfx_server* s, uint64_t* batches, uint64_t* queries)
+int sdbfx_serve_stats(sdbfx_server* s, uint64_t* batches, uint64_t* queries, uint64_t* mfma_batches)
 {
     if (!s)
         return -1;
@@ -305,6 +356,8 @@ int sdbfx_serve_stats(sdbfx_server* s, uint64_t* batches, uint64_t* queries)
         *batches = s->nbatches;
     if (queries)
         *queries = s->nqueries;
+    if (mfma_batches)
+        *mfma_batches = s->nmfma;
     return 0;
 }
 

@@ -12,6 +12,8 @@ PCG64 integer streams, so a seed gives the same bytes on every host.
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 SEG = 4096
@@ -124,3 +126,23 @@ def fragments(seed: int, image: np.ndarray, n: int, size: int = 4 * KIB) -> list
     fresh = rng.integers(0, 256, (n - half, size), dtype=np.uint8)
     out.extend(fresh[i] for i in range(n - half))
     return out
+
+
+_synth_lib = None
+
+
+def device_fill(ptr: int, nbytes: int, seed: int, offset: int = 0) -> None:
+    """Fills device memory at `ptr` with `nbytes` of the device-side segment
+    mix (libsdbfx_synth.so, same mix as segment_mix, different bytes), the
+    stream `seed` from byte `offset` on.  nbytes, offset: multiples of 16."""
+    import ctypes as C
+    global _synth_lib
+    if _synth_lib is None:
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsdbfx_synth.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built (run python -m sdhash_amd.build)")
+        _synth_lib = C.CDLL(path)
+        _synth_lib.sdbfx_synth_fill.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64]
+        _synth_lib.sdbfx_synth_fill.restype = C.c_int
+    if _synth_lib.sdbfx_synth_fill(C.c_void_p(ptr), nbytes, offset, seed) != 0:
+        raise RuntimeError("device fill failed")

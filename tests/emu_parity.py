@@ -49,7 +49,8 @@ def main(lib):
     print("block ok", flush=True)
 
     # stream mode: one tile, tile boundaries, several tiles, degenerate data
-    sizes = [512, 600, 4 * KIB, 16 * KIB + 64, 16 * KIB + 65, 40000, 70000]
+    # (the last one passes 8738 de-duplication elements: a reset)
+    sizes = [512, 600, 4 * KIB, 16 * KIB + 64, 16 * KIB + 65, 40000, 70000, 900 * KIB]
     bufs = [synth.segment_mix(rng, n) for n in sizes]
     bufs += [np.zeros(20000, np.uint8), np.full(20000, 7, np.uint8),
              np.tile(np.arange(64, dtype=np.uint8), 400), np.tile(np.arange(16, dtype=np.uint8), 2000)]

@@ -44,6 +44,7 @@ struct dim3 {
 struct uint4 {
     uint32_t x, y, z, w;
 };
+inline uint4 make_uint4(uint32_t x, uint32_t y, uint32_t z, uint32_t w) { return uint4{x, y, z, w}; }
 
 // Minimal x86-64 context switch: saves the callee-saved registers on the
 // current stack, stores its pointer in *save, and resumes the stack `next`.
@@ -427,6 +428,7 @@ inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v)
 
 inline uint32_t atomicAdd(uint32_t* p, uint32_t v) { const uint32_t o = *p; *p = o + v; return o; }
 inline uint32_t atomicOr(uint32_t* p, uint32_t v) { const uint32_t o = *p; *p = o | v; return o; }
+inline uint32_t atomicAnd(uint32_t* p, uint32_t v) { const uint32_t o = *p; *p = o & v; return o; }
 inline unsigned long long atomicMax(unsigned long long* p, unsigned long long v)
 {
     const unsigned long long o = *p;

@@ -36,6 +36,7 @@ def run(lib):
     assert "serve ok" in r.stdout
 
 
-@pytest.mark.parametrize("warm", [64, 12])
-def test_emulated_kernels(tmp_path, warm):
-    run(build(tmp_path, ["-DSDBFX_WARM=%d" % warm]))
+@pytest.mark.parametrize("variant", [[], ["-DSDBFX_WARM=12", "-DSDBFX_INSERT_REPLAY=1"]],
+                         ids=["default", "warm12-replay"])
+def test_emulated_kernels(tmp_path, variant):
+    run(build(tmp_path, variant))
