@@ -267,6 +267,55 @@ def batched_digest(eng, args) -> dict:
     }
 
 
+def allpairs(eng, args) -> dict:
+    """All-pairs compare: digests of 1-6,000 filters (uniform) hashed with
+    -b 16 from device-generated segment mix, about `allpairs_mfilters`
+    million filters in all; compare_all with each comparison kernel, same
+    scores required.  Rates are filter pairs per second of kernel time."""
+    import sdhash_amd
+    from sdhash_amd import synth
+    rng = np.random.Generator(np.random.PCG64(41))
+    target = int(args.allpairs_mfilters * 1e6)
+    counts = []
+    while sum(counts) < target:
+        counts.append(int(rng.integers(1, 6001)))
+    sizes = np.array(counts, np.int64) * 16384
+    offs = np.zeros(len(sizes), np.int64)
+    offs[1:] = np.cumsum(sizes[:-1])
+    total = int(sizes.sum())
+    dbuf = eng.alloc(total)
+    synth.device_fill(dbuf.ptr.value, total, seed=43)
+    s = eng.new_set()
+    s.digest_device(dbuf, offs, sizes, [b"d%04d" % i for i in range(len(sizes))], dd_block=16384)
+    eng.sync()
+    dbuf.free()
+    n = len(s)
+    nf = np.array([s.info(i).bf_count for i in range(n)], np.float64)
+    pairs = float((nf.sum() ** 2 - (nf ** 2).sum()) / 2)
+    out = {"digests": n, "filters": s.filter_count(), "filter_pairs": pairs}
+    ref = None
+    for label, method, kname in (("mfma", sdhash_amd.CMP_MFMA, "cmp_mfma_kernel"),
+                                 ("valu", sdhash_amd.CMP_VALU, "cmp_kernel")):
+        eng.compare_method(method)
+        eng.sync()
+        eng.profiling(True)
+        eng.reset_kernel_times()
+        t = time.perf_counter()
+        sc = eng.compare_all(s)
+        wall = time.perf_counter() - t
+        eng.profiling(False)
+        kms = sum(eng.kernel_time(k)[0] for k in (kname, "chunk_max_kernel", "pair_sum_kernel"))
+        if ref is None:
+            ref = sc
+        elif not np.array_equal(sc, ref):
+            raise RuntimeError("all-pairs scores differ between the comparison kernels")
+        out[label] = {"kernel_ms": kms, "wall_s": wall, "pairs_per_s": pairs / (kms / 1e3)}
+    eng.compare_method(sdhash_amd.CMP_AUTO)
+    out["scores_ge_21"] = int((ref >= 21).sum())
+    out["data"] = "device-generated segment mix (sdbfx_synth_fill), -b 16"
+    return out
+
+
 def copy_probe() -> int:
     """A plain 256 MiB device-to-device copy, three times: the reference
     point for scaling the FETCH_SIZE / WRITE_SIZE counters."""
@@ -294,6 +343,8 @@ def main() -> int:
     ap.add_argument("--serve-batch-mib", type=int, default=512)
     ap.add_argument("--batch-files", type=int, default=20000)
     ap.add_argument("--batch-plain", type=int, default=4000)
+    ap.add_argument("--no-allpairs", action="store_true", help="skip the all-pairs measurement")
+    ap.add_argument("--allpairs-mfilters", type=float, default=1.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -417,6 +468,9 @@ def main() -> int:
     if not args.no_batched:
         log("batched digest ...")
         out["batched_digest"] = batched_digest(eng, args)
+    if not args.no_allpairs:
+        log("all-pairs ...")
+        out["allpairs"] = allpairs(eng, args)
     if not args.no_serving:
         log("serving ...")
         out["serving"] = serving(eng, args)
