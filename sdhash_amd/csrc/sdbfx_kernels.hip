@@ -45,7 +45,12 @@
 #include "sdbfx_internal.h"
 
 #define RANK_LANES 128
+#define RUN_CHUNKS 9     // 16-window chunks per ranks lane: 128 * 9 * 16 >= LMAX
 #define LMAX (SDBFX_TILE + 64 + SDBFX_WARM + 64 + 8)
+// ranks runs cover every held position, and their 16-position chunk writes
+// stay inside P (a tile holds at most LMAX - 8 positions)
+static_assert(((LMAX + RANK_LANES - 1) / RANK_LANES + 15) / 16 <= RUN_CHUNKS, "ranks run too long");
+static_assert((LMAX - 8 + 15) / 16 * 16 <= LMAX, "ranks chunk writes past P");
 #define BCAND_CAP 1152   // block mode: positions with score >= 15 in <= 16320 positions
 #define NWAVE (SDBFX_NT / 64)
 
@@ -64,8 +69,8 @@ struct __align__(16) TileSmem {
             uint32_t csc[SDBFX_NT];         // block mode: scores of a batch
         } s;
     } u;
-    uint16_t rtab[1008];
-    int32_t dtab[72];
+    uint16_t rtab[SDBFX_NT * ((SDBFX_RANKS_LEN + SDBFX_NT - 1) / SDBFX_NT)];   // ENTR64_RANKS, tail padded
+    int32_t dtab[SDBFX_NT];                 // E[c] - E[c-1] for c in 1..64; 0 elsewhere
     uint32_t start_m[SDBFX_NT];
     uint32_t end_m[SDBFX_NT];
     uint32_t histo[68];
@@ -354,67 +359,99 @@ __device__ __forceinline__ uint32_t tile_core(TileSmem& sm, const uint8_t* __res
     const uint32_t NI = t_end - base;       // steps whose F is needed
     const uint32_t lim = n - 64;            // positions with a full window
 
-    for (int q = tid; q < SDBFX_RANKS_LEN; q += SDBFX_NT)
-        sm.rtab[q] = c_ranks[q];
-    if (tid < 65)
-        sm.dtab[tid] = tid ? (int32_t)c_entropy[tid] - (int32_t)c_entropy[tid - 1] : 0;
+    // ---- 1. ranks: each lane rolls a byte histogram over a run of at most
+    // RUN_CHUNKS * 16 windows that starts at a multiple of 16; the run's bytes
+    // and the 64 after it are loaded into registers before the first use.
+    // The count of byte value b for lane t is byte b & 3 of word
+    // (b >> 2) * RANK_LANES + t, so the lanes of a wave always touch 64
+    // different banks.  Counts change by word atomics that return the word
+    // before the change: the LDS applies one wave's operations in order, so
+    // the 32 updates of a 16-window chunk are all in flight at once and each
+    // still sees every update before it.  Removing the outgoing byte before
+    // adding the incoming one makes a step whose two bytes are equal change
+    // the entropy by dtab[c] - dtab[c] = 0, as it must.  Past the last window
+    // of the run the last chunk keeps sliding over real input bytes (at most
+    // 79 bytes past it, within the 16 readable bytes past n); those steps only
+    // touch the lane's own histogram and their ranks are replaced by 0.  The
+    // loads (chunk indices past the run clamped to its last needed chunk) are
+    // issued before the tables are filled, so their latency overlaps the fill.
+    const uint32_t PR = ((L + RANK_LANES - 1) / RANK_LANES + 15) & ~15u;
+    const uint32_t r0 = min(tid * PR, L);
+    const uint32_t r1 = min(r0 + PR, L);
+    const uint32_t p0 = base + r0;
+    const uint32_t pv = max(p0, min(base + r1, lim));   // windows exist for [p0, pv)
+    const uint32_t nch = (pv - p0 + 15) / 16;
+    // Every thread loads (no branches around the loads, so waiting for the
+    // tables does not wait for the run); threads without a run read the
+    // unit's first 64 bytes.
+    constexpr int RT = (SDBFX_RANKS_LEN + SDBFX_NT - 1) / SDBFX_NT;
+    uint32_t rt[RT];
+#pragma unroll
+    for (int k = 0; k < RT; k++)
+        rt[k] = c_ranks[min(tid + k * SDBFX_NT, SDBFX_RANKS_LEN - 1)];
+    const int32_t dt = (int32_t)c_entropy[min(tid, 64)] - (int32_t)c_entropy[min(max(tid, 1) - 1, 63)];
+    const bool has_run = tid < RANK_LANES && p0 < pv;
+    const uint4* q0 = (const uint4*)(u + (has_run ? p0 : 0));
+    const uint32_t lastc = has_run ? nch + 3 : 3;
+    uint4 X[RUN_CHUNKS + 4];
+#pragma unroll
+    for (int c = 0; c < RUN_CHUNKS + 4; c++)
+        X[c] = q0[min((uint32_t)c, lastc)];
+
+#pragma unroll
+    for (int k = 0; k < RT; k++)
+        sm.rtab[tid + k * SDBFX_NT] = rt[k];
+    sm.dtab[tid] = tid && tid < 65 ? dt : 0;
+    for (int q = tid; q < RANK_LANES * 256 / 16; q += SDBFX_NT)
+        ((uint4*)sm.u.hist)[q] = make_uint4(0, 0, 0, 0);
     __syncthreads();
 
-    // ---- 1. ranks: each lane rolls a byte histogram over a run of windows
-    // that starts at a multiple of 16; input bytes arrive 16 at a time, one
-    // chunk ahead of their use.
     if (tid < RANK_LANES) {
-        const uint32_t PR = ((L + RANK_LANES - 1) / RANK_LANES + 15) & ~15u;
-        const uint32_t r0 = min(tid * PR, L);
-        const uint32_t r1 = min(r0 + PR, L);
-        uint8_t* H = sm.u.hist + tid * 256;
-        const uint32_t swz = (uint32_t)(tid & 31) << 2;
-        uint32_t* H32 = (uint32_t*)H;
-        for (int q = 0; q < 64; q++)
-            H32[q] = 0;
-        const uint32_t p0 = base + r0;
-        const uint32_t pv = max(p0, min(base + r1, lim));   // windows exist for [p0, pv)
+        uint32_t* H = (uint32_t*)sm.u.hist + tid;
         if (p0 < pv) {
             int32_t e = 0;
-            const uint4* q0 = (const uint4*)(u + p0);
 #pragma unroll
             for (int c = 0; c < 4; c++) {
-                const uint4 v = q0[c];
+                uint32_t w[16];
 #pragma unroll
                 for (int j = 0; j < 16; j++) {
-                    const uint32_t cnt = ++H[byte_of(v, j) ^ swz];
-                    e += sm.dtab[cnt];
+                    const uint32_t b = byte_of(X[c], j);
+                    w[j] = atomicAdd(H + (b >> 2) * RANK_LANES, 1u << (8 * (b & 3)));
+                }
+#pragma unroll
+                for (int j = 0; j < 16; j++) {
+                    const uint32_t b = byte_of(X[c], j);
+                    e += sm.dtab[((w[j] >> (8 * (b & 3))) & 0xFFu) + 1];
                 }
             }
-            sm.P[r0] = sm.rtab[e >> 10];
-            uint4 O = q0[0], I = q0[4];
-            uint32_t oc = 0, ic = 0;
-            for (uint32_t cs = p0; cs < pv; cs += 16) {
-                uint4 On = O, In = I;
-                if (cs + 16 < pv) {
-                    On = *(const uint4*)(u + cs + 16);
-                    In = *(const uint4*)(u + cs + 80);
-                }
+            uint32_t* Pw = (uint32_t*)(sm.P + r0);
 #pragma unroll
-                for (int j = 0; j < 16; j++) {
-                    const uint32_t p = cs + j;
-                    if ((j > 0 || cs > p0) && p < pv) {
-                        const uint32_t bo = j ? byte_of(O, j - 1) : oc;
-                        const uint32_t bi = j ? byte_of(I, j - 1) : ic;
-                        if (bo != bi) {
-                            const uint32_t co = H[bo ^ swz];
-                            const uint32_t ci = H[bi ^ swz];
-                            H[bo ^ swz] = (uint8_t)(co - 1);
-                            H[bi ^ swz] = (uint8_t)(ci + 1);
-                            e += sm.dtab[ci + 1] - sm.dtab[co];
-                        }
-                        sm.P[p - base] = sm.rtab[e >> 10];
+            for (int c = 0; c < RUN_CHUNKS; c++) {
+                if ((uint32_t)c < nch) {
+                    uint32_t wo[16], wi[16];
+#pragma unroll
+                    for (int j = (c == 0); j < 16; j++) {
+                        const uint32_t bo = j ? byte_of(X[c], j - 1) : byte_of(X[c - 1], 15);
+                        const uint32_t bi = j ? byte_of(X[c + 4], j - 1) : byte_of(X[c + 3], 15);
+                        wo[j] = atomicSub(H + (bo >> 2) * RANK_LANES, 1u << (8 * (bo & 3)));
+                        wi[j] = atomicAdd(H + (bi >> 2) * RANK_LANES, 1u << (8 * (bi & 3)));
                     }
+                    uint32_t rk[8];
+#pragma unroll
+                    for (int j = 0; j < 16; j++) {
+                        if (c > 0 || j > 0) {
+                            const uint32_t bo = j ? byte_of(X[c], j - 1) : byte_of(X[c - 1], 15);
+                            const uint32_t bi = j ? byte_of(X[c + 4], j - 1) : byte_of(X[c + 3], 15);
+                            e += sm.dtab[((wi[j] >> (8 * (bi & 3))) & 0xFFu) + 1] -
+                                 sm.dtab[(wo[j] >> (8 * (bo & 3))) & 0xFFu];
+                        }
+                        const uint32_t r = sm.rtab[e >> 10];
+                        const uint32_t rv = p0 + 16 * c + j < pv ? r : 0u;
+                        rk[j >> 1] = (j & 1) ? (rk[j >> 1] | rv << 16) : rv;
+                    }
+                    *(uint4*)(Pw + 8 * c) = make_uint4(rk[0], rk[1], rk[2], rk[3]);
+                    *(uint4*)(Pw + 8 * c + 4) = make_uint4(rk[4], rk[5], rk[6], rk[7]);
                 }
-                oc = byte_of(O, 15);
-                ic = byte_of(I, 15);
-                O = On;
-                I = In;
             }
         }
         for (uint32_t p = max(pv, p0); p < base + r1; p++)

@@ -427,6 +427,7 @@ inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v)
 }
 
 inline uint32_t atomicAdd(uint32_t* p, uint32_t v) { const uint32_t o = *p; *p = o + v; return o; }
+inline uint32_t atomicSub(uint32_t* p, uint32_t v) { const uint32_t o = *p; *p = o - v; return o; }
 inline uint32_t atomicOr(uint32_t* p, uint32_t v) { const uint32_t o = *p; *p = o | v; return o; }
 inline uint32_t atomicAnd(uint32_t* p, uint32_t v) { const uint32_t o = *p; *p = o & v; return o; }
 inline unsigned long long atomicMax(unsigned long long* p, unsigned long long v)
