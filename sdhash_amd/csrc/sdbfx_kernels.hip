@@ -51,6 +51,13 @@
 // stay inside P (a tile holds at most LMAX - 8 positions)
 static_assert(((LMAX + RANK_LANES - 1) / RANK_LANES + 15) / 16 <= RUN_CHUNKS, "ranks run too long");
 static_assert((LMAX - 8 + 15) / 16 * 16 <= LMAX, "ranks chunk writes past P");
+// P stores position r (relative to the tile's base) at index pix(r): 8 bytes
+// of padding after every 64 positions, so lanes that each walk their own
+// 64-position block (F) or 64-step segment (scores) hit different banks.
+// Groups of 4 positions starting at a multiple of 4 stay contiguous.
+#define PIX(r) ((r) + (((r) >> 6) << 2))
+#define PSIZE (LMAX + 4 * ((LMAX + 63) / 64))
+__device__ __forceinline__ uint32_t pix(uint32_t r) { return PIX(r); }
 #define BCAND_CAP 1152   // block mode: positions with score >= 15 in <= 16320 positions
 #define NWAVE (SDBFX_NT / 64)
 
@@ -59,7 +66,7 @@ __constant__ uint16_t c_cut[SDBFX_CUTOFFS_LEN] = {SDBFX_CUTOFFS256_INIT};
 __constant__ uint32_t c_entropy[65] = {SDBFX_ENTROPY64_INIT};
 
 struct __align__(16) TileSmem {
-    uint16_t P[LMAX];                       // rank | (F - i) << 10, positions [base, base+L)
+    uint16_t P[PSIZE];                      // rank | (F - i) << 10, positions [base, base+L) at pix()
     union {
         uint8_t hist[RANK_LANES * 256];     // ranks phase: one byte histogram per lane
         struct {
@@ -153,9 +160,29 @@ __device__ __forceinline__ void sha1_64(const uint8_t* p, uint32_t out[5])
 
 // ------------------------------------------------------------------ helpers
 
+// Compiler controls; they change no result.  opaque() hides a value's origin
+// (so that a prefetch is not merged with the load it feeds) and makes it
+// needed at that point (so that a load issued early is not moved down to its
+// use).  The CPU emulation of the kernels (tests/hipemu, which defines
+// HIPEMU) uses the plain forms.
+#if defined(HIPEMU)
+__device__ __forceinline__ void opaque(uint32_t&) {}
+__device__ __forceinline__ void opaque(uint64_t&) {}
+#else
+__device__ __forceinline__ void opaque(uint32_t& x) { asm volatile("" : "+v"(x)); }
+__device__ __forceinline__ void opaque(uint64_t& x) { asm volatile("" : "+v"(x)); }
+#endif
+__device__ __forceinline__ void opaque(uint4& v)
+{
+    opaque(v.x);
+    opaque(v.y);
+    opaque(v.z);
+    opaque(v.w);
+}
+
 __device__ __forceinline__ uint32_t rank_at(const uint16_t* P, uint32_t base, uint32_t pos)
 {
-    return P[pos - base] & 0x3FFu;
+    return P[pix(pos - base)] & 0x3FFu;
 }
 
 // Canonical state before step i: the remembered minimum m matters only while
@@ -164,25 +191,6 @@ __device__ __forceinline__ uint32_t rank_at(const uint16_t* P, uint32_t base, ui
 __device__ __forceinline__ uint32_t canon(const uint16_t* P, uint32_t base, uint32_t i, uint32_t m)
 {
     return (m > i && rank_at(P, base, m) > 0) ? m : 0u;
-}
-
-// smallest t > q with bit t set in the 128-bit value hi:lo; 128 if none
-__device__ __forceinline__ uint32_t first_after(uint64_t lo, uint64_t hi, uint32_t q)
-{
-    uint32_t s = q + 1;
-    if (s < 64) {
-        uint64_t m = lo >> s;
-        if (m)
-            return s + (uint32_t)__builtin_ctzll(m);
-        if (hi)
-            return 64 + (uint32_t)__builtin_ctzll(hi);
-        return 128;
-    }
-    s -= 64;
-    if (s >= 64)
-        return 128;
-    uint64_t m = hi >> s;
-    return m ? 64 + s + (uint32_t)__builtin_ctzll(m) : 128u;
 }
 
 __device__ __forceinline__ uint32_t score_at(const TileSmem& sm, uint32_t rel)
@@ -279,12 +287,16 @@ __device__ __forceinline__ uint32_t select_bit(unsigned long long m, uint32_t r)
 // credits to one position are added at once.  With HAND, the canonical state
 // before step hand_i+1 is stored to sm.misc[2] when step hand_i runs.  Returns
 // the state after step i1-1.  The words a step reads do not depend on the
-// state, so four steps' words are loaded at once ahead of the branch-free
-// transitions.
+// state, so they are loaded ahead of the transitions: the two words of each
+// group of four steps two groups ahead, the four ranks at the groups' F
+// positions one group ahead (group indices clamped to the last group).  Only
+// the last group, when it is partial, checks step bounds.
 template <int SIGN, bool CREDIT, bool HAND>
 __device__ __forceinline__ uint32_t sm_run(TileSmem& sm, uint32_t base, uint32_t t0, uint32_t T, uint32_t i0,
                                            uint32_t i1, uint32_t m, uint32_t hand_i)
 {
+    if (i0 >= i1)
+        return m;
     const uint16_t* P = sm.P;
     uint32_t rm = m ? rank_at(P, base, m) : 0u;
     uint32_t cpos = 0xFFFFFFFFu, ccnt = 0;
@@ -293,27 +305,32 @@ __device__ __forceinline__ uint32_t sm_run(TileSmem& sm, uint32_t base, uint32_t
         if (ccnt && cpos >= t0 && rel < T)
             atomicAdd(&sm.u.s.score[rel >> 2], ((uint32_t)SIGN * ccnt) << (8 * (rel & 3)));
     };
-    for (uint32_t i = i0; i < i1; i += 4) {
-        const uint64_t W4 = *(const uint64_t*)(P + (i - base));
-        const uint64_t R4 = *(const uint64_t*)(P + (i + 64 - base));
-        uint32_t rf[4];
+    const uint32_t ilast = i0 + ((i1 - i0 - 1) & ~3u);   // first step of the last group
+    auto words = [&](uint32_t i, uint64_t& W, uint64_t& R) {
+        const uint32_t x = pix(i - base);
+        W = *(const uint64_t*)(P + x);
+        R = *(const uint64_t*)(P + x + 68);   // pix(i + 64 - base)
+    };
+    auto franks = [&](uint32_t i, uint64_t W, uint32_t rf[4]) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const uint32_t w = (uint32_t)(W4 >> (16 * j)) & 0xFFFFu;
-            rf[j] = P[i + j + (w >> 10) - base] & 0x3FFu;
-        }
+        for (int j = 0; j < 4; j++)
+            rf[j] = P[pix(i + j + (((uint32_t)(W >> (16 * j)) & 0xFFFFu) >> 10) - base)];
+    };
+    // steps [i, i + nv) of one group
+    auto group = [&](uint32_t i, uint64_t W4, uint64_t R4, const uint32_t rfw[4], uint32_t nv) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const uint32_t ii = i + j;
-            if (ii < i1) {
+            if ((uint32_t)j < nv) {
                 // cheap slide (sdbf_core.cc:61-68) or full rescan (:69-81)
                 const uint32_t w = (uint32_t)(W4 >> (16 * j)) & 0xFFFFu;
                 const uint32_t r64 = (uint32_t)(R4 >> (16 * j)) & 0x3FFu;
+                const uint32_t rf = rfw[j] & 0x3FFu;
                 const bool cheap = ii > 0 && rm > 0 && r64 >= rm && ii < m;
                 const uint32_t mc = r64 == rm ? ii + 64 : m;
                 m = cheap ? mc : ii + (w >> 10);
-                const bool has = cheap || rf[j] != 0;
-                rm = cheap ? rm : rf[j];
+                const bool has = cheap || rf != 0;
+                rm = cheap ? rm : rf;
                 if (CREDIT && has) {
                     if (m != cpos) {
                         flush();
@@ -326,7 +343,45 @@ __device__ __forceinline__ uint32_t sm_run(TileSmem& sm, uint32_t base, uint32_t
                     sm.misc[2] = canon(P, base, ii + 1, m);
             }
         }
+    };
+    // Two groups per iteration with two sets of rank registers, so that no
+    // register still being loaded is copied (a copy would wait for it).
+    uint64_t W0, R0, W1, R1, W2, R2;
+    uint32_t rfa[4], rfb[4];
+    words(i0, W0, R0);
+    franks(i0, W0, rfa);
+    words(min(i0 + 4, ilast), W1, R1);
+    uint32_t i = i0;
+    for (; i + 4 < ilast; i += 8) {
+        words(min(i + 8, ilast), W2, R2);
+        franks(i + 4, W1, rfb);
+        __builtin_amdgcn_sched_barrier(0);
+        group(i, W0, R0, rfa, 4);
+        opaque(W2);
+        opaque(R2);
+        words(min(i + 12, ilast), W0, R0);
+        franks(i + 8, W2, rfa);
+        __builtin_amdgcn_sched_barrier(0);
+        group(i + 4, W1, R1, rfb, 4);
+        opaque(W0);
+        opaque(R0);
+        W1 = W0;
+        R1 = R0;
+        W0 = W2;
+        R0 = R2;
     }
+    // one or two groups left: [i, i+4) and possibly the last group
+    if (i < ilast) {
+        franks(i + 4, W1, rfb);
+        group(i, W0, R0, rfa, 4);
+        i += 4;
+        W0 = W1;
+        R0 = R1;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            rfa[j] = rfb[j];
+    }
+    group(i, W0, R0, rfa, i1 - i);
     if (CREDIT)
         flush();
     return m;
@@ -424,7 +479,7 @@ __device__ __forceinline__ uint32_t tile_core(TileSmem& sm, const uint8_t* __res
                     e += sm.dtab[((w[j] >> (8 * (b & 3))) & 0xFFu) + 1];
                 }
             }
-            uint32_t* Pw = (uint32_t*)(sm.P + r0);
+            uint16_t* Pc = sm.P + pix(r0);
 #pragma unroll
             for (int c = 0; c < RUN_CHUNKS; c++) {
                 if ((uint32_t)c < nch) {
@@ -449,33 +504,44 @@ __device__ __forceinline__ uint32_t tile_core(TileSmem& sm, const uint8_t* __res
                         const uint32_t rv = p0 + 16 * c + j < pv ? r : 0u;
                         rk[j >> 1] = (j & 1) ? (rk[j >> 1] | rv << 16) : rv;
                     }
-                    *(uint4*)(Pw + 8 * c) = make_uint4(rk[0], rk[1], rk[2], rk[3]);
-                    *(uint4*)(Pw + 8 * c + 4) = make_uint4(rk[4], rk[5], rk[6], rk[7]);
+                    uint16_t* pc = Pc + 16 * c + 4 * ((r0 + 16 * c) >> 6) - 4 * (r0 >> 6);   // pix(r0 + 16c)
+#pragma unroll
+                    for (int g = 0; g < 4; g++)
+                        *(uint64_t*)(pc + 4 * g) = (uint64_t)rk[2 * g] | (uint64_t)rk[2 * g + 1] << 32;
                 }
             }
         }
         for (uint32_t p = max(pv, p0); p < base + r1; p++)
-            sm.P[p - base] = 0;
+            sm.P[pix(p - base)] = 0;
     }
     __syncthreads();
 
     // ---- 2. F (and clear the score words, which alias the histograms).
-    // One lane per 64-aligned block b.  A descending pass keeps the suffix
-    // minima of (rank << 6 | offset) over block b; an ascending pass carries
-    // the prefix minimum over block b+1 and its run-start bits, so the window
-    // starting at offset o takes the smaller rank of suffix[o] and prefix[o-1]
-    // (the suffix on ties: it is the earlier position).  Run-start bits of
-    // block b+1 at offsets >= o lie past the window and are never needed.
+    // One lane per 64-aligned block b.  The window starting at offset o of
+    // block b takes the smaller rank of the suffix minimum of block b over
+    // [o, 63] and the prefix minimum of block b+1 over [0, o-1] (the suffix on
+    // ties: it is the earlier position); F is the last position of the run of
+    // equal ranks that starts at that minimum, at most o + 63.  With da / db
+    // the run-start bits of blocks b / b+1, the run ends one before the first
+    // run start after the minimum.  A descending pass carries the suffix
+    // minimum together with that distance (the next run start after each
+    // offset is carried down the block); the ascending pass carries the prefix
+    // minimum and its distance, found from db by shifts that are constants of
+    // the unrolled loop.  Run-start bits past o + 64 never shorten a distance
+    // below the cap of 63, so a sentinel bit at db's top keeps every search
+    // bounded.
     for (int q = tid; q < SDBFX_TILE / 4; q += SDBFX_NT)
         sm.u.s.score[q] = 0;
     for (uint32_t blk = tid; blk * 64 < NI; blk += SDBFX_NT) {
         const uint32_t p0 = blk * 64;
+        uint16_t* Pa = sm.P + pix(p0);
+        const uint16_t* Pb = Pa + 68;   // pix(p0 + 64)
         uint64_t rw[16];
-        uint64_t da = 0;
-        uint32_t prev = p0 ? (sm.P[p0 - 1] & 0x3FFu) : 0xFFFFu;
+        uint64_t da = 0, db = 0;
+        uint32_t prev = p0 ? (Pa[-5] & 0x3FFu) : 0xFFFFu;   // pix(p0 - 1)
 #pragma unroll
         for (int q = 0; q < 16; q++) {
-            rw[q] = *(const uint64_t*)(sm.P + p0 + 4 * q);
+            rw[q] = *(const uint64_t*)(Pa + 4 * q);
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const uint32_t r = (uint32_t)(rw[q] >> (16 * j)) & 0x3FFu;
@@ -483,47 +549,80 @@ __device__ __forceinline__ uint32_t tile_core(TileSmem& sm, const uint8_t* __res
                 prev = r;
             }
         }
-        uint32_t suf2[32];
-        uint32_t sfx = 0xFFFFu;
-#pragma unroll
-        for (int o = 63; o >= 0; o--) {
-            const uint32_t r = (uint32_t)(rw[o >> 2] >> (16 * (o & 3))) & 0x3FFu;
-            sfx = min(sfx, r ? ((r << 6) | (uint32_t)o) : 0xFFFFu);
-            if (o & 1)
-                suf2[o >> 1] = sfx << 16;
-            else
-                suf2[o >> 1] |= sfx;
-        }
-        uint64_t db = 0;
-        uint32_t pfx = 0xFFFFu;
-        uint32_t prevb = prev;   // rank at offset 63 of block b
+        // block b+1: positions past L read as rank 0 and start no run
+        const uint32_t nvb = L > p0 + 64 ? min(L - p0 - 64, 64u) : 0u;   // valid positions
+        auto group_b = [&](int q) {
+            const int32_t rem = (int32_t)nvb - 4 * q;
+            const uint64_t keep = rem >= 4 ? ~0ull : rem <= 0 ? 0ull : (~0ull >> (64 - 16 * rem));
+            return rem > 0 ? *(const uint64_t*)(Pb + 4 * q) & keep : 0ull;
+        };
 #pragma unroll
         for (int q = 0; q < 16; q++) {
-            const uint32_t pq = p0 + 64 + 4 * q;
-            const uint64_t vb = pq < L ? *(const uint64_t*)(sm.P + pq) : 0ull;
+            const uint64_t vb = group_b(q);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t rb = (uint32_t)(vb >> (16 * j)) & 0x3FFu;
+                db |= (uint64_t)(rb != prev) << (4 * q + j);
+                prev = rb;
+            }
+        }
+        db &= nvb >= 64 ? ~0ull : (1ull << nvb) - 1;
+        // the passes below read bits and ranks back from these words rather
+        // than keeping 64 unpacked ranks and comparisons live
+        opaque(da);
+        opaque(db);
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+            opaque(rw[q]);
+        // Minima are taken over packed words rank' << 14 | offset << 8 | t,
+        // rank' = (rank - 1) mod 1024 (rank 0, which never wins, becomes
+        // 1023), t the first run start after that offset: one min selects
+        // the minimum and carries its t, with no lane masks.
+        // descending: suffix minimum rank' | distance from o to its run's end
+        uint32_t suf2[32];
+        {
+            uint32_t sfx = 0xFFFFFFFFu;
+            uint32_t nrs = db ? 64 + (uint32_t)__builtin_ctzll(db) : 128u;   // first run start after 63
+#pragma unroll
+            for (int o = 63; o >= 0; o--) {
+                if (o < 63) {
+                    const uint32_t bit = (uint32_t)(da >> (o + 1)) & 1u;
+                    nrs = min(nrs, (uint32_t)(o + 1 + 256) - (bit << 8));
+                }
+                const uint32_t r = (uint32_t)(rw[o >> 2] >> (16 * (o & 3))) & 0x3FFu;
+                sfx = min(sfx, (((r + 1023u) & 0x3FFu) << 14) | ((uint32_t)o << 8) | nrs);
+                const uint32_t v = (sfx >> 14) | (min((sfx & 0xFFu) - 1 - (uint32_t)o, 63u) << 10);
+                if (o & 1)
+                    suf2[o >> 1] = v << 16;
+                else
+                    suf2[o >> 1] |= v;
+            }
+        }
+        // ascending: the prefix minimum of block b+1 over [0, o-1]
+        uint32_t pfx = 0xFFFFFFFFu;
+        const uint64_t dbs = db | (1ull << 63);
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const uint64_t vb = group_b(q);   // read again: fewer live registers
             uint32_t out4[4];
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const int o = 4 * q + j;
                 const uint32_t r = (uint32_t)(rw[q] >> (16 * j)) & 0x3FFu;
-                uint32_t fr = 0;
-                if (r) {
-                    const uint32_t so = (o & 1) ? (suf2[o >> 1] >> 16) : (suf2[o >> 1] & 0xFFFFu);
-                    const uint32_t qq = (pfx >> 6) < (so >> 6) ? 64 + (pfx & 63u) : (so & 63u);
-                    const uint32_t t = first_after(da, db, qq);
-                    fr = min(t - 1, (uint32_t)o + 63) - o;
+                const uint32_t sv = (o & 1) ? (suf2[o >> 1] >> 16) : (suf2[o >> 1] & 0xFFFFu);
+                const uint32_t f =
+                    (pfx >> 14) < (sv & 0x3FFu) ? min((pfx & 0xFFu) - 1 - (uint32_t)o, 63u) : (sv >> 10);
+                out4[j] = r ? (r | (f << 10)) : 0u;
+                if (o < 63) {
+                    const uint32_t rb = (uint32_t)(vb >> (16 * j)) & 0x3FFu;
+                    const uint32_t nb = 64 + o + 1 + (uint32_t)__builtin_ctzll(dbs >> (o + 1));
+                    pfx = min(pfx, (((rb + 1023u) & 0x3FFu) << 14) | ((uint32_t)o << 8) | nb);
                 }
-                out4[j] = r | (fr << 10);
-                const uint32_t p = pq + j;
-                const uint32_t rb = p < L ? (uint32_t)(vb >> (16 * j)) & 0x3FFu : 0u;
-                pfx = min(pfx, rb ? ((rb << 6) | (uint32_t)o) : 0xFFFFu);
-                db |= (uint64_t)(p < L && rb != prevb) << o;
-                prevb = rb;
             }
             if (p0 + 4 * q < NI) {
                 const uint64_t v = (uint64_t)out4[0] | ((uint64_t)out4[1] << 16) | ((uint64_t)out4[2] << 32) |
                                    ((uint64_t)out4[3] << 48);
-                *(uint64_t*)(sm.P + p0 + 4 * q) = v;
+                *(uint64_t*)(Pa + 4 * q) = v;
             }
         }
     }
@@ -1316,26 +1415,14 @@ __device__ __forceinline__ sdbfx_v4i planes4(uint32_t w, int sh)
     return r;
 }
 
-// Compiler controls for the MFMA loop; they change no result.  opaque()
-// hides a value's origin so that a prefetch is not merged with the load it
-// feeds; in_agprs() places a 32x32 block of accumulators in AGPRs at that
-// point, so that its initial values are not held in VGPRs up to the loop.
-// The CPU emulation of the kernels (tests/hipemu, which defines HIPEMU) uses
-// the plain forms.
+// in_agprs() places a 32x32 block of accumulators in AGPRs at that point,
+// so that its initial values are not held in VGPRs up to the loop (plain form
+// under the CPU emulation).
 #if defined(HIPEMU)
-__device__ __forceinline__ void opaque(uint32_t&) {}
 __device__ __forceinline__ void in_agprs(sdbfx_v16i&) {}
 #else
-__device__ __forceinline__ void opaque(uint32_t& x) { asm volatile("" : "+v"(x)); }
 __device__ __forceinline__ void in_agprs(sdbfx_v16i& v) { asm volatile("" : "+a"(v)); }
 #endif
-__device__ __forceinline__ void opaque(uint4& v)
-{
-    opaque(v.x);
-    opaque(v.y);
-    opaque(v.z);
-    opaque(v.w);
-}
 
 // Index part of a filter with s elements in the cutoff table cutx: cutx[a + b]
 // is the reference's cutoff for two qualifying filters (>= 16 elements), 1250
