@@ -252,6 +252,12 @@ __device__ __forceinline__ uint32_t wave_excl_sum(uint32_t v, int lane, uint32_t
     return x + add - v;
 }
 
+// number of set bits of m below this lane's bit
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 // LDS written by some lanes of a wave, then read by others
 __device__ __forceinline__ void wave_lds_sync()
 {
@@ -464,41 +470,46 @@ __device__ __forceinline__ uint32_t tile_core(TileSmem& sm, const uint8_t* __res
     if (tid < RANK_LANES) {
         uint32_t* H = (uint32_t*)sm.u.hist + tid;
         if (p0 < pv) {
+            // the first window's 64 increments (in two halves) and its
+            // entropy from the counts they returned
             int32_t e = 0;
 #pragma unroll
-            for (int c = 0; c < 4; c++) {
-                uint32_t w[16];
+            for (int h = 0; h < 2; h++) {
+                uint32_t w[32];
 #pragma unroll
-                for (int j = 0; j < 16; j++) {
-                    const uint32_t b = byte_of(X[c], j);
-                    w[j] = atomicAdd(H + (b >> 2) * RANK_LANES, 1u << (8 * (b & 3)));
+                for (int k = 0; k < 32; k++) {
+                    const uint32_t b = byte_of(X[2 * h + (k >> 4)], k & 15);
+                    w[k] = atomicAdd(H + (b >> 2) * RANK_LANES, 1u << (8 * (b & 3)));
                 }
 #pragma unroll
-                for (int j = 0; j < 16; j++) {
-                    const uint32_t b = byte_of(X[c], j);
-                    e += sm.dtab[((w[j] >> (8 * (b & 3))) & 0xFFu) + 1];
+                for (int k = 0; k < 32; k++) {
+                    const uint32_t b = byte_of(X[2 * h + (k >> 4)], k & 15);
+                    e += sm.dtab[((w[k] >> (8 * (b & 3))) & 0xFFu) + 1];
                 }
             }
+            auto update = [&](int c, uint32_t* wo_, uint32_t* wi_) {
+#pragma unroll
+                for (int j = (c == 0); j < 16; j++) {
+                    const uint32_t bo = j ? byte_of(X[c], j - 1) : byte_of(X[c - 1], 15);
+                    const uint32_t bi = j ? byte_of(X[c + 4], j - 1) : byte_of(X[c + 3], 15);
+                    wo_[j] = atomicSub(H + (bo >> 2) * RANK_LANES, 1u << (8 * (bo & 3)));
+                    wi_[j] = atomicAdd(H + (bi >> 2) * RANK_LANES, 1u << (8 * (bi & 3)));
+                }
+            };
             uint16_t* Pc = sm.P + pix(r0);
 #pragma unroll
             for (int c = 0; c < RUN_CHUNKS; c++) {
                 if ((uint32_t)c < nch) {
-                    uint32_t wo[16], wi[16];
-#pragma unroll
-                    for (int j = (c == 0); j < 16; j++) {
-                        const uint32_t bo = j ? byte_of(X[c], j - 1) : byte_of(X[c - 1], 15);
-                        const uint32_t bi = j ? byte_of(X[c + 4], j - 1) : byte_of(X[c + 3], 15);
-                        wo[j] = atomicSub(H + (bo >> 2) * RANK_LANES, 1u << (8 * (bo & 3)));
-                        wi[j] = atomicAdd(H + (bi >> 2) * RANK_LANES, 1u << (8 * (bi & 3)));
-                    }
+                    uint32_t wo_[16], wi_[16];
+                    update(c, wo_, wi_);
                     uint32_t rk[8];
 #pragma unroll
                     for (int j = 0; j < 16; j++) {
                         if (c > 0 || j > 0) {
                             const uint32_t bo = j ? byte_of(X[c], j - 1) : byte_of(X[c - 1], 15);
                             const uint32_t bi = j ? byte_of(X[c + 4], j - 1) : byte_of(X[c + 3], 15);
-                            e += sm.dtab[((wi[j] >> (8 * (bi & 3))) & 0xFFu) + 1] -
-                                 sm.dtab[(wo[j] >> (8 * (bo & 3))) & 0xFFu];
+                            e += sm.dtab[((wi_[j] >> (8 * (bi & 3))) & 0xFFu) + 1] -
+                                 sm.dtab[(wo_[j] >> (8 * (bo & 3))) & 0xFFu];
                         }
                         const uint32_t r = sm.rtab[e >> 10];
                         const uint32_t rv = p0 + 16 * c + j < pv ? r : 0u;
@@ -697,60 +708,82 @@ __device__ __forceinline__ uint32_t tile_core(TileSmem& sm, const uint8_t* __res
 }
 
 // Ordered compaction of owned positions [0, cnt) whose score satisfies pred
-// into sm.u.s.cand as (position | score << 16).  256 positions per chunk,
-// four per lane, read one score word at a time.  Returns the count.
+// (pred(0) must be false) into sm.u.s.cand as (position | score << 16).  256 positions per chunk, four
+// per lane (one score word); each wave takes a contiguous range of chunks, so
+// one running offset per wave orders them.  A first pass counts each wave's
+// candidates from ballots, a second writes them: a lane's place within a
+// chunk is the number of candidates in the lanes below it (mbcnt over the
+// four byte ballots) plus its own earlier bytes.  Returns the count.
 template <typename Pred>
 __device__ __forceinline__ uint32_t compact(TileSmem& sm, uint32_t cnt, Pred pred)
 {
+    constexpr int MAXC = SDBFX_TILE / 256 / NWAVE;   // chunks per wave (cnt <= SDBFX_TILE)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t nchunk = (cnt + 255) / 256;
-    for (uint32_t c = wave; c < nchunk; c += NWAVE) {
-        const uint32_t q0 = c * 256 + lane * 4;
-        const uint32_t word = q0 < cnt ? sm.u.s.score[q0 >> 2] : 0u;
-        uint32_t k = 0;
+    const uint32_t per = (nchunk + NWAVE - 1) / NWAVE;
+    const uint32_t c0 = min((uint32_t)wave * per, nchunk), c1 = min(c0 + per, nchunk);
+    // the wave's score words, loaded once for both passes; bytes past the
+    // range read as score 0, which pred rejects for every caller
+    uint32_t word[MAXC];
+#pragma unroll
+    for (int k = 0; k < MAXC; k++) {
+        const uint32_t q0 = (c0 + k) * 256 + lane * 4;
+        const bool ok = c0 + k < c1 && q0 < cnt;
+        const uint32_t w = sm.u.s.score[ok ? q0 >> 2 : 0u];
+        const uint32_t rem = cnt - q0;
+        word[k] = ok ? (rem >= 4 ? w : w & ((1u << (8 * rem)) - 1)) : 0u;
+    }
+    auto ballots = [&](int k, unsigned long long b[4]) {
 #pragma unroll
         for (int j = 0; j < 4; j++)
-            k += (q0 + j < cnt && pred((word >> (8 * j)) & 0xFFu)) ? 1u : 0u;
-        uint32_t tot;
-        wave_excl_sum(k, lane, &tot);
-        if (lane == 0)
-            sm.chunk_cnt[c] = tot;
-    }
-    __syncthreads();
-    if (wave == 0) {
-        const uint32_t v = (uint32_t)lane < nchunk ? sm.chunk_cnt[lane] : 0u;
-        uint32_t tot;
-        const uint32_t ex = wave_excl_sum(v, lane, &tot);
-        if ((uint32_t)lane < nchunk)
-            sm.chunk_off[lane] = ex;
-        if (lane == 0)
-            sm.misc[6] = tot;
-    }
-    __syncthreads();
-    for (uint32_t c = wave; c < nchunk; c += NWAVE) {
-        const uint32_t q0 = c * 256 + lane * 4;
-        const uint32_t word = q0 < cnt ? sm.u.s.score[q0 >> 2] : 0u;
-        uint32_t k = 0;
+            b[j] = __ballot(pred((word[k] >> (8 * j)) & 0xFFu));
+    };
+    uint32_t mine = 0;
 #pragma unroll
-        for (int j = 0; j < 4; j++)
-            k += (q0 + j < cnt && pred((word >> (8 * j)) & 0xFFu)) ? 1u : 0u;
-        uint32_t tot;
-        uint32_t o = sm.chunk_off[c] + wave_excl_sum(k, lane, &tot);
+    for (int k = 0; k < MAXC; k++) {
+        unsigned long long b[4];
+        ballots(k, b);
+        mine += __popcll(b[0]) + __popcll(b[1]) + __popcll(b[2]) + __popcll(b[3]);
+        __builtin_amdgcn_sched_barrier(0);   // one chunk's ballots live at a time
+    }
+    opaque(mine);   // counted here, not inside the branch that stores it
+    if (lane == 0)
+        sm.chunk_cnt[wave] = mine;
+    // the second pass takes its ballots again rather than keeping 4 * MAXC
+    // of them in scalar registers
+#pragma unroll
+    for (int k = 0; k < MAXC; k++)
+        opaque(word[k]);
+    __syncthreads();
+    uint32_t o = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < NWAVE; w++) {
+        const uint32_t t = sm.chunk_cnt[w];
+        o += w < wave ? t : 0u;
+        total += t;
+    }
+#pragma unroll
+    for (int k = 0; k < MAXC; k++) {
+        unsigned long long b[4];
+        ballots(k, b);
+        uint32_t at = o + lanes_below(b[0]) + lanes_below(b[1]) + lanes_below(b[2]) + lanes_below(b[3]);
+        const uint32_t q0 = (c0 + k) * 256 + lane * 4;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const uint32_t sc = (word >> (8 * j)) & 0xFFu;
-            if (q0 + j < cnt && pred(sc))
-                sm.u.s.cand[o++] = (q0 + j) | (sc << 16);
+            if ((b[j] >> lane) & 1)
+                sm.u.s.cand[at++] = (q0 + j) | (((word[k] >> (8 * j)) & 0xFFu) << 16);
         }
+        o += __popcll(b[0]) + __popcll(b[1]) + __popcll(b[2]) + __popcll(b[3]);
+        __builtin_amdgcn_sched_barrier(0);
     }
     __syncthreads();
-    return sm.misc[6];
+    return total;
 }
 
 // ------------------------------------------------------------------ block mode
 
 // One workgroup per dd-block (sdbf_core.cc:252-282 per block, :305-319 tail).
-__global__ __launch_bounds__(SDBFX_NT) void block_digest_kernel(
+__global__ __launch_bounds__(SDBFX_NT, 2) void block_digest_kernel(
     const uint8_t* __restrict__ data, const BlockDesc* __restrict__ blocks, uint8_t* __restrict__ filters,
     uint16_t* __restrict__ elem_counts, uint16_t* __restrict__ hamming)
 {
@@ -779,19 +812,33 @@ __global__ __launch_bounds__(SDBFX_NT) void block_digest_kernel(
         }
     }
     __syncthreads();
-    if (tid == 0) {
+    if (wave == 0) {
+        // the reference's downward walk k = 65, 64, ... (sdbf_core.cc:271-276)
+        // stops at the first k whose count takes the running sum past the
+        // maximum; lane l holds k = 65 - l and the sum of the counts above it
         uint32_t k = SDBFX_THRESHOLD, sum = 0;
         int32_t allowed = SDBFX_MAX_ELEM_DD;
         if (!bd.tail) {
-            for (k = 65, sum = 0; k >= SDBFX_THRESHOLD; k--) {
-                if (sum <= SDBFX_MAX_ELEM_DD && sum + sm.histo[k] > SDBFX_MAX_ELEM_DD)
-                    break;
-                sum += sm.histo[k];
+            const bool in = lane <= 65 - SDBFX_THRESHOLD;
+            const uint32_t h = in ? sm.histo[65 - lane] : 0u;
+            uint32_t tot;
+            const uint32_t above = wave_excl_sum(h, lane, &tot);
+            const unsigned long long stop =
+                __ballot(in && above <= SDBFX_MAX_ELEM_DD && above + h > SDBFX_MAX_ELEM_DD);
+            if (stop) {
+                const int first = __builtin_ctzll(stop);
+                k = 65 - first;
+                sum = (uint32_t)__builtin_amdgcn_readlane((int)above, first);
+            } else {
+                k = SDBFX_THRESHOLD - 1;
+                sum = tot;
             }
             allowed = (int32_t)(SDBFX_MAX_ELEM_DD - sum);
         }
-        sm.misc[3] = k;
-        sm.misc[4] = (uint32_t)allowed;
+        if (lane == 0) {
+            sm.misc[3] = k;
+            sm.misc[4] = (uint32_t)allowed;
+        }
     }
     __syncthreads();
     const uint32_t k = sm.misc[3];
@@ -942,7 +989,7 @@ __device__ __forceinline__ void stream_tile(TileSmem& sm, const uint8_t* __restr
     __syncthreads();
 }
 
-__global__ __launch_bounds__(SDBFX_NT) void stream_tile_kernel(
+__global__ __launch_bounds__(SDBFX_NT, 2) void stream_tile_kernel(
     const uint8_t* __restrict__ data, const TileDesc* __restrict__ tiles, uint32_t* __restrict__ cand,
     TileState* __restrict__ tstate)
 {
@@ -1468,11 +1515,6 @@ __device__ __forceinline__ void merge_pair(uint32_t p32, uint32_t p64, uint32_t 
     const uint32_t num = match - cutoff, den = min(e1, e2) - cutoff;
     const unsigned long long key = ((unsigned long long)num << 22) / den;
     atomicMax(best, key << 32 | (unsigned long long)(num << 16 | den));
-}
-
-__device__ __forceinline__ uint32_t lanes_below(unsigned long long m)
-{
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
 // Per-wave LDS of cmp_mfma_kernel.
