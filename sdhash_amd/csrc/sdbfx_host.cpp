@@ -49,7 +49,7 @@ struct sdbfx_ctx {
     std::map<std::string, std::pair<double, uint64_t>> times;
     std::vector<hipEvent_t> evpool;
     DevBuf stage, blocks, tiles, tstate, units, files, cand, sfilt, sham, counts, gsrc, gdst, gcnt,
-        glast, groups, pairs, partial, imax, scores;
+        glast, groups, pairs, partial, imax, scores, gbest, rmap, rcode;
 };
 
 struct sdbfx_set {
@@ -492,6 +492,22 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
     }
     const bool mfma = use_mfma(ctx, ginst, gtgt);
     const uint32_t rows = mfma ? SDBFX_MROWS : SDBFX_CMP_ROWS;
+    // target filters per work item.  The MFMA kernel's fixed costs (row
+    // staging, publishing maxima) are per work item and its pruning works
+    // across the tiles of one, so it takes the largest chunk that still
+    // leaves a work item for every one of its 1024 wave slots.
+    uint32_t chunk = SDBFX_CMP_CHUNK;
+    if (mfma) {
+        for (uint32_t ch = 16384; ch > SDBFX_CMP_CHUNK; ch >>= 1) {
+            uint64_t n = 0;
+            for (size_t g = 0; g < ginst.size(); g++)
+                n += (ginst[g] + rows - 1) / rows * ((gtgt[g] + ch - 1) / ch);
+            if (n >= 1024) {
+                chunk = ch;
+                break;
+            }
+        }
+    }
     std::vector<CmpPair> pairs(npairs);
     std::vector<CmpGroup> groups;
     uint64_t inst = 0, wi = 0, partial_n = 0;
@@ -503,7 +519,8 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
         CmpGroup g{};
         g.tgt_first = T->first[t.idx];
         g.tgt_count = T->bfc[t.idx];
-        g.nch = (g.tgt_count + SDBFX_CMP_CHUNK - 1) / SDBFX_CMP_CHUNK;
+        g.chunk = chunk;
+        g.nch = (g.tgt_count + chunk - 1) / chunk;
         g.inst_begin = inst;
         g.pair_begin = kc[q];
         g.pair_end = kc[q + 1];
@@ -541,11 +558,20 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
     SetView va{(const uint32_t*)S[0]->filt.p, (const uint16_t*)S[0]->ecf.p, (const uint16_t*)S[0]->hamf.p};
     SetView vb{(const uint32_t*)S[1]->filt.p, (const uint16_t*)S[1]->ecf.p, (const uint16_t*)S[1]->hamf.p};
     if (wi && mfma) {
+        // per-instance maxima shared by the work items for pruning (from
+        // 0), and the row data of every instance
+        if ((rc = ensure(ctx, ctx->gbest, inst * 8 + 256)) || (rc = ensure(ctx, ctx->rmap, inst * 8 + 256)) ||
+            (rc = ensure(ctx, ctx->rcode, inst * 4 + 256)))
+            return rc;
+        HIPC(ctx, hipMemsetAsync(ctx->gbest.p, 0, inst * 8, ctx->stream));
+        LAUNCH(ctx, inst_map_kernel, (uint32_t)((inst + 255) / 256), 256, va, vb, (const CmpPair*)ctx->pairs.p,
+               (uint32_t)npairs, inst, (uint64_t*)ctx->rmap.p, (uint32_t*)ctx->rcode.p);
         // one wave per SIMD (the 128 x 128 tile holds 256 accumulators per
         // lane): a 4-wave workgroup per CU, a multiple of the 8 XCDs
         const uint32_t grid = (uint32_t)std::min<uint64_t>(((wi + 3) / 4 + 7) & ~7ull, 256);
         LAUNCH(ctx, cmp_mfma_kernel, grid, 256, va, vb, (const CmpGroup*)ctx->groups.p, (uint32_t)groups.size(),
-               (const CmpPair*)ctx->pairs.p, wi, (uint32_t*)ctx->partial.p);
+               (const uint64_t*)ctx->rmap.p, (const uint32_t*)ctx->rcode.p, wi, (uint32_t*)ctx->partial.p,
+               (unsigned long long*)ctx->gbest.p);
     } else if (wi) {
         const uint32_t grid = (uint32_t)std::min<uint64_t>(wi, 16384);
         LAUNCH(ctx, cmp_kernel, grid, 64, va, vb, (const CmpGroup*)ctx->groups.p, (uint32_t)groups.size(),
@@ -599,7 +625,8 @@ void sdbfx_ctx_destroy(sdbfx_ctx* ctx)
     (void)hipStreamSynchronize(ctx->stream);
     DevBuf* bufs[] = {&ctx->stage, &ctx->blocks, &ctx->tiles, &ctx->tstate, &ctx->units, &ctx->files,
                       &ctx->cand, &ctx->sfilt, &ctx->sham, &ctx->counts, &ctx->gsrc, &ctx->gdst,
-                      &ctx->gcnt, &ctx->glast, &ctx->groups, &ctx->pairs, &ctx->partial, &ctx->imax, &ctx->scores};
+                      &ctx->gcnt, &ctx->glast, &ctx->groups, &ctx->pairs, &ctx->partial, &ctx->imax, &ctx->scores,
+                      &ctx->gbest, &ctx->rmap, &ctx->rcode};
     for (DevBuf* b : bufs)
         if (b->p)
             (void)hipFree(b->p);

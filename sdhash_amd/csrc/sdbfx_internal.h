@@ -32,7 +32,8 @@
 #define SDBFX_PAD 256            // alignment / padding of inputs in device memory
 
 // compare
-#define SDBFX_CMP_CHUNK 1024     // target filters per compare work item
+#define SDBFX_CMP_CHUNK 1024     // target filters per compare work item (VALU kernel; the MFMA
+                                 // kernel takes 1024-16384 per call)
 #define SDBFX_CMP_ROWS 64        // reference-filter instances per work item, VALU compare
 #define SDBFX_MROWS 128          // reference-filter instances per work item, MFMA compare
 
@@ -88,7 +89,7 @@ struct CmpGroup {
     uint64_t wi_begin;     // first work item of the group
     uint64_t partial_begin;// first partial-maximum slot of the group
     uint32_t tgt_set;      // 0: target filters in set A, 1: in set B
-    uint32_t pad;
+    uint32_t chunk;        // target filters per work item (a multiple of 128)
 };
 
 // Compare: one (reference digest, target digest) pair after orientation.

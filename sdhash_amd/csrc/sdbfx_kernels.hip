@@ -1366,8 +1366,8 @@ __global__ __launch_bounds__(64) void cmp_kernel(
         }
 
         uint32_t best_n = 0, best_d = 0;
-        const uint64_t j0 = g.tgt_first + (uint64_t)c * SDBFX_CMP_CHUNK;
-        const uint64_t j1 = min(j0 + SDBFX_CMP_CHUNK, g.tgt_first + g.tgt_count);
+        const uint64_t j0 = g.tgt_first + (uint64_t)c * g.chunk;
+        const uint64_t j1 = min(j0 + g.chunk, g.tgt_first + g.tgt_count);
         const uint32_t* __restrict__ tf = g.tgt_set ? B.filt : A.filt;
         const uint16_t* __restrict__ tec = g.tgt_set ? B.ecf : A.ecf;
         const uint16_t* __restrict__ tham = g.tgt_set ? B.hamf : A.hamf;
@@ -1444,7 +1444,7 @@ typedef int sdbfx_v4i __attribute__((vector_size(16)));
 typedef int sdbfx_v16i __attribute__((vector_size(64)));
 
 #define MF_B 4                                  // 32-wide blocks per tile side
-#define MF_CAND_CAP 768                         // candidates listed per tile
+#define MF_CAND_CAP 704                         // candidates listed per tile
 
 __device__ __forceinline__ uint32_t word_of(const uint4& v, int k)
 {
@@ -1526,13 +1526,54 @@ struct MfmaWaveSmem {
     const uint32_t* rptr[SDBFX_MROWS];              // reference filter of each row
     unsigned long long best[SDBFX_MROWS];
     uint32_t clist[MF_CAND_CAP];    // row | column << 7
+    uint32_t bfix[SDBFX_MROWS];     // per row: floor(best fraction * 2^16), or its initial accumulator
 };
+
+// One instance per thread: where its reference filter is, and its size code
+// and Hamming weight (the row data of cmp_mfma_kernel).
+__global__ void inst_map_kernel(SetView A, SetView B, const CmpPair* __restrict__ pairs, uint32_t npairs,
+                                uint64_t ninst, uint64_t* __restrict__ rmap, uint32_t* __restrict__ rcode)
+{
+    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (t >= ninst)
+        return;
+    uint32_t lo = 0, hi = npairs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pairs[mid].inst_begin <= t)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const CmpPair P = pairs[lo];
+    const SetView R = P.ref_set ? B : A;
+    const uint64_t rf = P.ref_first + (t - P.inst_begin);
+    rmap[t] = rf | (uint64_t)P.ref_set << 63;
+    rcode[t] = size_code(R.ecf[rf]) | (uint32_t)R.hamf[rf] << 16;
+}
 
 // Four waves per workgroup, one per SIMD, each taking work items on its own;
 // they share only the cutoff table.
+//
+// Pruning.  Only the maximum fraction of each reference filter (row) is
+// needed.  A pair (i, j) can raise row i's maximum b only if its
+// match - cutoff exceeds b * (min(e1, e2) - cutoff), so its accumulator
+// starts m = floor(B * (min(e1, e2) - cutoff) / 2^16) lower, with
+// B = floor(b * 2^16): only pairs with match - cutoff > m become candidates,
+// and every pair that could raise the maximum is one.  b is the row's best so
+// far: this work item's, seeded from gbest, the per-instance maximum that all
+// work items of the call publish when they finish (any value there is a real
+// pair's fraction of the same instance and target digest, so taking it into
+// this chunk's partial maximum changes no result).
+//
+// Latency.  The last K step of a tile loads the first words of the next tile
+// of the work item, and the next tile's column data is loaded before the K
+// loop, so that consecutive tiles follow each other without waiting for
+// memory.
 __global__ __launch_bounds__(256) void cmp_mfma_kernel(
     SetView A, SetView B, const CmpGroup* __restrict__ groups, uint32_t ngroups,
-    const CmpPair* __restrict__ pairs, uint64_t nwork, uint32_t* __restrict__ partial)
+    const uint64_t* __restrict__ rmap, const uint32_t* __restrict__ rcode, uint64_t nwork,
+    uint32_t* __restrict__ partial, unsigned long long* __restrict__ gbest)
 {
     __shared__ int16_t ncut[4097];                  // -(cutoff + 1) by size_code(s1) + size_code(s2)
     __shared__ MfmaWaveSmem wsm[4];
@@ -1550,6 +1591,8 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
     const uint32_t** const rptr = wsm[wv].rptr;
     unsigned long long* const best = wsm[wv].best;
     uint32_t* const clist = wsm[wv].clist;
+    uint32_t* const bfix = wsm[wv].bfix;
+    const char* const ncutb = (const char*)ncut;
 
     // each XCD (workgroups are dealt round-robin over 8) takes a contiguous
     // range of work items, so neighbouring items share its L2
@@ -1566,125 +1609,236 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
                 hi = mid;
         }
         const CmpGroup g = groups[lo];
+        // chunk-major: the items of one row block are nblk apart, so its
+        // later chunks start from the maxima its earlier ones published
         const uint64_t local = w - g.wi_begin;
-        const uint32_t b = (uint32_t)(local / g.nch);
-        const uint32_t c = (uint32_t)(local % g.nch);
+        const uint32_t c = (uint32_t)(local / g.nblk);
+        const uint32_t b = (uint32_t)(local % g.nblk);
 
         // rows: instance b*128 + 32*bm + r32 (lanes r32 and r32+32 share it)
+        if (h == 0) {
 #pragma unroll
-        for (int bm = 0; bm < MF_B; bm++) {
-            const uint32_t row = 32 * bm + r32;
-            const uint32_t li = b * SDBFX_MROWS + row;
-            const bool valid = li < g.inst_count;
-            const uint64_t inst = g.inst_begin + (valid ? li : g.inst_count - 1);
-            uint32_t plo = g.pair_begin, phi = g.pair_end;
-            while (phi - plo > 1) {
-                const uint32_t mid = (plo + phi) >> 1;
-                if (pairs[mid].inst_begin <= inst)
-                    plo = mid;
-                else
-                    phi = mid;
-            }
-            const CmpPair pr = pairs[plo];
-            const SetView R = pr.ref_set ? B : A;
-            const uint64_t rf = pr.ref_first + (inst - pr.inst_begin);
-            const uint32_t* f = R.filt + rf * SDBFX_BFW;
-            if (h == 0) {
-                rinfo[row] = valid ? (size_code(R.ecf[rf]) | (uint32_t)R.hamf[rf] << 16) : size_code(0);
-                rptr[row] = f;
-                best[row] = 0;
+            for (int bm = 0; bm < MF_B; bm++) {
+                const uint32_t row = 32 * bm + r32;
+                const uint32_t li = b * SDBFX_MROWS + row;
+                const bool valid = li < g.inst_count;
+                const uint64_t inst = g.inst_begin + (valid ? li : g.inst_count - 1);
+                const uint64_t rm = rmap[inst];
+                const SetView R = (rm >> 63) ? B : A;
+                rptr[row] = R.filt + (rm & ~(1ull << 63)) * SDBFX_BFW;
+                rinfo[row] = valid ? rcode[inst] : size_code(0);
+                best[row] = valid ? gbest[inst] : 0ull;
             }
         }
         wave_lds_sync();
-#pragma unroll 4
-        for (uint32_t k = lane; k < SDBFX_MROWS * 16; k += 64) {
-            const uint32_t r = k >> 4, ch = k & 15;
-            atile[r * 16 + (ch ^ (r & 15))] = ((const uint4*)rptr[r])[ch];
+        // 16 loads in flight per lane, then their LDS stores
+#pragma unroll
+        for (uint32_t k0 = 0; k0 < SDBFX_MROWS * 16; k0 += 64 * 16) {
+            uint4 v[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const uint32_t k = k0 + lane + 64 * i;
+                v[i] = ((const uint4*)rptr[k >> 4])[k & 15];
+            }
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const uint32_t k = k0 + lane + 64 * i, r = k >> 4, ch = k & 15;
+                atile[r * 16 + (ch ^ (r & 15))] = v[i];
+            }
         }
         wave_lds_sync();
 
-        const uint64_t j0 = g.tgt_first + (uint64_t)c * SDBFX_CMP_CHUNK;
-        const uint64_t j1 = min(j0 + SDBFX_CMP_CHUNK, g.tgt_first + g.tgt_count);
+        const uint64_t j0 = g.tgt_first + (uint64_t)c * g.chunk;
+        const uint32_t nj = (uint32_t)min((uint64_t)g.chunk, g.tgt_count - (uint64_t)c * g.chunk);
         const uint32_t* __restrict__ tf = g.tgt_set ? B.filt : A.filt;
         const uint16_t* __restrict__ tec = g.tgt_set ? B.ecf : A.ecf;
         const uint16_t* __restrict__ tham = g.tgt_set ? B.hamf : A.hamf;
+        // the work item's target filters as 16-byte chunks: column j, chunk q at j * 16 + q
+        const uint4* __restrict__ tbase = (const uint4*)(tf + j0 * SDBFX_BFW);
         bool anyq = false;
 
-        for (uint64_t nt = j0; nt < j1; nt += 32 * MF_B) {
-            // target chunk index of each lane's column, relative to column nt
-            const uint4* __restrict__ tb = (const uint4*)(tf + nt * SDBFX_BFW);
-            const uint32_t ncol = (uint32_t)min((uint64_t)(32 * MF_B), j1 - nt);
-            uint32_t boff[MF_B], ccode[MF_B];
+        // column data of a tile (element count, Hamming weight) and the
+        // uint4 index of each lane's first chunk (columns past the end
+        // repeat the last one)
+        auto col_load = [&](uint32_t t0, uint32_t* s2, uint32_t* e2, uint32_t* cb) {
+            const uint32_t ncol = min(128u, nj - t0);
 #pragma unroll
             for (int bn = 0; bn < MF_B; bn++) {
                 const uint32_t col = 32 * bn + r32;
                 const bool ok = col < ncol;
-                const uint64_t j = nt + col;
-                boff[bn] = (ok ? col : ncol - 1) * 16 + h;
-                const uint32_t s2 = ok ? tec[j] : 0u;
-                anyq |= s2 >= SDBFX_MIN_ELEM;
-                ccode[bn] = 2 * size_code(s2);
+                s2[bn] = ok ? tec[j0 + t0 + col] : 0u;
+                e2[bn] = ok ? tham[j0 + t0 + col] : 0u;
+                cb[bn] = (t0 + (ok ? col : ncol - 1)) * 16 + h;
             }
-            const char* ncutb = (const char*)ncut;
+        };
+        uint32_t s2[MF_B], e2[MF_B], cb[MF_B];
+        col_load(0, s2, e2, cb);
+        // words 8q + 4h .. 8q + 4h + 3 of a row or column are its chunk 2q + h
+        uint4 aw[MF_B], bw[MF_B];
+        bool pre = false;       // aw, bw already hold this tile's first words
 
-            // accumulators start at -(cutoff + 1) of their pair, so that
-            // after the 2048 bits they are >= 0 exactly when count > cutoff
+        for (uint32_t t0 = 0; t0 < nj; t0 += 32 * MF_B) {
+            const uint32_t tn0 = t0 + 32 * MF_B < nj ? t0 + 32 * MF_B : t0;   // the next tile (or this one)
+            uint32_t ccode[MF_B], sx = 0, cmn = 0xFFFFu, cmx = 0;
+#pragma unroll
+            for (int bn = 0; bn < MF_B; bn++) {
+                const bool q = s2[bn] >= SDBFX_MIN_ELEM;
+                anyq |= q;
+                sx |= q;
+                ccode[bn] = 2 * size_code(s2[bn]);
+                cmn = min(cmn, ccode[bn]);
+                cmx = max(cmx, ccode[bn]);
+            }
+            uint32_t e2c[MF_B], cbt[MF_B];
+#pragma unroll
+            for (int bn = 0; bn < MF_B; bn++) {
+                e2c[bn] = e2[bn] + 1;
+                cbt[bn] = cb[bn];
+            }
+            // the next tile's column data, consumed after this tile
+            col_load(tn0, s2, e2, cb);
+            // no qualifying target filter: no pair of the tile can pass
+            if (__ballot(sx != 0) == 0) {
+                pre = false;
+                continue;
+            }
+            // all columns of one size code: the cutoff depends on the row only
+            const uint32_t umin = (uint32_t)__builtin_amdgcn_readlane((int)wave_prefix_min(cmn, lane), 63);
+            const uint32_t umax =
+                0xFFFFu - (uint32_t)__builtin_amdgcn_readlane((int)wave_prefix_min(0xFFFFu - cmx, lane), 63);
+            const bool uniform = umin == umax;
+            if (!pre) {
+#pragma unroll
+                for (int k = 0; k < MF_B; k++) {
+                    aw[k] = atile[(32 * k + r32) * 16 + (h ^ (r32 & 15))];
+                    bw[k] = tbase[cbt[k]];
+                }
+            }
+
+            // the smallest qualifying Hamming weight (+1) of the tile
+            uint32_t emn = 0xFFFFu;
+#pragma unroll
+            for (int bn = 0; bn < MF_B; bn++)
+                emn = min(emn, ccode[bn] < 4096u ? e2c[bn] : 0xFFFFu);
+            const uint32_t e2min = (uint32_t)__builtin_amdgcn_readlane((int)wave_prefix_min(emn, lane), 63);
+            // each row's best fraction so far as B = floor(b * 2^16) (two rows
+            // per lane).  When all columns share one size code, the row's
+            // initial accumulator itself: -(cutoff + 1) - m with the margin
+            // from the smallest Hamming weight of the tile, which bounds every
+            // pair's margin from below.
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const uint32_t row = lane + 64 * k;
+                const uint32_t key = (uint32_t)best[row];
+                const uint32_t B = key ? ((key >> 16) << 16) / (key & 0xFFFFu) : 0u;
+                if (uniform) {
+                    const uint32_t x = rinfo[row];
+                    const int n = *(const int16_t*)(ncutb + 2 * (x & 0xFFFFu) + umin);
+                    const int d = (int)min((x >> 16) + 1, e2min) + n;
+                    bfix[row] = (uint32_t)(n - (int)((B * (uint32_t)max(d, 0)) >> 16));
+                } else {
+                    bfix[row] = B;
+                }
+            }
+            wave_lds_sync();
+
+            // accumulators start at -(cutoff + 1) - m of their pair, so that
+            // after the 2048 bits they are >= 0 exactly when
+            // count - cutoff > m.  m is computed with a 24-bit multiply; when
+            // min(e1, e2) <= cutoff (d <= 0) the pair cannot pass whatever m
+            // is, and the wrapped product still gives m >= 0.
             sdbfx_v16i acc[MF_B][MF_B];
-            uint32_t rb = 8 * h;
+            uint32_t rb = h;
             opaque(rb);
 #pragma unroll
             for (int bm = 0; bm < MF_B; bm++) {
-                uint32_t rc[16];
-#pragma unroll
-                for (int i = 0; i < 16; i++)
-                    rc[i] = 2 * ((const uint16_t*)rinfo)[rb + 64 * bm + 16 * (i >> 2) + 2 * (i & 3)];
-#pragma unroll
-                for (int bn = 0; bn < MF_B; bn++) {
+                if (uniform) {
+                    int v[16];
 #pragma unroll
                     for (int i = 0; i < 16; i++)
-                        acc[bm][bn][i] = *(const int16_t*)(ncutb + rc[i] + ccode[bn]);
-                    in_agprs(acc[bm][bn]);
-                    __builtin_amdgcn_sched_barrier(0);
+                        v[i] = (int)bfix[4 * rb + 32 * bm + 8 * (i >> 2) + (i & 3)];
+#pragma unroll
+                    for (int bn = 0; bn < MF_B; bn++) {
+#pragma unroll
+                        for (int i = 0; i < 16; i++)
+                            acc[bm][bn][i] = v[i];
+                        in_agprs(acc[bm][bn]);
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
+                } else {
+                    uint32_t rc[16], e1c[16], bq[16];
+#pragma unroll
+                    for (int i = 0; i < 16; i++) {
+                        const uint32_t row = 4 * rb + 32 * bm + 8 * (i >> 2) + (i & 3);
+                        const uint32_t x = rinfo[row];
+                        rc[i] = 2 * (x & 0xFFFFu);
+                        e1c[i] = (x >> 16) + 1;
+                        bq[i] = bfix[row];
+                    }
+#pragma unroll
+                    for (int bn = 0; bn < MF_B; bn++) {
+#pragma unroll
+                        for (int i = 0; i < 16; i++) {
+                            const int n = *(const int16_t*)(ncutb + rc[i] + ccode[bn]);
+                            const uint32_t d = min(e1c[i], e2c[bn]) + (uint32_t)n;
+                            acc[bm][bn][i] = n - (int)(((bq[i] & 0xFFFFFFu) * (d & 0xFFFFFFu)) >> 16);
+                        }
+                        in_agprs(acc[bm][bn]);
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
                 }
             }
-            // words 8q + 4h .. 8q + 4h + 3 of a row or column are its chunk 2q + h
-            uint4 aw[MF_B], bw[MF_B];
+
+            // bit-planes of the first step
+            sdbfx_v4i av[MF_B], bv[MF_B];
 #pragma unroll
             for (int k = 0; k < MF_B; k++) {
-                aw[k] = atile[(32 * k + r32) * 16 + (h ^ (r32 & 15))];
-                bw[k] = tb[boff[k]];
-                opaque(aw[k]);
-                opaque(bw[k]);
+                av[k] = planes4(aw[k].x, 0);
+                bv[k] = planes4(bw[k].x, 0);
             }
 #pragma unroll 1
             for (int q = 0; q < 8; q++) {
-                const int qn = q < 7 ? q + 1 : 7;
-                // prefetch of the next 8 words, consumed one iteration later
+                // prefetch of the next 8 words, consumed one iteration later;
+                // the last iteration fetches the next tile's first words
+                const int qn = q < 7 ? q + 1 : 0;
                 uint32_t ao = (uint32_t)(r32 * 16 + ((2 * qn + h) ^ (r32 & 15)));
-                uint32_t bo = (uint32_t)(2 * qn);
                 opaque(ao);
-                opaque(bo);
                 uint4 an[MF_B], bnx[MF_B];
 #pragma unroll
                 for (int k = 0; k < MF_B; k++) {
                     an[k] = atile[ao + 32 * 16 * k];
-                    bnx[k] = tb[boff[k] + bo];
+                    bnx[k] = tbase[q < 7 ? cbt[k] + 2 * qn : cb[k]];
                 }
                 __builtin_amdgcn_sched_barrier(0);
+                // step t: word t / 2, planes 4 (t % 2) .. 4 (t % 2) + 3.  Each
+                // MFMA of step t is followed by two plane elements (four VALU
+                // instructions) of step t + 1, so that they run in its shadow.
 #pragma unroll
                 for (int t = 0; t < 8; t++) {
-                    const int sh = 4 * (t & 1);
-                    sdbfx_v4i av[MF_B], bv[MF_B];
+                    const int shn = 4 * ((t + 1) & 1);
+                    uint32_t wn[2 * MF_B];
 #pragma unroll
                     for (int k = 0; k < MF_B; k++) {
-                        av[k] = planes4(word_of(aw[k], t >> 1), sh);
-                        bv[k] = planes4(word_of(bw[k], t >> 1), sh);
+                        wn[k] = t < 7 ? word_of(aw[k], (t + 1) >> 1) : an[k].x;
+                        wn[MF_B + k] = t < 7 ? word_of(bw[k], (t + 1) >> 1) : bnx[k].x;
+                    }
+                    sdbfx_v4i nv[2 * MF_B];
+#pragma unroll
+                    for (int gi = 0; gi < MF_B * MF_B; gi++) {
+                        const int bm = gi / MF_B, bn = gi % MF_B;
+                        acc[bm][bn] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[bm], bv[bn], acc[bm][bn], 0, 0, 0);
+#pragma unroll
+                        for (int e2i = 0; e2i < 2; e2i++) {
+                            const int el = 2 * gi + e2i, p = el >> 2, e = el & 3;
+                            nv[p][e] = (int)((wn[p] >> (shn + e)) & 0x01010101u);
+                        }
+                        __builtin_amdgcn_sched_barrier(0);
                     }
 #pragma unroll
-                    for (int bm = 0; bm < MF_B; bm++)
-#pragma unroll
-                        for (int bn = 0; bn < MF_B; bn++)
-                            acc[bm][bn] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[bm], bv[bn], acc[bm][bn], 0, 0, 0);
+                    for (int k = 0; k < MF_B; k++) {
+                        av[k] = nv[k];
+                        bv[k] = nv[MF_B + k];
+                    }
                 }
 #pragma unroll
                 for (int k = 0; k < MF_B; k++) {
@@ -1692,6 +1846,7 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
                     bw[k] = bnx[k];
                 }
             }
+            pre = true;
 
             // candidates: accumulators >= 0, one bit per accumulator, from
             // the sign bits (shifted in by v_alignbit).  Word w holds blocks
@@ -1700,16 +1855,16 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
             uint32_t cbits[MF_B * MF_B / 2];
             uint32_t anyc = 0;
 #pragma unroll
-            for (int w = 0; w < MF_B * MF_B / 2; w++) {
+            for (int wi = 0; wi < MF_B * MF_B / 2; wi++) {
                 uint32_t x = 0;
 #pragma unroll
                 for (int k = 0; k < 2; k++) {
-                    const int bm = (2 * w + k) / MF_B, bn = (2 * w + k) % MF_B;
+                    const int bm = (2 * wi + k) / MF_B, bn = (2 * wi + k) % MF_B;
 #pragma unroll
                     for (int i = 0; i < 16; i++)
                         x = __builtin_amdgcn_alignbit(x, (uint32_t)acc[bm][bn][i], 31);
                 }
-                cbits[w] = ~x;
+                cbits[wi] = ~x;
                 anyc |= ~x;
             }
             if (__ballot(anyc != 0) == 0)
@@ -1718,21 +1873,22 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
             // MF_CAND_CAP of them per round
             uint32_t own = 0;
 #pragma unroll
-            for (int w = 0; w < MF_B * MF_B / 2; w++)
-                own += (uint32_t)__popc(cbits[w]);
+            for (int wi = 0; wi < MF_B * MF_B / 2; wi++)
+                own += (uint32_t)__popc(cbits[wi]);
             uint32_t ncand;
             const uint32_t first = wave_excl_sum(own, lane, &ncand);
+            const uint64_t nt = j0 + t0;
             for (uint32_t round = 0; round < ncand; round += MF_CAND_CAP) {
                 uint32_t pos = first;
                 if (pos < round + MF_CAND_CAP && pos + own > round) {
 #pragma unroll
-                    for (int w = 0; w < MF_B * MF_B / 2; w++) {
-                        uint32_t x = cbits[w];
+                    for (int wi = 0; wi < MF_B * MF_B / 2; wi++) {
+                        uint32_t x = cbits[wi];
                         while (x) {
                             const uint32_t bit = 31u - (uint32_t)__clz(x);
                             x ^= 1u << bit;
                             if (pos >= round && pos < round + MF_CAND_CAP) {
-                                const uint32_t blk = 2 * w + (bit < 16 ? 1 : 0), i = (bit < 16 ? 15u : 31u) - bit;
+                                const uint32_t blk = 2 * wi + (bit < 16 ? 1 : 0), i = (bit < 16 ? 15u : 31u) - bit;
                                 const uint32_t row = 32 * (blk / MF_B) + 8 * (i >> 2) + 4 * h + (i & 3);
                                 clist[pos - round] = row | (32 * (blk % MF_B) + r32) << 7;
                             }
@@ -1769,6 +1925,8 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
                 if (li < g.inst_count) {
                     const unsigned long long v = best[row];
                     partial[g.partial_begin + (uint64_t)li * g.nch + c] = !hq ? 0xFFFFFFFFu : v ? (uint32_t)v : 1u;
+                    if (v)
+                        atomicMax(&gbest[g.inst_begin + li], v);
                 }
             }
         }
