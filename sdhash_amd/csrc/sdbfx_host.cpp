@@ -50,6 +50,11 @@ struct sdbfx_ctx {
     std::vector<hipEvent_t> evpool;
     DevBuf stage, blocks, tiles, tstate, units, files, cand, sfilt, sham, counts, gsrc, gdst, gcnt,
         glast, groups, pairs, partial, imax, scores, gbest, rmap, rcode;
+    // two pinned host buffers for staging caller memory to the device
+    void* pin[2] = {nullptr, nullptr};
+    hipEvent_t pin_ev[2] = {nullptr, nullptr};
+    bool pin_busy[2] = {false, false};
+    int pin_next = 0;
 };
 
 struct sdbfx_set {
@@ -247,10 +252,15 @@ int digest_pass(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t i0, uint32_t i1, const 
         if (!tiles.empty())
             LAUNCH(ctx, stream_tile_kernel, (uint32_t)tiles.size(), SDBFX_NT, d_base,
                    (const TileDesc*)ctx->tiles.p, (uint32_t*)ctx->cand.p, (TileState*)ctx->tstate.p);
-        if (!units.empty())
+        if (!units.empty()) {
+            LAUNCH(ctx, stream_fixup_plan_kernel, (uint32_t)((tiles.size() + 255) / 256), 256,
+                   (const TileDesc*)ctx->tiles.p, (uint32_t)tiles.size(), (TileState*)ctx->tstate.p);
+            LAUNCH(ctx, stream_fixup_par_kernel, (uint32_t)tiles.size(), SDBFX_NT, d_base,
+                   (const TileDesc*)ctx->tiles.p, (uint32_t*)ctx->cand.p, (TileState*)ctx->tstate.p);
             LAUNCH(ctx, stream_fixup_kernel, (uint32_t)units.size(), SDBFX_NT, d_base,
                    (const TileDesc*)ctx->tiles.p, (const UnitDesc*)ctx->units.p, (uint32_t*)ctx->cand.p,
                    (TileState*)ctx->tstate.p);
+        }
         LAUNCH(ctx, stream_insert_kernel, (uint32_t)sfiles.size(), 64, (const StreamFile*)ctx->files.p,
                (const TileDesc*)ctx->tiles.p, (const TileState*)ctx->tstate.p,
                (const uint32_t*)ctx->cand.p, (uint8_t*)ctx->sfilt.p, (uint16_t*)ctx->sham.p,
@@ -358,6 +368,36 @@ int digest_pass(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t i0, uint32_t i1, const 
     return 0;
 }
 
+constexpr uint64_t kPinBytes = 8ull << 20;   // per pinned staging buffer
+constexpr uint64_t kPinMin = 64ull << 10;    // smaller copies go directly
+
+// Copies caller (pageable) memory to the device through the two pinned
+// buffers: one is filled on the host while the other's copy is in flight.
+int upload_host(sdbfx_ctx* ctx, uint8_t* dst, const uint8_t* src, uint64_t len)
+{
+    if (len < kPinMin) {
+        HIPC(ctx, hipMemcpyAsync(dst, src, len, hipMemcpyHostToDevice, ctx->stream));
+        return 0;
+    }
+    if (!ctx->pin[0])
+        for (int k = 0; k < 2; k++) {
+            HIPC(ctx, hipHostMalloc(&ctx->pin[k], kPinBytes, hipHostMallocDefault));
+            HIPC(ctx, hipEventCreate(&ctx->pin_ev[k]));
+        }
+    for (uint64_t o = 0; o < len; o += kPinBytes) {
+        const int k = ctx->pin_next;
+        ctx->pin_next ^= 1;
+        if (ctx->pin_busy[k])
+            HIPC(ctx, hipEventSynchronize(ctx->pin_ev[k]));
+        const uint64_t c = std::min(kPinBytes, len - o);
+        memcpy(ctx->pin[k], src + o, c);
+        HIPC(ctx, hipMemcpyAsync(dst + o, ctx->pin[k], c, hipMemcpyHostToDevice, ctx->stream));
+        HIPC(ctx, hipEventRecord(ctx->pin_ev[k], ctx->stream));
+        ctx->pin_busy[k] = true;
+    }
+    return 0;
+}
+
 int digest_common(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t n, const uint8_t* d_base, const uint64_t* off,
                   const uint64_t* lens, const char* const* names, const uint32_t* dd, int32_t* status,
                   const uint8_t* const* host_bufs)
@@ -397,8 +437,8 @@ int digest_common(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t n, const uint8_t* d_b
             HIPC(ctx, hipMemsetAsync(ctx->stage.p, 0, SDBFX_PAD, ctx->stream));
             for (uint32_t i = i0; i < i1; i++)
                 if (lens[i])
-                    HIPC(ctx, hipMemcpyAsync((uint8_t*)ctx->stage.p + hoff[i], host_bufs[i], lens[i],
-                                             hipMemcpyHostToDevice, ctx->stream));
+                    if (int rc = upload_host(ctx, (uint8_t*)ctx->stage.p + hoff[i], host_bufs[i], lens[i]))
+                        return rc;
             base = (const uint8_t*)ctx->stage.p;
             o = hoff.data();
         }
@@ -630,6 +670,12 @@ void sdbfx_ctx_destroy(sdbfx_ctx* ctx)
     for (DevBuf* b : bufs)
         if (b->p)
             (void)hipFree(b->p);
+    for (int k = 0; k < 2; k++) {
+        if (ctx->pin[k])
+            (void)hipHostFree(ctx->pin[k]);
+        if (ctx->pin_ev[k])
+            (void)hipEventDestroy(ctx->pin_ev[k]);
+    }
     for (auto& ev : ctx->pending) {
         (void)hipEventDestroy(ev.a);
         (void)hipEventDestroy(ev.b);

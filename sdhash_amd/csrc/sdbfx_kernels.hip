@@ -998,34 +998,73 @@ __global__ __launch_bounds__(SDBFX_NT, 2) void stream_tile_kernel(
     stream_tile(sm, data, td, blockIdx.x, -1, cand, tstate);
 }
 
-// Re-runs, in order, every tile of a unit whose incoming state was not proven,
-// from the exact hand-off state of the tile before it.
+// A tile whose incoming state was not proven is re-run from the exact
+// hand-off state of the tile before it (stream_tile with that state marks it
+// certified).  Three passes: stream_fixup_plan_kernel marks the tiles whose
+// predecessor is certified; stream_fixup_par_kernel re-runs those, all at
+// once (their predecessors are not re-run, so their hand-offs do not change
+// meanwhile); stream_fixup_kernel then re-runs what is left -- tiles whose
+// predecessor was also unproven -- in order, one workgroup per unit.
+__global__ void stream_fixup_plan_kernel(const TileDesc* __restrict__ tiles, uint32_t ntiles,
+                                         TileState* __restrict__ tstate)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles)
+        return;
+    tstate[t].pad = (!tiles[t].first && !tstate[t].certified && tstate[t - 1].certified) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(SDBFX_NT) void stream_fixup_par_kernel(
+    const uint8_t* __restrict__ data, const TileDesc* __restrict__ tiles, uint32_t* __restrict__ cand,
+    TileState* __restrict__ tstate)
+{
+    __shared__ TileSmem sm;
+    const uint32_t t = blockIdx.x;
+    if (!tstate[t].pad)
+        return;
+    const uint32_t h = tstate[t - 1].handoff;
+    stream_tile(sm, data, tiles[t], t, (int64_t)h, cand, tstate);
+}
+
 __global__ __launch_bounds__(SDBFX_NT) void stream_fixup_kernel(
     const uint8_t* __restrict__ data, const TileDesc* __restrict__ tiles, const UnitDesc* __restrict__ units,
     uint32_t* __restrict__ cand, TileState* __restrict__ tstate)
 {
     __shared__ TileSmem sm;
-    __shared__ uint32_t flag[2];
+    __shared__ uint32_t list[SDBFX_NT];   // unproven tiles of a run of SDBFX_NT, in order
+    __shared__ uint32_t wcount[NWAVE];
+    __shared__ uint32_t hand;
     const UnitDesc ud = units[blockIdx.x];
-    for (uint32_t t = ud.tile_begin; t < ud.tile_end; t++) {
-        // one thread reads the flags; every thread takes the same branch.
-        // The first tile of a unit starts at step 0 and is always exact, so
-        // a tile that needs a hand-off always has a predecessor in the unit.
-        if (threadIdx.x == 0) {
-            const uint32_t c = ((volatile TileState*)tstate)[t].certified;
-            const bool redo = !c && t > ud.tile_begin;
-            flag[0] = redo ? 0u : 1u;
-            flag[1] = redo ? ((volatile TileState*)tstate)[t - 1].handoff : 0u;
+    const int wave = threadIdx.x >> 6;
+    // the first tile of a unit starts at step 0 and is always exact
+    for (uint32_t b0 = ud.tile_begin + 1; b0 < ud.tile_end; b0 += SDBFX_NT) {
+        const uint32_t t = b0 + threadIdx.x;
+        const bool unc = t < ud.tile_end && !((volatile TileState*)tstate)[t].certified;
+        const unsigned long long m = __ballot(unc);
+        if ((threadIdx.x & 63) == 0)
+            wcount[wave] = (uint32_t)__popc((uint32_t)m) + (uint32_t)__popc((uint32_t)(m >> 32));
+        __syncthreads();
+        uint32_t pos = 0, total = 0;
+        for (int w = 0; w < NWAVE; w++) {
+            pos += w < wave ? wcount[w] : 0u;
+            total += wcount[w];
         }
+        if (unc)
+            list[pos + lanes_below(m)] = t;
         __syncthreads();
-        const uint32_t cert = flag[0], h = flag[1];
-        __syncthreads();
-        if (cert)
-            continue;
-        const TileDesc td = tiles[t];
-        stream_tile(sm, data, td, t, (int64_t)h, cand, tstate);
-        __threadfence();
-        __syncthreads();
+        for (uint32_t k = 0; k < total; k++) {
+            const uint32_t tt = list[k];
+            // the predecessor is exact: certified from the start, re-run by
+            // the parallel pass, or re-run just before in this loop
+            if (threadIdx.x == 0)
+                hand = ((volatile TileState*)tstate)[tt - 1].handoff;
+            __syncthreads();
+            const uint32_t h = hand;
+            __syncthreads();
+            stream_tile(sm, data, tiles[tt], tt, (int64_t)h, cand, tstate);
+            __threadfence();
+            __syncthreads();
+        }
     }
 }
 

@@ -13,9 +13,10 @@
 // Autotuning: the two comparison kernels give identical scores at different
 // speeds depending on the shape of the batch (query filters per reference
 // digest) and on how many filter pairs come close to their cutoff.  The first
-// batch of each size class (log2 of its filter count, up to 2048 filters) is
-// compared with both, and the faster one is used for that class from then
-// on; a larger class takes the choice of the largest tuned class below it.
+// batch of each size class (log2 of its filter count) is compared with both
+// against a sample of the reference set (its first whole digests, at least
+// 65,536 filters, copied once when the server starts), and the faster one is
+// used for that class from then on.
 //
 // The reference counterpart is sdhash-server's handler (sdhash-srv-main.cc:
 // 224-402), which hashes and compares each request on its own.
@@ -37,7 +38,7 @@
 namespace {
 
 // batches of at most this many query filters are timed with both kernels
-constexpr uint64_t SERVE_TUNE_MAX_FILTERS = 2048;
+constexpr uint64_t SERVE_TUNE_SAMPLE_FILTERS = 65536;
 
 uint64_t now_ns()
 {
@@ -68,6 +69,8 @@ struct Batch {
 struct sdbfx_server {
     const sdbfx_set* ref = nullptr;
     uint32_t nref = 0;
+    const sdbfx_set* sample = nullptr;   // the reference set's first digests, for autotuning
+    sdbfx_set* own_sample = nullptr;     // owned copy when the sample is not the whole set
     uint64_t batch_bytes = 0;
     uint32_t batch_max = 0;
     sdbfx_ctx* cd = nullptr;   // digest context
@@ -141,12 +144,12 @@ struct sdbfx_server {
         }
     }
 
-    int timed_compare(const sdbfx_set* q, int method, int32_t* out, double* seconds)
+    int timed_compare(const sdbfx_set* q, const sdbfx_set* t, int method, int32_t* out, double* seconds)
     {
         const auto t0 = std::chrono::steady_clock::now();
         int rc = sdbfx_set_compare_method(cc, method);
         if (rc == 0)
-            rc = sdbfx_compare_to(cc, q, ref, out);
+            rc = sdbfx_compare_to(cc, q, t, out);
         *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         return rc;
     }
@@ -172,31 +175,23 @@ struct sdbfx_server {
                 const uint64_t nf = sdbfx_set_filter_count(b.set);
                 const int cls = nf ? 63 - __builtin_clzll(nf) : 0;
                 method = tuned[cls];
-                if (method == 0 && nf > SERVE_TUNE_MAX_FILTERS) {
-                    // too large to time twice: the choice of the largest
-                    // tuned class below, else the engine's own estimate
-                    method = SDBFX_CMP_AUTO;
-                    for (int c = cls - 1; c >= 0; c--)
-                        if (tuned[c]) {
-                            method = tuned[c];
-                            break;
-                        }
-                    double t;
-                    rc = timed_compare(b.set, method, sc.data(), &t);
-                } else if (method == 0) {
-                    // time both kernels on this batch; keep the faster
-                    std::vector<int32_t> sv(sc.size());
+                if (method == 0) {
+                    // time both kernels on this batch against the sample;
+                    // keep the faster
+                    const size_t ns = (size_t)nd * sdbfx_set_size(sample);
+                    std::vector<int32_t> sv(ns), sm(ns);
                     double tv = 0, tm = 0;
-                    rc = timed_compare(b.set, SDBFX_CMP_VALU, sv.data(), &tv);
+                    rc = timed_compare(b.set, sample, SDBFX_CMP_VALU, sv.data(), &tv);
                     if (rc == 0)
-                        rc = timed_compare(b.set, SDBFX_CMP_MFMA, sc.data(), &tm);
-                    if (rc == 0 && sv != sc)
+                        rc = timed_compare(b.set, sample, SDBFX_CMP_MFMA, sm.data(), &tm);
+                    if (rc == 0 && sv != sm)
                         set_error("compare: the two kernels disagree");
                     method = tm <= tv ? SDBFX_CMP_MFMA : SDBFX_CMP_VALU;
                     tuned[cls] = (int8_t)method;
-                } else {
+                }
+                if (rc == 0) {
                     double t;
-                    rc = timed_compare(b.set, method, sc.data(), &t);
+                    rc = timed_compare(b.set, ref, method, sc.data(), &t);
                 }
             }
             if (rc != 0)
@@ -229,6 +224,46 @@ struct sdbfx_server {
 
 extern "C" {
 
+namespace {
+
+// The autotuning sample: the reference set's first whole digests up to
+// SERVE_TUNE_SAMPLE_FILTERS filters, copied into a set of the compare
+// context (or the reference set itself when it is not much larger).
+int make_sample(sdbfx_server* s)
+{
+    uint32_t k = 0;
+    uint64_t nf = 0;
+    sdbfx_digest_info info;
+    while (k < s->nref && nf < SERVE_TUNE_SAMPLE_FILTERS) {
+        if (sdbfx_set_info(s->ref, k, &info) != 0)
+            return -1;
+        nf += info.bf_count;
+        k++;
+    }
+    if (k == s->nref || sdbfx_set_filter_count(s->ref) <= 2 * nf) {
+        s->sample = s->ref;
+        return 0;
+    }
+    if (sdbfx_set_create(s->cc, &s->own_sample) != 0)
+        return -1;
+    std::vector<uint8_t> filt;
+    std::vector<uint16_t> ec;
+    for (uint32_t i = 0; i < k; i++) {
+        if (sdbfx_set_info(s->ref, i, &info) != 0)
+            return -1;
+        filt.resize((size_t)info.bf_count * 256);
+        ec.resize(info.bf_count);
+        if (sdbfx_set_read(s->cc, s->ref, i, filt.data(), ec.data(), nullptr) != 0 ||
+            sdbfx_set_add(s->cc, s->own_sample, info.name, info.orig_size, info.bf_count, info.last_count,
+                          info.dd_block_size, info.max_elem, filt.data(), ec.data()) != 0)
+            return -1;
+    }
+    s->sample = s->own_sample;
+    return 0;
+}
+
+}  // namespace
+
 int sdbfx_serve_create(int device, const sdbfx_set* ref, uint64_t batch_bytes, uint32_t batch_max,
                        sdbfx_server** out)
 {
@@ -244,6 +279,14 @@ int sdbfx_serve_create(int device, const sdbfx_set* ref, uint64_t batch_bytes, u
     if (sdbfx_ctx_create(device, 0, &s->cd) != 0 || sdbfx_ctx_create(device, 0, &s->cc) != 0) {
         if (s->cd)
             sdbfx_ctx_destroy(s->cd);
+        delete s;
+        return -1;
+    }
+    if (make_sample(s) != 0) {
+        if (s->own_sample)
+            sdbfx_set_destroy(s->own_sample);
+        sdbfx_ctx_destroy(s->cd);
+        sdbfx_ctx_destroy(s->cc);
         delete s;
         return -1;
     }
@@ -268,6 +311,8 @@ void sdbfx_serve_destroy(sdbfx_server* s)
     for (auto& b : s->handoff)
         if (b.set)
             sdbfx_set_destroy(b.set);
+    if (s->own_sample)
+        sdbfx_set_destroy(s->own_sample);
     sdbfx_ctx_destroy(s->cd);
     sdbfx_ctx_destroy(s->cc);
     delete s;
