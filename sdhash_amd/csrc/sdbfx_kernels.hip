@@ -1526,13 +1526,24 @@ __device__ __forceinline__ uint32_t size_code(uint32_t s)
 __device__ __forceinline__ void prefix_counts(const uint4* ra, uint32_t sw, const uint4* __restrict__ fb,
                                               uint32_t& p32, uint32_t& p64, uint32_t& p128, uint32_t& p256)
 {
+    // chunks 0-1, 2-3, 4-7, 8-15: four 16-byte chunks of each side in
+    // registers at a time
     uint32_t c[4] = {0, 0, 0, 0};
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-        const uint4 x = ra[q ^ sw];
-        const uint4 y = fb[q];
-        c[q < 2 ? 0 : q < 4 ? 1 : q < 8 ? 2 : 3] +=
-            __popc(x.x & y.x) + __popc(x.y & y.y) + __popc(x.z & y.z) + __popc(x.w & y.w);
+    for (int g = 0; g < 4; g++) {
+        uint4 x[4], y[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            x[k] = ra[(4 * g + k) ^ sw];
+            y[k] = fb[4 * g + k];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int q = 4 * g + k;
+            c[q < 2 ? 0 : q < 4 ? 1 : q < 8 ? 2 : 3] +=
+                __popc(x[k].x & y[k].x) + __popc(x[k].y & y[k].y) + __popc(x[k].z & y[k].z) + __popc(x[k].w & y[k].w);
+        }
+        __builtin_amdgcn_sched_barrier(0);
     }
     p32 = c[0];
     p64 = p32 + c[1];
@@ -1605,10 +1616,8 @@ __global__ void inst_map_kernel(SetView A, SetView B, const CmpPair* __restrict_
 // pair's fraction of the same instance and target digest, so taking it into
 // this chunk's partial maximum changes no result).
 //
-// Latency.  The last K step of a tile loads the first words of the next tile
-// of the work item, and the next tile's column data is loaded before the K
-// loop, so that consecutive tiles follow each other without waiting for
-// memory.
+// Latency.  The next tile's column data (element counts, Hamming weights) is
+// loaded before a tile's K loop and used after it.
 __global__ __launch_bounds__(256) void cmp_mfma_kernel(
     SetView A, SetView B, const CmpGroup* __restrict__ groups, uint32_t ngroups,
     const uint64_t* __restrict__ rmap, const uint32_t* __restrict__ rcode, uint64_t nwork,
@@ -1696,10 +1705,9 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
         const uint4* __restrict__ tbase = (const uint4*)(tf + j0 * SDBFX_BFW);
         bool anyq = false;
 
-        // column data of a tile (element count, Hamming weight) and the
-        // uint4 index of each lane's first chunk (columns past the end
-        // repeat the last one)
-        auto col_load = [&](uint32_t t0, uint32_t* s2, uint32_t* e2, uint32_t* cb) {
+        // column data of a tile: element count and Hamming weight (0 past
+        // the end)
+        auto col_load = [&](uint32_t t0, uint32_t* s2, uint32_t* e2) {
             const uint32_t ncol = min(128u, nj - t0);
 #pragma unroll
             for (int bn = 0; bn < MF_B; bn++) {
@@ -1707,14 +1715,10 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
                 const bool ok = col < ncol;
                 s2[bn] = ok ? tec[j0 + t0 + col] : 0u;
                 e2[bn] = ok ? tham[j0 + t0 + col] : 0u;
-                cb[bn] = (t0 + (ok ? col : ncol - 1)) * 16 + h;
             }
         };
-        uint32_t s2[MF_B], e2[MF_B], cb[MF_B];
-        col_load(0, s2, e2, cb);
-        // words 8q + 4h .. 8q + 4h + 3 of a row or column are its chunk 2q + h
-        uint4 aw[MF_B], bw[MF_B];
-        bool pre = false;       // aw, bw already hold this tile's first words
+        uint32_t s2[MF_B], e2[MF_B];
+        col_load(0, s2, e2);
 
         for (uint32_t t0 = 0; t0 < nj; t0 += 32 * MF_B) {
             const uint32_t tn0 = t0 + 32 * MF_B < nj ? t0 + 32 * MF_B : t0;   // the next tile (or this one)
@@ -1728,30 +1732,31 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
                 cmn = min(cmn, ccode[bn]);
                 cmx = max(cmx, ccode[bn]);
             }
+            // each lane's columns as uint4 indices of their first chunk
+            // (columns past the end repeat the last one)
             uint32_t e2c[MF_B], cbt[MF_B];
+            const uint32_t ncol = min(128u, nj - t0);
 #pragma unroll
             for (int bn = 0; bn < MF_B; bn++) {
                 e2c[bn] = e2[bn] + 1;
-                cbt[bn] = cb[bn];
+                cbt[bn] = (t0 + min(32u * bn + r32, ncol - 1)) * 16 + h;
             }
             // the next tile's column data, consumed after this tile
-            col_load(tn0, s2, e2, cb);
+            col_load(tn0, s2, e2);
             // no qualifying target filter: no pair of the tile can pass
-            if (__ballot(sx != 0) == 0) {
-                pre = false;
+            if (__ballot(sx != 0) == 0)
                 continue;
-            }
             // all columns of one size code: the cutoff depends on the row only
             const uint32_t umin = (uint32_t)__builtin_amdgcn_readlane((int)wave_prefix_min(cmn, lane), 63);
             const uint32_t umax =
                 0xFFFFu - (uint32_t)__builtin_amdgcn_readlane((int)wave_prefix_min(0xFFFFu - cmx, lane), 63);
             const bool uniform = umin == umax;
-            if (!pre) {
+            // words 8q + 4h .. 8q + 4h + 3 of a row or column are its chunk 2q + h
+            uint4 aw[MF_B], bw[MF_B];
 #pragma unroll
-                for (int k = 0; k < MF_B; k++) {
-                    aw[k] = atile[(32 * k + r32) * 16 + (h ^ (r32 & 15))];
-                    bw[k] = tbase[cbt[k]];
-                }
+            for (int k = 0; k < MF_B; k++) {
+                aw[k] = atile[(32 * k + r32) * 16 + (h ^ (r32 & 15))];
+                bw[k] = tbase[cbt[k]];
             }
 
             // the smallest qualifying Hamming weight (+1) of the tile
@@ -1837,16 +1842,15 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
             }
 #pragma unroll 1
             for (int q = 0; q < 8; q++) {
-                // prefetch of the next 8 words, consumed one iteration later;
-                // the last iteration fetches the next tile's first words
-                const int qn = q < 7 ? q + 1 : 0;
+                // prefetch of the next 8 words, consumed one iteration later
+                const int qn = q < 7 ? q + 1 : 7;
                 uint32_t ao = (uint32_t)(r32 * 16 + ((2 * qn + h) ^ (r32 & 15)));
                 opaque(ao);
                 uint4 an[MF_B], bnx[MF_B];
 #pragma unroll
                 for (int k = 0; k < MF_B; k++) {
                     an[k] = atile[ao + 32 * 16 * k];
-                    bnx[k] = tbase[q < 7 ? cbt[k] + 2 * qn : cb[k]];
+                    bnx[k] = tbase[cbt[k] + 2 * qn];
                 }
                 __builtin_amdgcn_sched_barrier(0);
                 // step t: word t / 2, planes 4 (t % 2) .. 4 (t % 2) + 3.  Each
@@ -1885,12 +1889,13 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
                     bw[k] = bnx[k];
                 }
             }
-            pre = true;
 
             // candidates: accumulators >= 0, one bit per accumulator, from
             // the sign bits (shifted in by v_alignbit).  Word w holds blocks
             // 2w (bits 31..16, register i at bit 31 - i) and 2w + 1 (bits
             // 15..0, register i at bit 15 - i).
+            // One block of 16 accumulators at a time (the scheduling
+            // barriers keep the compiler from reading all 256 at once).
             uint32_t cbits[MF_B * MF_B / 2];
             uint32_t anyc = 0;
 #pragma unroll
@@ -1902,6 +1907,7 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
 #pragma unroll
                     for (int i = 0; i < 16; i++)
                         x = __builtin_amdgcn_alignbit(x, (uint32_t)acc[bm][bn][i], 31);
+                    __builtin_amdgcn_sched_barrier(0);
                 }
                 cbits[wi] = ~x;
                 anyc |= ~x;
