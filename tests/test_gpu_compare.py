@@ -141,3 +141,35 @@ def test_compare_repeated_filter(cmp_engine):
     want = oracle.compare_all(reps)
     assert np.array_equal(cmp_engine.compare_all(s), want)
     assert np.all(want == 100)
+
+
+def test_compare_pruned_multi_chunk(cmp_engine):
+    # Enough row blocks and target filters that the MFMA kernel takes
+    # 2048-filter chunks (16 tiles per work item, running maxima shared
+    # across the 10 chunks of each row block).  One target digest has one
+    # element count everywhere (uniform tiles), the other counts of 8-192
+    # (mixed tiles, sparse filters, and low cutoffs that make most pairs
+    # candidates).  Query filters are thinned copies of target filters plus
+    # noise, so the best fraction of a row varies widely.
+    rng = np.random.Generator(np.random.PCG64(41))
+    nt, nq, per = 20000, 30, 300
+
+    def bits(n, tenths):
+        return rng.integers(0, 10, (n, 2048), dtype=np.uint8) < tenths
+
+    tb = [bits(nt, 3), bits(nt, 3)]
+    tec = [np.full(nt, 192, np.uint16), rng.integers(8, 193, nt).astype(np.uint16)]
+    t = [oracle.Digest(b"t%d" % k, np.packbits(tb[k], axis=1, bitorder="little").reshape(-1), nt, 0, tec[k],
+                       nt * 16384, 16384) for k in range(2)]
+    q = []
+    for d in range(nq):
+        src, which = rng.integers(0, nt, per), rng.integers(0, 2, per)
+        b = np.where(which[:, None] == 0, tb[0][src], tb[1][src])
+        keep = rng.uniform(0.2, 1.0, per)[:, None]
+        b = (b & (rng.random((per, 2048), dtype=np.float32) < keep)) | bits(per, 0.5)
+        q.append(oracle.Digest(b"q%02d" % d, np.packbits(b, axis=1, bitorder="little").reshape(-1), per,
+                               100 + d, None, per * 12000, 0))
+    want = oracle.compare_to(q, t, threads=oracle.threads_available())
+    got = cmp_engine.compare_to(upload_oracle(cmp_engine, q), upload_oracle(cmp_engine, t))
+    assert np.array_equal(got, want), np.argwhere(got != want)[:10]
+    assert len(set(want.reshape(-1).tolist())) > 10
