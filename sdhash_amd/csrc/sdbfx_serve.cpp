@@ -16,7 +16,8 @@
 // batch of each size class (log2 of its filter count) is compared with both
 // against a sample of the reference set (its first whole digests, at least
 // 65,536 filters, copied once when the server starts), and the faster one is
-// used for that class from then on.
+// used for that class from then on.  When the engine's cost model puts the
+// MFMA kernel under half the VALU kernel's cost, the trial is skipped.
 //
 // The reference counterpart is sdhash-server's handler (sdhash-srv-main.cc:
 // 224-402), which hashes and compares each request on its own.
@@ -24,6 +25,7 @@
 #include <string.h>
 
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <deque>
 #include <memory>
@@ -144,6 +146,27 @@ struct sdbfx_server {
         }
     }
 
+    // True when the MFMA kernel's padded work on (batch x sample) is under
+    // half the VALU kernel's, by the engine's cost model (sdbfx_host.cpp,
+    // use_mfma: 2.75 vs 8 SIMD cycles per padded pair slot; query filters
+    // padded to 128 and 64 rows, target filters to 128 and 1); then the
+    // trial is skipped.  The batch's digests all face every sample digest,
+    // so the instances per target digest are the batch's filter count.
+    bool clear_mfma(const sdbfx_set* q) const
+    {
+        const double nq = (double)sdbfx_set_filter_count(q);
+        double valu = 0, mfma = 0;
+        sdbfx_digest_info info;
+        for (uint32_t i = 0; i < sdbfx_set_size(sample); i++) {
+            if (sdbfx_set_info(sample, i, &info) != 0)
+                return false;
+            const double t = info.bf_count;
+            valu += 8.0 * std::ceil(nq / 64) * 64 * t;
+            mfma += 2.75 * std::ceil(nq / 128) * 128 * std::ceil(t / 128) * 128;
+        }
+        return mfma < 0.5 * valu;
+    }
+
     int timed_compare(const sdbfx_set* q, const sdbfx_set* t, int method, int32_t* out, double* seconds)
     {
         const auto t0 = std::chrono::steady_clock::now();
@@ -175,6 +198,10 @@ struct sdbfx_server {
                 const uint64_t nf = sdbfx_set_filter_count(b.set);
                 const int cls = nf ? 63 - __builtin_clzll(nf) : 0;
                 method = tuned[cls];
+                if (method == 0 && clear_mfma(b.set)) {
+                    method = SDBFX_CMP_MFMA;
+                    tuned[cls] = (int8_t)method;
+                }
                 if (method == 0) {
                     // time both kernels on this batch against the sample;
                     // keep the faster
