@@ -128,6 +128,7 @@ def serving(eng, args) -> dict:
     eng.sync()
     t_ref = time.perf_counter() - t
     assert np.all(st == 0)
+    text = text_form(eng, ref)
     rng = np.random.Generator(np.random.PCG64(12))
     nq = args.serve_queries
     sizes = np.minimum((2.0 ** rng.uniform(12, 24, nq)).astype(np.int64), 16 * MIB - 16)
@@ -214,7 +215,35 @@ def serving(eng, args) -> dict:
         "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
         "identical_results": True,
         "data": "device-generated segment mix (sdbfx_synth_fill); reference -b 10 over 1 GiB inputs",
+        "text_form": text,
     }
+
+
+def text_form(eng, ref) -> dict:
+    """The sdbf text of the whole reference set (sdbf_set::to_string), base64
+    written by the device, against the oracle's encoder on one of its
+    digests on the host; the first digest's text must agree."""
+    import oracle
+    eng.sync()
+    eng.profiling(True)
+    eng.reset_kernel_times()
+    t = time.perf_counter()
+    txt = ref.to_text()
+    wall = time.perf_counter() - t
+    eng.profiling(False)
+    kms = sum(eng.kernel_time(k)[0] for k in ("text_header_kernel", "text_b64_kernel"))
+    inf = ref.info(0)
+    f, ec, _ = ref.read(0)
+    od = oracle.Digest(inf.name.encode("utf-8", "surrogateescape"), f, inf.bf_count, inf.last_count,
+                       ec if inf.dd_block_size else None, inf.orig_size, inf.dd_block_size)
+    t = time.perf_counter()
+    one = od.to_string().encode()
+    cpu = time.perf_counter() - t
+    if txt[:len(one)] != one:
+        raise RuntimeError("device text differs from the oracle's")
+    return {"filters": ref.filter_count(), "text_bytes": len(txt), "kernel_ms": kms,
+            "kernel_GBps": len(txt) / (kms / 1e3) / 1e9, "wall_s": wall,
+            "cpu_oracle_MBps": len(one) / cpu / 1e6, "cpu_sample": "one digest, %d filters, 1 thread" % inf.bf_count}
 
 
 def batched_digest(eng, args) -> dict:

@@ -103,6 +103,15 @@ int sdbfx_set_info(const sdbfx_set *set, uint32_t idx, sdbfx_digest_info *info);
 int sdbfx_set_read(sdbfx_ctx *ctx, const sdbfx_set *set, uint32_t idx, uint8_t *filters,
                    uint16_t *elem_counts, uint16_t *hamming);
 
+/*
+ * The text form of every digest of the set, in set order, exactly as
+ * sdbf::to_string writes each one (sdbf/sdbf_class.cc:294-334) and
+ * sdbf_set::to_string concatenates them (sdbf/sdbf_set.cc:175-184); the base64 is
+ * encoded on the device.  *len receives the length in bytes (no terminating
+ * NUL).  out == NULL only sets *len; a cap below it is an error.
+ */
+int sdbfx_set_to_text(sdbfx_ctx *ctx, const sdbfx_set *set, char *out, uint64_t cap, uint64_t *len);
+
 /* ---- comparison ----------------------------------------------------- */
 
 /*

@@ -19,7 +19,7 @@ SYMBOLS = (
     "sdbfx_ctx_create", "sdbfx_ctx_destroy", "sdbfx_last_error", "sdbfx_sync",
     "sdbfx_set_create", "sdbfx_set_destroy", "sdbfx_set_size", "sdbfx_set_filter_count",
     "sdbfx_set_digest", "sdbfx_set_digest_device", "sdbfx_set_add", "sdbfx_set_info",
-    "sdbfx_set_read", "sdbfx_compare_all", "sdbfx_compare_to", "sdbfx_device_alloc",
+    "sdbfx_set_read", "sdbfx_set_to_text", "sdbfx_compare_all", "sdbfx_compare_to", "sdbfx_device_alloc",
     "sdbfx_device_free", "sdbfx_upload", "sdbfx_set_profiling", "sdbfx_kernel_time",
     "sdbfx_reset_kernel_times", "sdbfx_set_compare_method",
     "sdbfx_serve_create", "sdbfx_serve_destroy", "sdbfx_serve_error", "sdbfx_serve_ref_size",
@@ -77,6 +77,7 @@ def load():
     L.sdbfx_kernel_time.argtypes = [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(U64)]
     L.sdbfx_reset_kernel_times.argtypes = [P]
     L.sdbfx_set_compare_method.argtypes = [P, C.c_int]
+    L.sdbfx_set_to_text.argtypes = [P, P, P, U64, C.POINTER(U64)]
     L.sdbfx_serve_create.argtypes = [C.c_int, P, U64, U32, C.POINTER(P)]
     L.sdbfx_serve_destroy.argtypes = [P]
     L.sdbfx_serve_destroy.restype = None
@@ -92,6 +93,7 @@ def load():
         getattr(L, name).restype = getattr(L, name).restype or I32
     for name in ("sdbfx_ctx_create", "sdbfx_sync", "sdbfx_set_create", "sdbfx_set_digest",
                  "sdbfx_set_digest_device", "sdbfx_set_add", "sdbfx_set_info", "sdbfx_set_read",
+                 "sdbfx_set_to_text",
                  "sdbfx_compare_all", "sdbfx_compare_to", "sdbfx_device_alloc", "sdbfx_device_free",
                  "sdbfx_upload", "sdbfx_set_profiling", "sdbfx_kernel_time",
                  "sdbfx_reset_kernel_times", "sdbfx_set_compare_method", "sdbfx_serve_create",
@@ -175,6 +177,16 @@ class DigestSet:
         self.engine._check(self.engine.lib.sdbfx_set_read(self.engine.ctx, self.handle, i, _ptr(f),
                                                           _ptr(ec), _ptr(hm)))
         return f, ec, hm
+
+    def to_text(self) -> bytes:
+        """The sdbf text of every digest, in order (sdbf_set::to_string), with
+        the base64 encoded on the device."""
+        n = C.c_uint64(0)
+        lib, ctx = self.engine.lib, self.engine.ctx
+        self.engine._check(lib.sdbfx_set_to_text(ctx, self.handle, None, 0, C.byref(n)))
+        out = np.empty(max(n.value, 1), np.uint8)
+        self.engine._check(lib.sdbfx_set_to_text(ctx, self.handle, _ptr(out), n.value, C.byref(n)))
+        return out[: n.value].tobytes()
 
     def digest(self, bufs, names=None, dd_block=0) -> np.ndarray:
         """Digests host buffers into this set; returns the per-input status."""

@@ -49,7 +49,7 @@ struct sdbfx_ctx {
     std::map<std::string, std::pair<double, uint64_t>> times;
     std::vector<hipEvent_t> evpool;
     DevBuf stage, blocks, tiles, tstate, units, files, cand, sfilt, sham, counts, gsrc, gdst, gcnt,
-        glast, groups, pairs, partial, imax, scores, gbest, rmap, rcode;
+        glast, groups, pairs, partial, imax, scores, gbest, rmap, rcode, text, tdesc, thdr;
     // two pinned host buffers for staging caller memory to the device
     void* pin[2] = {nullptr, nullptr};
     hipEvent_t pin_ev[2] = {nullptr, nullptr};
@@ -666,7 +666,7 @@ void sdbfx_ctx_destroy(sdbfx_ctx* ctx)
     DevBuf* bufs[] = {&ctx->stage, &ctx->blocks, &ctx->tiles, &ctx->tstate, &ctx->units, &ctx->files,
                       &ctx->cand, &ctx->sfilt, &ctx->sham, &ctx->counts, &ctx->gsrc, &ctx->gdst,
                       &ctx->gcnt, &ctx->glast, &ctx->groups, &ctx->pairs, &ctx->partial, &ctx->imax, &ctx->scores,
-                      &ctx->gbest, &ctx->rmap, &ctx->rcode};
+                      &ctx->gbest, &ctx->rmap, &ctx->rcode, &ctx->text, &ctx->tdesc, &ctx->thdr};
     for (DevBuf* b : bufs)
         if (b->p)
             (void)hipFree(b->p);
@@ -825,6 +825,67 @@ int sdbfx_set_read(sdbfx_ctx* ctx, const sdbfx_set* set, uint32_t idx, uint8_t* 
     if (hamming)
         HIPC(ctx, hipMemcpyAsync(hamming, (const uint16_t*)set->hamf.p + f, c * 2, hipMemcpyDeviceToHost,
                                  ctx->stream));
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int sdbfx_set_to_text(sdbfx_ctx* ctx, const sdbfx_set* set, char* out, uint64_t cap, uint64_t* len)
+{
+    if (!ctx || !set || !len)
+        return fail(ctx, "null argument");
+    HIPC(ctx, hipSetDevice(ctx->device));
+    // headers and layout (sdbf/sdbf_class.cc:294-334)
+    const uint32_t n = (uint32_t)set->names.size();
+    std::string hdr;
+    std::vector<TextDigest> td(n);
+    uint64_t pos = 0, q = 0;
+    for (uint32_t d = 0; d < n; d++) {
+        char buf[96];
+        std::string h = set->dd[d] ? "sdbf-dd:03:" : "sdbf:03:";
+        h += std::to_string(set->names[d].size()) + ":" + set->names[d] + ":";
+        if (set->dd[d])
+            snprintf(buf, sizeof buf, "%llu:sha1:256:5:7ff:%u:%u:%u", (unsigned long long)set->orig[d],
+                     set->maxe[d], set->bfc[d], set->dd[d]);
+        else
+            snprintf(buf, sizeof buf, "%llu:sha1:256:5:7ff:%u:%u:%u:", (unsigned long long)set->orig[d],
+                     set->maxe[d], set->bfc[d], set->last[d]);
+        h += buf;
+        if (hdr.size() + h.size() > 0xFFFFFFFFull)
+            return fail(ctx, "text headers too long");
+        TextDigest& t = td[d];
+        t.filt_first = set->first[d];
+        t.q_begin = q;
+        t.text_off = pos;
+        t.body_off = pos + h.size();
+        t.hdr_off = (uint32_t)hdr.size();
+        t.hdr_len = (uint32_t)h.size();
+        t.bf_count = set->bfc[d];
+        t.dd = set->dd[d];
+        const uint64_t raw = (uint64_t)t.bf_count * SDBFX_BF;
+        const uint64_t quanta = t.dd ? (uint64_t)t.bf_count * 87 : (raw + 2) / 3;
+        const uint64_t body = t.dd ? (uint64_t)t.bf_count * 348 : (raw + 2) / 3 * 4;
+        q += quanta;
+        pos = t.body_off + body + 1;   // + newline
+        hdr += h;
+    }
+    *len = pos;
+    if (!out)
+        return 0;
+    if (cap < pos)
+        return fail(ctx, "text buffer too small");
+    if (n == 0)
+        return 0;
+    int rc;
+    if ((rc = ensure(ctx, ctx->text, pos + 256)) || (rc = upload(ctx, ctx->tdesc, td)) ||
+        (rc = ensure(ctx, ctx->thdr, hdr.size() + 256)))
+        return rc;
+    HIPC(ctx, hipMemcpyAsync(ctx->thdr.p, hdr.data(), hdr.size(), hipMemcpyHostToDevice, ctx->stream));
+    LAUNCH(ctx, text_header_kernel, n, 256, (const TextDigest*)ctx->tdesc.p, (const char*)ctx->thdr.p, pos,
+           (char*)ctx->text.p);
+    if (q)
+        LAUNCH(ctx, text_b64_kernel, (uint32_t)((q + 255) / 256), 256, (const uint8_t*)set->filt.p,
+               (const uint16_t*)set->ecf.p, (const TextDigest*)ctx->tdesc.p, n, q, (char*)ctx->text.p);
+    HIPC(ctx, hipMemcpyAsync(out, ctx->text.p, pos, hipMemcpyDeviceToHost, ctx->stream));
     HIPC(ctx, hipStreamSynchronize(ctx->stream));
     return 0;
 }

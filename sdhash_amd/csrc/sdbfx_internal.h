@@ -103,4 +103,16 @@ struct CmpPair {
     uint32_t pad;
 };
 
+// Text form: one digest of a set (sdbf::to_string layout).
+struct TextDigest {
+    uint64_t filt_first;   // first filter of the digest in the set
+    uint64_t q_begin;      // first base64 quantum (3 input bytes) of the digest
+    uint64_t body_off;     // text offset of the digest's first filter text
+    uint64_t text_off;     // text offset of the digest's header
+    uint32_t hdr_off;      // header bytes in the header blob
+    uint32_t hdr_len;
+    uint32_t bf_count;
+    uint32_t dd;           // 0: stream mode (one base64 run), else block mode
+};
+
 #endif

@@ -2046,3 +2046,81 @@ __global__ void pair_sum_kernel(const CmpPair* __restrict__ pairs, uint32_t npai
         sum = -1.0;
     scores[p] = sum < 0 ? -1 : (int32_t)floor(100.0 * sum / (double)denom + 0.5);
 }
+
+
+// ------------------------------------------------------------------ text form
+
+// The text form of a set (sdbf::to_string, sdbf/sdbf_class.cc:294-334, for
+// each digest in order).  text_header_kernel places each digest's header
+// (formatted on the host) and its closing newline; text_b64_kernel writes
+// the bodies, one thread per base64 quantum (3 input bytes -> 4 characters,
+// RFC 4648 alphabet with '=' padding, as base64/modp_b64.cc:79-118).  A
+// stream-mode body is the base64 of all its filters as one run; a block-mode
+// body is, per filter, ':', its element count as two lower-case hex digits,
+// ':' and the 344-character base64 of the filter (86 quanta, the last one of
+// a single byte), so 87 quanta per filter with the first one the prefix.
+__constant__ char c_b64[65] = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/";
+__constant__ char c_hex[17] = "0123456789abcdef";
+
+__global__ void text_header_kernel(const TextDigest* __restrict__ td, const char* __restrict__ hdr,
+                                   uint64_t text_len, char* __restrict__ text)
+{
+    const TextDigest D = td[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < D.hdr_len; i += blockDim.x)
+        text[D.text_off + i] = hdr[D.hdr_off + i];
+    if (threadIdx.x == 0) {
+        const uint64_t end = blockIdx.x + 1 < gridDim.x ? td[blockIdx.x + 1].text_off : text_len;
+        text[end - 1] = '\n';
+    }
+}
+
+__global__ void text_b64_kernel(const uint8_t* __restrict__ filt, const uint16_t* __restrict__ ecf,
+                                const TextDigest* __restrict__ td, uint32_t nd, uint64_t nq,
+                                char* __restrict__ text)
+{
+    const uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (q >= nq)
+        return;
+    uint32_t lo = 0, hi = nd;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (td[mid].q_begin <= q)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const TextDigest D = td[lo];
+    const uint64_t lq = q - D.q_begin;
+    const uint8_t* src = filt + D.filt_first * SDBFX_BF;
+    uint64_t ib;
+    uint32_t nb;
+    char* dst;
+    if (D.dd == 0) {
+        const uint64_t total = (uint64_t)D.bf_count * SDBFX_BF;
+        ib = 3 * lq;
+        nb = (uint32_t)min((uint64_t)3, total - ib);
+        dst = text + D.body_off + 4 * lq;
+    } else {
+        const uint64_t f = lq / 87;
+        const uint32_t r = (uint32_t)(lq % 87);
+        char* fd = text + D.body_off + f * 348;
+        if (r == 0) {
+            const uint32_t ec = ecf[D.filt_first + f];
+            fd[0] = ':';
+            fd[1] = c_hex[(ec >> 4) & 15];
+            fd[2] = c_hex[ec & 15];
+            fd[3] = ':';
+            return;
+        }
+        ib = f * SDBFX_BF + 3 * (r - 1);
+        nb = min(3u, (uint32_t)SDBFX_BF - 3 * (r - 1));
+        dst = fd + 4 + 4 * (r - 1);
+    }
+    const uint32_t v = (uint32_t)src[ib] << 16 | (nb > 1 ? (uint32_t)src[ib + 1] << 8 : 0u) |
+                       (nb > 2 ? (uint32_t)src[ib + 2] : 0u);
+    dst[0] = c_b64[v >> 18 & 63];
+    dst[1] = c_b64[v >> 12 & 63];
+    dst[2] = nb > 1 ? c_b64[v >> 6 & 63] : '=';
+    dst[3] = nb > 2 ? c_b64[v & 63] : '=';
+}
+

@@ -73,6 +73,9 @@ def main(lib):
     for i, od in enumerate(ods):
         same(s, i, od, "set %d" % i)
     assert np.array_equal(eng.compare_all(s), oracle.compare_all(ods))
+    # the text form of the set, base64 encoded by the device
+    assert s.to_text() == "".join(od.to_string() for od in ods).encode()
+    print("text ok", flush=True)
     # compare_to with a target digest of more than 1024 filters (chunked)
     big = synth.disk_image(8, 1100 * 16 * KIB)
     t = [oracle.digest_block(big, 16384, b"big")]

@@ -115,3 +115,20 @@ def test_device_resident_input(engine):
         od = oracle.digest_block(f, 16384) if i % 2 else oracle.digest_stream(f)
         check_same(s, i, od, f"device {i}")
     buf.free()
+
+
+def test_set_text_form(engine):
+    # sdbf_set::to_string of a mixed stream / block-mode set, the base64
+    # written by the device: byte-identical with the oracle's text
+    files = synth.file_set(61, 6, 300 * KIB)
+    files.append(synth.disk_image(62, 3 * MIB + 700))
+    names = [b"dir/f%d.bin" % i for i in range(len(files))]
+    modes = [0, 16384, 0, 4096, 0, 16384, 16384]
+    s = engine.new_set()
+    assert np.all(s.digest(files, names, dd_block=modes) == 0)
+    ods = [oracle.digest_block(f, m, n) if m else oracle.digest_stream(f, n)
+           for f, n, m in zip(files, names, modes)]
+    want = "".join(od.to_string() for od in ods).encode()
+    got = s.to_text()
+    assert got == want, next(i for i in range(min(len(got), len(want))) if got[i] != want[i])
+    assert engine.new_set().to_text() == b""
