@@ -241,9 +241,26 @@ def text_form(eng, ref) -> dict:
     cpu = time.perf_counter() - t
     if txt[:len(one)] != one:
         raise RuntimeError("device text differs from the oracle's")
+    # and read back into a new set (sdbf_set's file reader), bodies decoded
+    # by the device; the first and last digests must come back unchanged
+    back = eng.new_set()
+    eng.sync()
+    eng.profiling(True)
+    eng.reset_kernel_times()
+    t = time.perf_counter()
+    n = back.add_text(txt)
+    pwall = time.perf_counter() - t
+    eng.profiling(False)
+    pms = sum(eng.kernel_time(k)[0] for k in ("text_decode_kernel", "text_finish_kernel"))
+    for i in (0, len(ref) - 1):
+        a, b = ref.read(i), back.read(i)
+        if n != len(ref) or any(not np.array_equal(x, y) for x, y in zip(a, b)):
+            raise RuntimeError("text read back differs from the set written")
+    back.close()
     return {"filters": ref.filter_count(), "text_bytes": len(txt), "kernel_ms": kms,
             "kernel_GBps": len(txt) / (kms / 1e3) / 1e9, "wall_s": wall,
-            "cpu_oracle_MBps": len(one) / cpu / 1e6, "cpu_sample": "one digest, %d filters, 1 thread" % inf.bf_count}
+            "cpu_oracle_MBps": len(one) / cpu / 1e6, "cpu_sample": "one digest, %d filters, 1 thread" % inf.bf_count,
+            "parse_kernel_ms": pms, "parse_kernel_GBps": len(txt) / (pms / 1e3) / 1e9, "parse_wall_s": pwall}
 
 
 def batched_digest(eng, args) -> dict:

@@ -112,6 +112,20 @@ int sdbfx_set_read(sdbfx_ctx *ctx, const sdbfx_set *set, uint32_t idx, uint8_t *
  */
 int sdbfx_set_to_text(sdbfx_ctx *ctx, const sdbfx_set *set, char *out, uint64_t cap, uint64_t *len);
 
+/*
+ * Appends every digest of a text (lines in sdbf::to_string's form) to the
+ * set: the reading loop of sdbf_set::sdbf_set(const char*) (sdbf/sdbf_set.cc:
+ * 51-74, lines up to the first empty one) with each line read as
+ * sdbf::sdbf(const std::string&) reads it (sdbf/sdbf_class.cc:144-215).  The
+ * headers are read on the host; the base64 bodies and block-mode element
+ * counts are decoded on the device straight into the set, whose Hamming
+ * weights are computed there too.  Returns -1 for a line under 344 bytes and
+ * -2 for a malformed one (the reference's throw -1 / throw -2), with nothing
+ * added; on success *added (may be NULL) receives the number of digests.
+ * Only 256-byte filters are accepted.
+ */
+int sdbfx_set_add_text(sdbfx_ctx *ctx, sdbfx_set *set, const char *text, uint64_t len, uint32_t *added);
+
 /* ---- comparison ----------------------------------------------------- */
 
 /*

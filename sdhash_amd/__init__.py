@@ -19,8 +19,8 @@ SYMBOLS = (
     "sdbfx_ctx_create", "sdbfx_ctx_destroy", "sdbfx_last_error", "sdbfx_sync",
     "sdbfx_set_create", "sdbfx_set_destroy", "sdbfx_set_size", "sdbfx_set_filter_count",
     "sdbfx_set_digest", "sdbfx_set_digest_device", "sdbfx_set_add", "sdbfx_set_info",
-    "sdbfx_set_read", "sdbfx_set_to_text", "sdbfx_compare_all", "sdbfx_compare_to", "sdbfx_device_alloc",
-    "sdbfx_device_free", "sdbfx_upload", "sdbfx_set_profiling", "sdbfx_kernel_time",
+    "sdbfx_set_read", "sdbfx_set_to_text", "sdbfx_set_add_text", "sdbfx_compare_all", "sdbfx_compare_to",
+    "sdbfx_device_alloc", "sdbfx_device_free", "sdbfx_upload", "sdbfx_set_profiling", "sdbfx_kernel_time",
     "sdbfx_reset_kernel_times", "sdbfx_set_compare_method",
     "sdbfx_serve_create", "sdbfx_serve_destroy", "sdbfx_serve_error", "sdbfx_serve_ref_size",
     "sdbfx_serve_submit", "sdbfx_serve_poll", "sdbfx_serve_wait", "sdbfx_serve_stats",
@@ -78,6 +78,7 @@ def load():
     L.sdbfx_reset_kernel_times.argtypes = [P]
     L.sdbfx_set_compare_method.argtypes = [P, C.c_int]
     L.sdbfx_set_to_text.argtypes = [P, P, P, U64, C.POINTER(U64)]
+    L.sdbfx_set_add_text.argtypes = [P, P, P, U64, C.POINTER(U32)]
     L.sdbfx_serve_create.argtypes = [C.c_int, P, U64, U32, C.POINTER(P)]
     L.sdbfx_serve_destroy.argtypes = [P]
     L.sdbfx_serve_destroy.restype = None
@@ -93,7 +94,7 @@ def load():
         getattr(L, name).restype = getattr(L, name).restype or I32
     for name in ("sdbfx_ctx_create", "sdbfx_sync", "sdbfx_set_create", "sdbfx_set_digest",
                  "sdbfx_set_digest_device", "sdbfx_set_add", "sdbfx_set_info", "sdbfx_set_read",
-                 "sdbfx_set_to_text",
+                 "sdbfx_set_to_text", "sdbfx_set_add_text",
                  "sdbfx_compare_all", "sdbfx_compare_to", "sdbfx_device_alloc", "sdbfx_device_free",
                  "sdbfx_upload", "sdbfx_set_profiling", "sdbfx_kernel_time",
                  "sdbfx_reset_kernel_times", "sdbfx_set_compare_method", "sdbfx_serve_create",
@@ -187,6 +188,16 @@ class DigestSet:
         out = np.empty(max(n.value, 1), np.uint8)
         self.engine._check(lib.sdbfx_set_to_text(ctx, self.handle, _ptr(out), n.value, C.byref(n)))
         return out[: n.value].tobytes()
+
+    def add_text(self, text: bytes) -> int:
+        """Appends the digests of an sdbf text (sdbf_set's file reader: lines up
+        to the first empty one), decoding the base64 on the device; returns
+        how many were added.  Raises SdbfxError on a malformed line."""
+        buf = np.frombuffer(bytes(text), np.uint8)
+        n = C.c_uint32(0)
+        self.engine._check(self.engine.lib.sdbfx_set_add_text(
+            self.engine.ctx, self.handle, _ptr(buf) if buf.size else None, buf.size, C.byref(n)))
+        return n.value
 
     def digest(self, bufs, names=None, dd_block=0) -> np.ndarray:
         """Digests host buffers into this set; returns the per-input status."""

@@ -890,6 +890,150 @@ int sdbfx_set_to_text(sdbfx_ctx* ctx, const sdbfx_set* set, char* out, uint64_t 
     return 0;
 }
 
+int sdbfx_set_add_text(sdbfx_ctx* ctx, sdbfx_set* set, const char* text, uint64_t len, uint32_t* added)
+{
+    if (!ctx || !set || (!text && len))
+        return fail(ctx, "null argument");
+    if (added)
+        *added = 0;
+    HIPC(ctx, hipSetDevice(ctx->device));
+    auto bad = [&](const char* why) {
+        ctx->err = std::string("text: ") + why;
+        return -2;   // the reference's "invalid format" (sdbf_class.cc:152)
+    };
+    struct Hdr {
+        std::string name;
+        uint64_t orig, body;
+        uint32_t maxe, bfc, last, dd;
+        uint16_t last_ec;
+    };
+    std::vector<Hdr> hs;
+    uint64_t pos = 0, used = 0, nf = 0;
+    while (pos < len) {
+        const char* nl = (const char*)memchr(text + pos, '\n', len - pos);
+        const uint64_t end = nl ? (uint64_t)(nl - text) : len;
+        if (end == pos)
+            break;   // an empty line ends the set (sdbf_set.cc:57-58)
+        if (end - pos < 344) {
+            ctx->err = "text: line too short";
+            return -1;   // sdbf_class.cc:145-146
+        }
+        // header fields, ':'-separated, as sdbf_class.cc:149-215 reads them
+        uint64_t p = pos;
+        auto field = [&](std::string& f) {
+            const char* c = (const char*)memchr(text + p, ':', end - p);
+            const uint64_t e = c ? (uint64_t)(c - text) : end;
+            f.assign(text + p, e - p);
+            p = c ? e + 1 : end;
+            return c != nullptr;
+        };
+        auto number = [&](uint64_t& v, int base) {
+            std::string f;
+            if (!field(f) || f.empty())
+                return false;
+            char* e = nullptr;
+            v = strtoull(f.c_str(), &e, base);
+            return *e == 0;
+        };
+        Hdr h{};
+        std::string magic, f;
+        uint64_t v[7];
+        if (!field(magic) || magic.compare(0, 4, "sdbf") != 0)
+            return bad("not an sdbf line");
+        const bool dd = magic == "sdbf-dd";
+        if (!field(f) || !field(f) || !field(h.name) || !number(v[0], 10) || !field(f) || !number(v[1], 10) ||
+            !number(v[2], 10) || !number(v[3], 16) || !number(v[4], 10) || !number(v[5], 10))
+            return bad("header");
+        h.orig = v[0];
+        if (v[1] != SDBFX_BF || v[5] == 0 || v[5] > 0xFFFFFFFFull || v[4] > 0xFFFFull)
+            return bad("filter size or count");
+        h.maxe = (uint32_t)v[4];
+        h.bfc = (uint32_t)v[5];
+        const uint64_t raw = (uint64_t)h.bfc * SDBFX_BF;
+        if (dd) {
+            if (!number(v[6], 10) || v[6] == 0)
+                return bad("dd block size");
+            h.dd = (uint32_t)v[6];
+            h.body = p - 1;   // each filter: ':' 2 hex digits ':' 344 characters
+            if (text[h.body] != ':' || end - h.body < (uint64_t)h.bfc * 348)
+                return bad("block-mode body");
+            const uint64_t lf = h.body + (uint64_t)(h.bfc - 1) * 348;
+            unsigned x = 0;
+            if (sscanf(std::string(text + lf + 1, 2).c_str(), "%2x", &x) != 1)
+                return bad("element count");
+            h.last_ec = (uint16_t)x;
+        } else {
+            std::string lc;
+            // last count, then the base64 run (sdbf_class.cc:198-214)
+            const char* c = (const char*)memchr(text + p, ':', end - p);
+            if (!c)
+                return bad("last count");
+            lc.assign(text + p, (uint64_t)(c - text) - p);
+            char* e = nullptr;
+            h.last = (uint32_t)strtoul(lc.c_str(), &e, 10);
+            if (lc.empty() || *e)
+                return bad("last count");
+            h.body = (uint64_t)(c - text) + 1;
+            if (end - h.body < (raw + 2) / 3 * 4)
+                return bad("stream body");
+            h.last_ec = (uint16_t)h.last;
+        }
+        nf += h.bfc;
+        used = end;
+        hs.push_back(std::move(h));
+        pos = end + 1;
+    }
+    if (hs.empty())
+        return 0;
+    // descriptors; the text goes to the device through the pinned buffers
+    std::vector<TextParse> tp(hs.size());
+    uint64_t q = 0, fcur = set->nfilt;
+    for (size_t d = 0; d < hs.size(); d++) {
+        TextParse& t = tp[d];
+        t.filt_first = fcur;
+        t.q_begin = q;
+        t.body_off = hs[d].body;
+        t.bf_count = hs[d].bfc;
+        t.dd = hs[d].dd;
+        t.max_elem = hs[d].maxe;
+        t.last_count = hs[d].last;
+        q += t.dd ? (uint64_t)t.bf_count * 87 : ((uint64_t)t.bf_count * SDBFX_BF + 2) / 3;
+        fcur += t.bf_count;
+    }
+    int rc;
+    if ((rc = set_reserve(ctx, set, set->nfilt + nf)) || (rc = ensure(ctx, ctx->text, used + 256)) ||
+        (rc = upload(ctx, ctx->tdesc, tp)) || (rc = ensure(ctx, ctx->thdr, 256)))
+        return rc;
+    if ((rc = upload_host(ctx, (uint8_t*)ctx->text.p, (const uint8_t*)text, used)))
+        return rc;
+    HIPC(ctx, hipMemsetAsync(ctx->thdr.p, 0, 4, ctx->stream));
+    LAUNCH(ctx, text_decode_kernel, (uint32_t)((q + 255) / 256), 256, (const char*)ctx->text.p,
+           (const TextParse*)ctx->tdesc.p, (uint32_t)tp.size(), q, (uint8_t*)set->filt.p, (uint16_t*)set->ecf.p,
+           (uint32_t*)ctx->thdr.p);
+    LAUNCH(ctx, text_finish_kernel, (uint32_t)((nf + 255) / 256), 256, (const TextParse*)ctx->tdesc.p,
+           (uint32_t)tp.size(), set->nfilt, nf, (const uint8_t*)set->filt.p, (uint16_t*)set->ecf.p,
+           (uint16_t*)set->hamf.p);
+    uint32_t err = 0;
+    HIPC(ctx, hipMemcpyAsync(&err, ctx->thdr.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    if (err)
+        return bad("base64 or element counts");
+    for (size_t d = 0; d < hs.size(); d++) {
+        set->names.push_back(hs[d].name);
+        set->orig.push_back(hs[d].orig);
+        set->first.push_back(tp[d].filt_first);
+        set->bfc.push_back(hs[d].bfc);
+        set->last.push_back(hs[d].dd ? 0 : hs[d].last);
+        set->dd.push_back(hs[d].dd);
+        set->maxe.push_back(hs[d].maxe);
+        set->last_ec.push_back(hs[d].last_ec);
+    }
+    set->nfilt += nf;
+    if (added)
+        *added = (uint32_t)hs.size();
+    return 0;
+}
+
 int sdbfx_compare_all(sdbfx_ctx* ctx, const sdbfx_set* set, int32_t* scores)
 {
     if (!ctx || !set || !scores)

@@ -115,4 +115,15 @@ struct TextDigest {
     uint32_t dd;           // 0: stream mode (one base64 run), else block mode
 };
 
+// Text form, parsing: one digest's body in the uploaded text.
+struct TextParse {
+    uint64_t filt_first;   // destination filter of the digest in the set
+    uint64_t q_begin;      // first base64 quantum of the digest
+    uint64_t body_off;     // text offset of its base64 run (stream) or first ':' (block mode)
+    uint32_t bf_count;
+    uint32_t dd;           // 0: stream mode
+    uint32_t max_elem;
+    uint32_t last_count;
+};
+
 #endif

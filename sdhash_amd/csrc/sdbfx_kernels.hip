@@ -2124,3 +2124,114 @@ __global__ void text_b64_kernel(const uint8_t* __restrict__ filt, const uint16_t
     dst[3] = nb > 2 ? c_b64[v & 63] : '=';
 }
 
+// Parsing the text form (sdbf::sdbf(const std::string&), sdbf/sdbf_class.cc:
+// 144-215): the host has read each line's header fields; text_decode_kernel
+// decodes the bodies, one thread per base64 quantum, straight into the set
+// (block mode: quantum 0 of each filter is its ':xx:' element count), and
+// raises *err on a character outside the alphabet or a misplaced '='.
+// text_finish_kernel then fills each new filter's Hamming weight
+// (sdbf::compute_hamming, sdbf_class.cc:389-401) and, in stream mode, its
+// element count (max_elem, or last_count for the last filter).
+__device__ __forceinline__ int b64_value(uint32_t c)
+{
+    return c >= 'A' && c <= 'Z' ? (int)(c - 'A')
+           : c >= 'a' && c <= 'z' ? (int)(c - 'a' + 26)
+           : c >= '0' && c <= '9' ? (int)(c - '0' + 52)
+           : c == '+' ? 62 : c == '/' ? 63 : -1;
+}
+
+__device__ __forceinline__ int hex_value(uint32_t c)
+{
+    return c >= '0' && c <= '9' ? (int)(c - '0') : c >= 'a' && c <= 'f' ? (int)(c - 'a' + 10)
+           : c >= 'A' && c <= 'F' ? (int)(c - 'A' + 10) : -1;
+}
+
+__device__ __forceinline__ uint32_t find_digest(const TextParse* __restrict__ tp, uint32_t nd, uint64_t q)
+{
+    uint32_t lo = 0, hi = nd;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (tp[mid].q_begin <= q)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ void text_decode_kernel(const char* __restrict__ text, const TextParse* __restrict__ tp, uint32_t nd,
+                                   uint64_t nq, uint8_t* __restrict__ filt, uint16_t* __restrict__ ecf,
+                                   uint32_t* __restrict__ err)
+{
+    const uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (q >= nq)
+        return;
+    const TextParse D = tp[find_digest(tp, nd, q)];
+    const uint64_t lq = q - D.q_begin;
+    uint64_t ob;   // output byte within the digest
+    uint32_t nb;
+    const char* src;
+    if (D.dd == 0) {
+        const uint64_t total = (uint64_t)D.bf_count * SDBFX_BF;
+        ob = 3 * lq;
+        nb = (uint32_t)min((uint64_t)3, total - ob);
+        src = text + D.body_off + 4 * lq;
+    } else {
+        const uint64_t f = lq / 87;
+        const uint32_t r = (uint32_t)(lq % 87);
+        const char* fs = text + D.body_off + f * 348;
+        if (r == 0) {
+            const int h1 = hex_value((uint8_t)fs[1]), h0 = hex_value((uint8_t)fs[2]);
+            if (fs[0] != ':' || fs[3] != ':' || h1 < 0 || h0 < 0)
+                atomicOr(err, 1u);
+            ecf[D.filt_first + f] = (uint16_t)((h1 & 15) << 4 | (h0 & 15));
+            return;
+        }
+        ob = f * SDBFX_BF + 3 * (r - 1);
+        nb = min(3u, (uint32_t)SDBFX_BF - 3 * (r - 1));
+        src = fs + 4 + 4 * (r - 1);
+    }
+    const int v0 = b64_value((uint8_t)src[0]), v1 = b64_value((uint8_t)src[1]);
+    const int v2 = nb > 1 ? b64_value((uint8_t)src[2]) : (src[2] == '=' ? 0 : -1);
+    const int v3 = nb > 2 ? b64_value((uint8_t)src[3]) : (src[3] == '=' ? 0 : -1);
+    if ((v0 | v1 | v2 | v3) < 0)
+        atomicOr(err, 1u);
+    const uint32_t v = (uint32_t)(v0 & 63) << 18 | (uint32_t)(v1 & 63) << 12 | (uint32_t)(v2 & 63) << 6 |
+                       (uint32_t)(v3 & 63);
+    uint8_t* dst = filt + D.filt_first * SDBFX_BF + ob;
+    dst[0] = (uint8_t)(v >> 16);
+    if (nb > 1)
+        dst[1] = (uint8_t)(v >> 8);
+    if (nb > 2)
+        dst[2] = (uint8_t)v;
+}
+
+__global__ void text_finish_kernel(const TextParse* __restrict__ tp, uint32_t nd, uint64_t f0, uint64_t nf,
+                                   const uint8_t* __restrict__ filt, uint16_t* __restrict__ ecf,
+                                   uint16_t* __restrict__ hamf)
+{
+    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (t >= nf)
+        return;
+    const uint64_t fi = f0 + t;
+    uint32_t lo = 0, hi = nd;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (tp[mid].filt_first <= fi)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const TextParse D = tp[lo];
+    const uint4* w = (const uint4*)(filt + fi * SDBFX_BF);
+    uint32_t h = 0;
+#pragma unroll
+    for (int k = 0; k < SDBFX_BF / 16; k++) {
+        const uint4 x = w[k];
+        h += __popc(x.x) + __popc(x.y) + __popc(x.z) + __popc(x.w);
+    }
+    hamf[fi] = (uint16_t)h;
+    if (D.dd == 0)
+        ecf[fi] = (uint16_t)(fi + 1 < D.filt_first + D.bf_count ? D.max_elem : D.last_count);
+}
+

@@ -75,6 +75,29 @@ def main(lib):
     assert np.array_equal(eng.compare_all(s), oracle.compare_all(ods))
     # the text form of the set, base64 encoded by the device
     assert s.to_text() == "".join(od.to_string() for od in ods).encode()
+    # ... and read back: headers on the host, bodies decoded by the device
+    text = s.to_text()
+    r = eng.new_set()
+    assert r.add_text(text + b"\n" + b"junk after the empty line") == len(ods)
+    for i, od in enumerate(ods):
+        same(r, i, od, "text %d" % i)
+        assert r.info(i).name.encode() == od.name and r.info(i).orig_size == od.orig_size
+    assert r.to_text() == text
+    assert np.array_equal(eng.compare_all(r), oracle.compare_all(ods))
+    lines = text.split(b"\n")[:-1]
+    bad = [lines[0][:300], b"sdbx" + lines[0][4:], lines[0][:-8], lines[1][:-9],
+           lines[0].replace(b":256:", b":128:", 1), lines[0][:-5] + b"@" + lines[0][-4:],
+           lines[1][:-5] + b"*" + lines[1][-4:]]
+    for k, b in enumerate(bad):
+        n0 = len(r)
+        try:
+            r.add_text(lines[2] + b"\n" + b + b"\n")
+        except sdhash_amd.SdbfxError as e:
+            assert "text" in str(e), (k, str(e))
+        else:
+            raise AssertionError("malformed line %d accepted" % k)
+        assert len(r) == n0
+    assert r.add_text(b"") == 0 and r.add_text(b"\n" + lines[0]) == 0
     print("text ok", flush=True)
     # compare_to with a target digest of more than 1024 filters (chunked)
     big = synth.disk_image(8, 1100 * 16 * KIB)

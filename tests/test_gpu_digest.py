@@ -4,6 +4,7 @@ import numpy as np
 import pytest
 
 import oracle
+import sdhash_amd
 from sdhash_amd import synth
 
 pytestmark = pytest.mark.gpu
@@ -132,3 +133,43 @@ def test_set_text_form(engine):
     got = s.to_text()
     assert got == want, next(i for i in range(min(len(got), len(want))) if got[i] != want[i])
     assert engine.new_set().to_text() == b""
+
+
+def test_set_add_text(engine):
+    # the reverse: the oracle's text read into a set (sdbf_set's file reader),
+    # the bodies decoded by the device; every filter, element count and
+    # Hamming weight, the scores and the text written back
+    files = synth.file_set(63, 40, 200 * KIB)
+    files += [synth.disk_image(64, 20 * MIB + 123), synth.disk_image(65, 9 * MIB)]
+    names = [b"in/%03d" % i for i in range(len(files))]
+    modes = [16384 if i % 3 == 0 else 0 for i in range(len(files))]
+    modes[-2:] = [0, 4096]
+    ods = [oracle.digest_block(f, m, n) if m else oracle.digest_stream(f, n)
+           for f, n, m in zip(files, names, modes)]
+    text = "".join(od.to_string() for od in ods).encode()
+    s = engine.new_set()
+    assert s.add_text(text) == len(ods)
+    for i, od in enumerate(ods):
+        f, ec, hm = s.read(i)
+        info = s.info(i)
+        assert info.bf_count == od.bf_count and info.name.encode() == od.name, i
+        assert info.orig_size == od.orig_size and info.dd_block_size == od.dd_block_size, i
+        assert np.array_equal(f, od.filters), i
+        od.cstruct()
+        assert np.array_equal(hm, od.hamming[: od.bf_count]), i
+        if od.elem_counts is not None:
+            assert np.array_equal(ec, od.elem_counts), i
+        else:
+            assert info.last_count == od.last_count, i
+    assert s.to_text() == text
+    assert np.array_equal(engine.compare_all(s), oracle.compare_all(ods, threads=oracle.threads_available()))
+    # appended after digests already in the set
+    s2 = engine.new_set()
+    assert np.all(s2.digest(files[:3], names[:3], dd_block=modes[:3]) == 0)
+    assert s2.add_text(text) == len(ods)
+    assert s2.to_text() == b"".join(od.to_string().encode() for od in ods[:3]) + text
+    # a malformed body leaves the set as it was
+    bad = text.replace(b"A", b"!", 1)
+    with pytest.raises(sdhash_amd.SdbfxError):
+        s2.add_text(bad)
+    assert len(s2) == 3 + len(ods)
