@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -49,7 +50,7 @@ struct sdbfx_ctx {
     std::map<std::string, std::pair<double, uint64_t>> times;
     std::vector<hipEvent_t> evpool;
     DevBuf stage, blocks, tiles, tstate, units, files, cand, sfilt, sham, counts, gsrc, gdst, gcnt,
-        glast, groups, pairs, partial, imax, scores, gbest, rmap, rcode, text, tdesc, thdr;
+        glast, groups, pairs, partial, imax, scores, gbest, rmap, rcode, text, tdesc, thdr, cofirst, cocount;
     // two pinned host buffers for staging caller memory to the device
     void* pin[2] = {nullptr, nullptr};
     hipEvent_t pin_ev[2] = {nullptr, nullptr};
@@ -64,6 +65,13 @@ struct sdbfx_set {
     std::vector<uint32_t> bfc, last, dd, maxe, last_ec;
     uint64_t nfilt = 0;
     DevBuf filt, ecf, hamf;
+    // column order of the digests as comparison targets (col_order_kernel),
+    // built on first use: per digest whether it is built, and the event
+    // after the last build
+    DevBuf cord, ckey;
+    mutable std::vector<uint8_t> cbuilt;
+    mutable hipEvent_t cev = nullptr;
+    mutable std::mutex cmu;
 };
 
 namespace {
@@ -186,6 +194,10 @@ int set_reserve(sdbfx_ctx* ctx, sdbfx_set* s, uint64_t total)
     if (int rc = grow_keep(ctx, s->ecf, total * 2 + 256, s->nfilt * 2))
         return rc;
     if (int rc = grow_keep(ctx, s->hamf, total * 2 + 256, s->nfilt * 2))
+        return rc;
+    if (int rc = grow_keep(ctx, s->cord, total * 4 + 256, s->nfilt * 4))
+        return rc;
+    if (int rc = grow_keep(ctx, s->ckey, total * 4 + 256, s->nfilt * 4))
         return rc;
     return 0;
 }
@@ -463,6 +475,11 @@ struct PairRef {
 // included).
 bool use_mfma(const sdbfx_ctx* ctx, const std::vector<uint64_t>& inst, const std::vector<uint64_t>& tgt)
 {
+    // the MFMA kernel addresses a target digest's filters by a 32-bit
+    // 16-byte-chunk index
+    for (uint64_t t : tgt)
+        if (t >= (1ull << 28))
+            return false;
     if (ctx->cmp_method != SDBFX_CMP_AUTO)
         return ctx->cmp_method == SDBFX_CMP_MFMA;
     double valu = 0, mfma = 0;
@@ -595,9 +612,48 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
         (rc = ensure(ctx, ctx->partial, partial_n * 4 + 256)) || (rc = ensure(ctx, ctx->imax, inst * 4 + 256)) ||
         (rc = ensure(ctx, ctx->scores, npairs * 4 + 256)))
         return rc;
-    SetView va{(const uint32_t*)S[0]->filt.p, (const uint16_t*)S[0]->ecf.p, (const uint16_t*)S[0]->hamf.p};
-    SetView vb{(const uint32_t*)S[1]->filt.p, (const uint16_t*)S[1]->ecf.p, (const uint16_t*)S[1]->hamf.p};
+    auto view = [](const sdbfx_set* X) {
+        return SetView{(const uint32_t*)X->filt.p, (const uint16_t*)X->ecf.p, (const uint16_t*)X->hamf.p,
+                       (const uint32_t*)X->cord.p, (const uint32_t*)X->ckey.p};
+    };
+    const SetView va = view(S[0]), vb = view(S[1]);
     if (wi && mfma) {
+        // the column order of every target digest not ordered yet (after
+        // any earlier build, possibly on another stream)
+        std::vector<uint64_t> cfirst[2];
+        std::vector<uint32_t> ccount[2];
+        for (int si = 0; si < (all ? 1 : 2); si++) {
+            const sdbfx_set* T = S[si];
+            std::lock_guard<std::mutex> lk(T->cmu);
+            if (T->cev)
+                HIPC(ctx, hipStreamWaitEvent(ctx->stream, T->cev, 0));
+            T->cbuilt.resize(T->names.size(), 0);
+            std::vector<uint64_t>& dfirst = cfirst[si];
+            std::vector<uint32_t>& dcount = ccount[si];
+            for (const CmpGroup& g : groups) {
+                const uint32_t d = (uint32_t)(g.tgt_set && !all ? 1 : 0);
+                if ((int)d != si)
+                    continue;
+                // the digest of this group: the one whose first filter it is
+                const uint32_t idx = (uint32_t)(std::lower_bound(T->first.begin(), T->first.end(), g.tgt_first) -
+                                                T->first.begin());
+                if (idx < T->first.size() && !T->cbuilt[idx]) {
+                    T->cbuilt[idx] = 1;
+                    dfirst.push_back(g.tgt_first);
+                    dcount.push_back(g.tgt_count);
+                }
+            }
+            if (!dfirst.empty()) {
+                if ((rc = upload(ctx, ctx->cofirst, dfirst)) || (rc = upload(ctx, ctx->cocount, dcount)))
+                    return rc;
+                LAUNCH(ctx, col_order_kernel, (uint32_t)dfirst.size(), 1024, (const uint16_t*)T->ecf.p,
+                       (const uint16_t*)T->hamf.p, (const uint64_t*)ctx->cofirst.p,
+                       (const uint32_t*)ctx->cocount.p, (uint32_t*)T->cord.p, (uint32_t*)T->ckey.p);
+                if (!T->cev)
+                    HIPC(ctx, hipEventCreate(&T->cev));
+                HIPC(ctx, hipEventRecord(T->cev, ctx->stream));
+            }
+        }
         // per-instance maxima shared by the work items for pruning (from
         // 0), and the row data of every instance
         if ((rc = ensure(ctx, ctx->gbest, inst * 8 + 256)) || (rc = ensure(ctx, ctx->rmap, inst * 8 + 256)) ||
@@ -666,7 +722,8 @@ void sdbfx_ctx_destroy(sdbfx_ctx* ctx)
     DevBuf* bufs[] = {&ctx->stage, &ctx->blocks, &ctx->tiles, &ctx->tstate, &ctx->units, &ctx->files,
                       &ctx->cand, &ctx->sfilt, &ctx->sham, &ctx->counts, &ctx->gsrc, &ctx->gdst,
                       &ctx->gcnt, &ctx->glast, &ctx->groups, &ctx->pairs, &ctx->partial, &ctx->imax, &ctx->scores,
-                      &ctx->gbest, &ctx->rmap, &ctx->rcode, &ctx->text, &ctx->tdesc, &ctx->thdr};
+                      &ctx->gbest, &ctx->rmap, &ctx->rcode, &ctx->text, &ctx->tdesc, &ctx->thdr,
+                      &ctx->cofirst, &ctx->cocount};
     for (DevBuf* b : bufs)
         if (b->p)
             (void)hipFree(b->p);
@@ -718,6 +775,12 @@ void sdbfx_set_destroy(sdbfx_set* set)
         (void)hipFree(set->ecf.p);
     if (set->hamf.p)
         (void)hipFree(set->hamf.p);
+    if (set->cord.p)
+        (void)hipFree(set->cord.p);
+    if (set->ckey.p)
+        (void)hipFree(set->ckey.p);
+    if (set->cev)
+        (void)hipEventDestroy(set->cev);
     delete set;
 }
 

@@ -1334,11 +1334,14 @@ __global__ __launch_bounds__(64) void gather_filters_kernel(
 // stored as an exact fraction (num << 16 | den); 0xFFFFFFFF when no target
 // filter qualified (the reference's -1).
 // A filter set on the device: filters (64 words each), per-filter element
-// counts and Hamming weights.
+// counts and Hamming weights, and the column order of its target digests
+// (col_order_kernel; read by cmp_mfma_kernel only).
 struct SetView {
     const uint32_t* filt;
     const uint16_t* ecf;
     const uint16_t* hamf;
+    const uint32_t* cord;   // position -> filter index within its digest
+    const uint32_t* ckey;   // position -> element count | Hamming weight << 16
 };
 
 __global__ __launch_bounds__(64) void cmp_kernel(
@@ -1579,6 +1582,127 @@ struct MfmaWaveSmem {
     uint32_t bfix[SDBFX_MROWS];     // per row: floor(best fraction * 2^16), or its initial accumulator
 };
 
+// Column order of a target digest for cmp_mfma_kernel.  A 128-column tile
+// whose columns share one size code takes per-row initial values (a quarter
+// of the tile's set-up otherwise), so the filters of every code held by at
+// least 128 filters of the digest are sorted by code, one tile after another.
+// The filters of rarer codes (among them the low element counts, whose low
+// cutoffs let many unrelated pairs through) stay mixed, as in the digest: met
+// as a run of 128, they would be compared with rows whose best fraction is
+// still low, and most of their pairs would become candidates.  The rare ones
+// come first, then the sorted ones; the 128-filter tiles are then dealt over
+// the digest's positions with a golden-ratio stride (position tile P holds
+// tile P * k mod m of that order, m full tiles, k coprime with m; a partial
+// last tile stays last), so that every range of positions that a work item
+// takes mixes all of them.  A counting sort, one workgroup per digest.  The
+// maximum over a digest's filters does not depend on their order, so no
+// score changes.  Position p of the digest (from its first filter f0) holds
+// cord[f0 + p], the index of a filter within the digest, and ckey[f0 + p], its
+// element count | Hamming weight << 16.
+#define CO_BINS 1024
+__device__ __forceinline__ uint32_t co_bin(uint32_t s)
+{
+    return s >= SDBFX_MIN_ELEM ? min(s, 1025u) - SDBFX_MIN_ELEM : CO_BINS - 1;
+}
+
+// k^-1 mod m for k near 0.618 m with gcd(k, m) = 1 (1 when m <= 2).
+__device__ uint32_t co_stride_inverse(uint32_t m)
+{
+    if (m <= 2)
+        return 1;
+    uint32_t k = max(1u, (uint32_t)(((uint64_t)m * 2654435769ull) >> 32));
+    for (;; k++) {
+        // extended Euclid on (m, k): gcd and the inverse of k
+        int64_t t0 = 0, t1 = 1, r0 = m, r1 = k;
+        while (r1) {
+            const int64_t q = r0 / r1, t2 = t0 - q * t1, r2 = r0 - q * r1;
+            t0 = t1;
+            t1 = t2;
+            r0 = r1;
+            r1 = r2;
+        }
+        if (r0 == 1)
+            return (uint32_t)(t0 < 0 ? t0 + m : t0);
+    }
+}
+
+__global__ __launch_bounds__(1024) void col_order_kernel(const uint16_t* __restrict__ ecf,
+                                                         const uint16_t* __restrict__ hamf,
+                                                         const uint64_t* __restrict__ dfirst,
+                                                         const uint32_t* __restrict__ dcount,
+                                                         uint32_t* __restrict__ cord, uint32_t* __restrict__ ckey)
+{
+    constexpr uint32_t kRare = 0x80000000u;   // bin start of a rare code
+    constexpr uint32_t kPer = 8;              // filters per thread per step, loaded first
+    // per bin: its count, then its next rank; [CO_BINS]: the next rank of the rare codes
+    __shared__ uint32_t hist[CO_BINS + 1];
+    __shared__ uint32_t wsum[16], wsumr[16];
+    __shared__ uint32_t kinv;
+    const uint64_t f0 = dfirst[blockIdx.x];
+    const uint32_t n = dcount[blockIdx.x];
+    const uint32_t t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const uint32_t m = n / 128;   // full tiles
+    hist[t] = 0;
+    if (t == 0) {
+        kinv = m ? co_stride_inverse(m) : 1u;
+        hist[CO_BINS] = 0;
+    }
+    __syncthreads();
+    for (uint32_t i0 = 0; i0 < n; i0 += kPer * 1024) {
+        uint32_t e[kPer];
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; k++) {
+            const uint32_t i = i0 + k * 1024 + t;
+            e[k] = i < n ? ecf[f0 + i] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; k++)
+            if (i0 + k * 1024 + t < n)
+                atomicAdd(&hist[co_bin(e[k])], 1u);
+    }
+    __syncthreads();
+    // bin starts: rare codes share ranks 0 .. R-1 (hist[CO_BINS]), the others
+    // follow in code order.  Thread t owns bin t.
+    const uint32_t v = hist[t];
+    uint32_t wtot, wrare;
+    uint32_t run = wave_excl_sum(v >= 128 ? v : 0u, (int)lane, &wtot);
+    (void)wave_excl_sum(v >= 128 ? 0u : v, (int)lane, &wrare);
+    if (lane == 0) {
+        wsum[wv] = wtot;
+        wsumr[wv] = wrare;
+    }
+    __syncthreads();
+    uint32_t nrare = 0;
+    for (uint32_t q = 0; q < 16; q++)
+        nrare += wsumr[q];
+    for (uint32_t q = 0; q < wv; q++)
+        run += wsum[q];
+    hist[t] = v >= 128 ? nrare + run : kRare;
+    __syncthreads();
+    const uint32_t ki = kinv;
+    for (uint32_t i0 = 0; i0 < n; i0 += kPer * 1024) {
+        uint32_t e[kPer], hm[kPer];
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; k++) {
+            const uint32_t i = i0 + k * 1024 + t;
+            e[k] = i < n ? ecf[f0 + i] : 0u;
+            hm[k] = i < n ? hamf[f0 + i] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; k++) {
+            const uint32_t i = i0 + k * 1024 + t;
+            if (i < n) {
+                const uint32_t bin = co_bin(e[k]);
+                // bins of rare codes keep kRare; the counters stay below 2^31
+                const uint32_t r = atomicAdd(&hist[(hist[bin] & kRare) ? (uint32_t)CO_BINS : bin], 1u);
+                const uint32_t p = r < m * 128 ? (uint32_t)((uint64_t)(r / 128) * ki % m) * 128 + r % 128 : r;
+                cord[f0 + p] = i;
+                ckey[f0 + p] = e[k] | hm[k] << 16;
+            }
+        }
+    }
+}
+
 // One instance per thread: where its reference filter is, and its size code
 // and Hamming weight (the row data of cmp_mfma_kernel).
 __global__ void inst_map_kernel(SetView A, SetView B, const CmpPair* __restrict__ pairs, uint32_t npairs,
@@ -1699,50 +1823,51 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
         const uint64_t j0 = g.tgt_first + (uint64_t)c * g.chunk;
         const uint32_t nj = (uint32_t)min((uint64_t)g.chunk, g.tgt_count - (uint64_t)c * g.chunk);
         const uint32_t* __restrict__ tf = g.tgt_set ? B.filt : A.filt;
-        const uint16_t* __restrict__ tec = g.tgt_set ? B.ecf : A.ecf;
-        const uint16_t* __restrict__ tham = g.tgt_set ? B.hamf : A.hamf;
-        // the work item's target filters as 16-byte chunks: column j, chunk q at j * 16 + q
-        const uint4* __restrict__ tbase = (const uint4*)(tf + j0 * SDBFX_BFW);
+        const uint32_t* __restrict__ tord = g.tgt_set ? B.cord : A.cord;
+        const uint32_t* __restrict__ tkey = g.tgt_set ? B.ckey : A.ckey;
+        // the target digest's filters as 16-byte chunks: filter L of the
+        // digest, chunk q at L * 16 + q
+        const uint4* __restrict__ tbase = (const uint4*)(tf + g.tgt_first * SDBFX_BFW);
         bool anyq = false;
 
-        // column data of a tile: element count and Hamming weight (0 past
-        // the end)
-        auto col_load = [&](uint32_t t0, uint32_t* s2, uint32_t* e2) {
+        // column data of a tile in column order: the filter (index in the
+        // digest; columns past the end repeat the last one), and its element
+        // count | Hamming weight << 16 (0 past the end)
+        auto col_load = [&](uint32_t t0, uint32_t* cl, uint32_t* ky) {
             const uint32_t ncol = min(128u, nj - t0);
 #pragma unroll
             for (int bn = 0; bn < MF_B; bn++) {
                 const uint32_t col = 32 * bn + r32;
-                const bool ok = col < ncol;
-                s2[bn] = ok ? tec[j0 + t0 + col] : 0u;
-                e2[bn] = ok ? tham[j0 + t0 + col] : 0u;
+                const uint64_t p = j0 + t0 + min(col, ncol - 1);
+                cl[bn] = tord[p];
+                ky[bn] = col < ncol ? tkey[p] : 0u;
             }
         };
-        uint32_t s2[MF_B], e2[MF_B];
-        col_load(0, s2, e2);
+        uint32_t cl[MF_B], ky[MF_B];
+        col_load(0, cl, ky);
 
         for (uint32_t t0 = 0; t0 < nj; t0 += 32 * MF_B) {
             const uint32_t tn0 = t0 + 32 * MF_B < nj ? t0 + 32 * MF_B : t0;   // the next tile (or this one)
             uint32_t ccode[MF_B], sx = 0, cmn = 0xFFFFu, cmx = 0;
 #pragma unroll
             for (int bn = 0; bn < MF_B; bn++) {
-                const bool q = s2[bn] >= SDBFX_MIN_ELEM;
+                const uint32_t s2 = ky[bn] & 0xFFFFu;
+                const bool q = s2 >= SDBFX_MIN_ELEM;
                 anyq |= q;
                 sx |= q;
-                ccode[bn] = 2 * size_code(s2[bn]);
+                ccode[bn] = 2 * size_code(s2);
                 cmn = min(cmn, ccode[bn]);
                 cmx = max(cmx, ccode[bn]);
             }
             // each lane's columns as uint4 indices of their first chunk
-            // (columns past the end repeat the last one)
             uint32_t e2c[MF_B], cbt[MF_B];
-            const uint32_t ncol = min(128u, nj - t0);
 #pragma unroll
             for (int bn = 0; bn < MF_B; bn++) {
-                e2c[bn] = e2[bn] + 1;
-                cbt[bn] = (t0 + min(32u * bn + r32, ncol - 1)) * 16 + h;
+                e2c[bn] = (ky[bn] >> 16) + 1;
+                cbt[bn] = cl[bn] * 16 + h;
             }
             // the next tile's column data, consumed after this tile
-            col_load(tn0, s2, e2);
+            col_load(tn0, cl, ky);
             // no qualifying target filter: no pair of the tile can pass
             if (__ballot(sx != 0) == 0)
                 continue;
@@ -1948,13 +2073,13 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
                     if (base + lane < nlist) {
                         const uint32_t e = clist[base + lane];
                         const uint32_t row = e & 127u, col = e >> 7;
-                        const uint64_t j = nt + col;
+                        const uint64_t p = nt + col;
+                        const uint32_t L = tord[p], key = tkey[p];
                         const uint32_t ri = rinfo[row];
-                        const uint32_t cutoff = (uint32_t)(-1 - (int)ncut[(ri & 0xFFFFu) + size_code(tec[j])]);
+                        const uint32_t cutoff = (uint32_t)(-1 - (int)ncut[(ri & 0xFFFFu) + size_code(key & 0xFFFFu)]);
                         uint32_t p32, p64, p128, p256;
-                        prefix_counts(atile + row * 16, row & 15, (const uint4*)(tf + j * SDBFX_BFW), p32, p64,
-                                      p128, p256);
-                        merge_pair(p32, p64, p128, p256, cutoff, ri >> 16, tham[j], &best[row]);
+                        prefix_counts(atile + row * 16, row & 15, tbase + (uint64_t)L * 16, p32, p64, p128, p256);
+                        merge_pair(p32, p64, p128, p256, cutoff, ri >> 16, key >> 16, &best[row]);
                     }
                 }
                 wave_lds_sync();

@@ -150,6 +150,14 @@ def main(lib):
         assert np.array_equal(eng.compare_all(gd), want_dup), method
         assert np.array_equal(eng.compare_all(gr), want_rep), method
         print("compare method %d ok" % method, flush=True)
+    # a target set that grows after the column order of its first digest
+    # was built: the new digest is ordered on its own, the old order kept
+    eng.compare_method(sdhash_amd.CMP_MFMA)
+    t2 = oracle.digest_block(big[: 300 * 16 * KIB], 4096, b"big2")
+    gt.add(t2.name, t2.orig_size, t2.bf_count, t2.last_count, t2.dd_block_size, t2.max_elem, t2.filters,
+           t2.elem_counts)
+    assert np.array_equal(eng.compare_to(gq2, gt), oracle.compare_to(q2, t + [t2]))
+    eng.compare_method(sdhash_amd.CMP_AUTO)
     print("compare ok", flush=True)
 
     # serving: queries batched with others and alone give the oracle's scores

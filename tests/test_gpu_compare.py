@@ -173,3 +173,26 @@ def test_compare_pruned_multi_chunk(cmp_engine):
     got = cmp_engine.compare_to(upload_oracle(cmp_engine, q), upload_oracle(cmp_engine, t))
     assert np.array_equal(got, want), np.argwhere(got != want)[:10]
     assert len(set(want.reshape(-1).tolist())) > 10
+
+
+def test_compare_target_set_grows(cmp_engine):
+    # the MFMA kernel orders each target digest's filters by size code on
+    # first use (col_order_kernel); digests appended to the set later are
+    # ordered on their own and the earlier order is carried over
+    img = synth.disk_image(91, 40 * MIB)
+    frags = synth.fragments(92, img, 300, 8 * KIB)
+    th = oracle.threads_available()
+    t1 = oracle.digest_block(img[:24 * MIB], 16384, b"a", threads=th)
+    t2 = oracle.digest_block(img[24 * MIB:], 4096, b"b", threads=th)
+    q = [oracle.digest_stream(f, b"q%03d" % i) for i, f in enumerate(frags)]
+    gq = upload_oracle(cmp_engine, q)
+    gt = upload_oracle(cmp_engine, [t1])
+    got = cmp_engine.compare_to(gq, gt)
+    want = oracle.compare_to(q, [t1], threads=th)
+    assert np.array_equal(got, want), np.argwhere(got != want)[:10]
+    gt.add(t2.name, t2.orig_size, t2.bf_count, t2.last_count, t2.dd_block_size, t2.max_elem, t2.filters,
+           t2.elem_counts)
+    got = cmp_engine.compare_to(gq, gt)
+    want = oracle.compare_to(q, [t1, t2], threads=th)
+    assert np.array_equal(got, want), np.argwhere(got != want)[:10]
+    assert (want > 0).sum() > 100
