@@ -482,6 +482,22 @@ def main() -> int:
         forms[label] = {"kernel": kname, "kernel_ms": kms / max(kn, 1), "pair_sum_ms": sms / max(kn, 1),
                         "pairs_per_s": rate, "peak_pairs_per_s": peak, "frac_of_unit_peak": rate / peak}
     eng.compare_method(sdhash_amd.CMP_AUTO)
+    # fast mode (sdhash --fast): both sets folded to 64-byte filters, scored
+    # by the 16-word form of the VALU kernel (a quarter of the AND-popcounts)
+    qset.fold()
+    tset.fold()
+    eng.compare_to(qset, tset)
+    eng.sync()
+    eng.profiling(True)
+    eng.reset_kernel_times()
+    for _ in range(args.steps):
+        eng.compare_to(qset, tset)
+    eng.sync()
+    eng.profiling(False)
+    kms, kn = eng.kernel_time("cmp_fast_kernel")
+    forms["fast"] = {"kernel": "cmp_fast_kernel", "kernel_ms": kms / max(kn, 1),
+                     "pairs_per_s": pairs / (kms / max(kn, 1) / 1e3),
+                     "note": "64-byte folded filters: not the same scores as the 256-byte forms"}
     blk_ms, blk_n = k["block_digest_kernel"]
     blk_avg = blk_ms / max(blk_n, 1)
     digest_ms = blk_ms / args.steps

@@ -5,8 +5,12 @@
 //
 // Differences from the reference, all outside the hot path:
 //   - index_info (--index / --index-search) is accepted as a parameter but
-//     must be NULL; sampling (`sample` > 0) and fast mode are not supported
-//     and throw -1.
+//     must be NULL; sampling (`sample` > 0) is not supported and throws -1.
+//   - fast mode (`fast` = true) folds the set's device copy (sdbfx_set_fold),
+//     as sdbf::fast folds each item; the host sdbf objects keep their
+//     unfolded filters, so to_string after a fast comparison prints those.
+//     Once a set has been compared in fast mode it stays folded, as in the
+//     reference.
 //   - sdbf_set::bf_vector (the per-filter bloom_filter copies of
 //     vector_init, sdbf/sdbf_set.cc:409-420) is not materialised; the
 //     device-resident set replaces it.
@@ -133,6 +137,7 @@ private:
     char sep = '|';
     sdbfx_set *dev = nullptr;
     size_t dev_items = 0;
+    bool fast_ = false;        // compared in fast mode: the device copy is folded
 };
 
 std::ostream &operator<<(std::ostream &os, const sdbf &s);

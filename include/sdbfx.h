@@ -126,6 +126,23 @@ int sdbfx_set_to_text(sdbfx_ctx *ctx, const sdbfx_set *set, char *out, uint64_t 
  */
 int sdbfx_set_add_text(sdbfx_ctx *ctx, sdbfx_set *set, const char *text, uint64_t len, uint32_t *added);
 
+/*
+ * Fast mode: folds every digest of the set to 64-byte filters in place, as
+ * sdbf::fast does (sdbf/sdbf_class.cc:426-439, bloom_filter::fold(2) at
+ * sdbf/bloom_filter.cc:263-289): each filter's four 64-byte quarters are
+ * OR-ed into its first 64 bytes and its Hamming weight becomes their
+ * popcount.  Comparisons of folded sets then score as sdbf_max_score's fast
+ * branch (sdbf/sdbf_core.cc:422-437: CUTOFFS64, bf_bitcount_cut_64), which
+ * the sdhash --fast option selects (sdbf_set.cc:240-248, 318-333).  Folding
+ * again is a no-op; digests added later stay plain until the next call.  A
+ * set that mixes folded and plain digests, or a folded set against a plain
+ * one, is an error in comparisons.
+ */
+int sdbfx_set_fold(sdbfx_ctx *ctx, sdbfx_set *set);
+
+/* 1 when every digest of a non-empty set is folded, else 0. */
+int sdbfx_set_folded(const sdbfx_set *set);
+
 /* ---- comparison ----------------------------------------------------- */
 
 /*

@@ -43,6 +43,9 @@ def lib():
         L.orc_hamming.argtypes = [P, C.c_uint32, P]
         L.orc_bitcount_cut_256.argtypes = [P, P, C.c_uint32, C.c_int32]
         L.orc_bitcount_cut_256.restype = C.c_uint32
+        L.orc_bitcount_cut_64.argtypes = [P, P, C.c_uint32, C.c_int32]
+        L.orc_bitcount_cut_64.restype = C.c_uint32
+        L.orc_fold_fast.argtypes = [P, C.c_uint32, P]
         L.orc_max_score.argtypes = [P, C.c_uint32, P]
         L.orc_max_score.restype = C.c_double
         L.orc_score.argtypes = [P, P]
@@ -117,10 +120,14 @@ def bitcount_cut(a, b, cut_off: int, slack: int) -> int:
     return int(lib().orc_bitcount_cut_256(_ptr(_u8(a)), _ptr(_u8(b)), cut_off, slack))
 
 
+def bitcount_cut_64(a, b, cut_off: int, slack: int) -> int:
+    return int(lib().orc_bitcount_cut_64(_ptr(_u8(a)), _ptr(_u8(b)), cut_off, slack))
+
+
 class _CDigest(C.Structure):
     _fields_ = [("bf", C.c_void_p), ("elem_counts", C.c_void_p), ("hamming", C.c_void_p),
                 ("name", C.c_char_p), ("bf_count", C.c_uint32), ("last_count", C.c_uint32),
-                ("max_elem", C.c_uint32)]
+                ("max_elem", C.c_uint32), ("fast", C.c_uint32)]
 
 
 @dataclass
@@ -134,6 +141,7 @@ class Digest:
     orig_size: int = 0
     dd_block_size: int = 0
     hamming: np.ndarray = field(default=None)
+    fast: bool = False             # filters folded to 64 bytes (sdbf::fast)
     _c: _CDigest | None = field(default=None, repr=False)
 
     @property
@@ -143,15 +151,26 @@ class Digest:
     def cstruct(self) -> _CDigest:
         if self._c is None:
             self.filters = np.ascontiguousarray(self.filters, dtype=np.uint8)
-            self.hamming = np.zeros(max(self.bf_count, 1), dtype=np.uint16)
-            lib().orc_hamming(_ptr(self.filters), self.bf_count, _ptr(self.hamming))
+            if not self.fast:
+                self.hamming = np.zeros(max(self.bf_count, 1), dtype=np.uint16)
+                lib().orc_hamming(_ptr(self.filters), self.bf_count, _ptr(self.hamming))
             ec = None
             if self.elem_counts is not None:
                 self.elem_counts = np.ascontiguousarray(self.elem_counts, dtype=np.uint16)
                 ec = self.elem_counts.ctypes.data
             self._c = _CDigest(self.filters.ctypes.data, ec, self.hamming.ctypes.data,
-                               self.name, self.bf_count, self.last_count, self.max_elem)
+                               self.name, self.bf_count, self.last_count, self.max_elem, int(self.fast))
         return self._c
+
+    def folded(self) -> "Digest":
+        """A copy in fast mode (sdbf::fast): every filter folded to 64 bytes
+        in place, Hamming weights of the folded bytes."""
+        f = np.array(self.filters, dtype=np.uint8, copy=True)
+        ham = np.zeros(max(self.bf_count, 1), dtype=np.uint16)
+        lib().orc_fold_fast(_ptr(f), self.bf_count, _ptr(ham))
+        ec = None if self.elem_counts is None else np.array(self.elem_counts, dtype=np.uint16, copy=True)
+        return Digest(self.name, f, self.bf_count, self.last_count, ec, self.orig_size, self.dd_block_size,
+                      ham, True)
 
     def elem_count(self, i: int) -> int:
         return int(lib().orc_elem_count(C.byref(self.cstruct()), i))

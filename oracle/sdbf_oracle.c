@@ -21,6 +21,10 @@
  *                        OR of the selected features' bits) + hand checks of
  *                        the block threshold/allowance rule
  *   bitcount_cut         brute-force popcount
+ *   fast mode            fold = OR of the four 64-byte quarters (identity of
+ *                        the two halving folds); 64-byte counts by brute
+ *                        force; self-compare = 100 and disjoint = 0 closed
+ *                        forms; the CUTOFFS64 table fingerprint
  *   sdbf_score           closed forms (self-compare = 100, disjoint = 0,
  *                        all-sparse target = -1) + hand-computed pairs
  *   base64 / to_string   RFC 4648 vectors + format grammar
@@ -396,6 +400,7 @@ typedef struct {
     uint32_t bf_count;
     uint32_t last_count;        /* stream mode */
     uint32_t max_elem;          /* 160 stream, 192 dd */
+    uint32_t fast;              /* filters folded to 64 bytes (sdbf::fast) */
 } orc_digest;
 
 /* sdbf/sdbf_class.cc:389-401 */
@@ -442,7 +447,49 @@ uint32_t orc_bitcount_cut_256(const uint8_t *a, const uint8_t *b, uint32_t cut_o
     return result;
 }
 
-/* sdbf/sdbf_core.cc:396-455 (sdbf_max_score, 256-byte filters, not fast mode) */
+/* sdbf/bf_utils.cc:79-110 (bf_bitcount_cut_64_asm): AND-popcount of 64-byte
+ * (folded) filters with one early exit after 32 bytes */
+uint32_t orc_bitcount_cut_64(const uint8_t *a, const uint8_t *b, uint32_t cut_off, int32_t slack)
+{
+    uint64_t x[8], y[8];
+    memcpy(x, a, 64);
+    memcpy(y, b, 64);
+    uint32_t result = 0;
+    for (int w = 0; w < 4; w++)
+        result += (uint32_t)__builtin_popcountll(x[w] & y[w]);
+    if (cut_off > 0 && (2 * result + slack) < cut_off)
+        return 0;
+    for (int w = 4; w < 8; w++)
+        result += (uint32_t)__builtin_popcountll(x[w] & y[w]);
+    return result;
+}
+
+/* sdbf/sdbf_class.cc:426-439 (sdbf::fast) with bloom_filter::fold(2)
+ * (sdbf/bloom_filter.cc:263-289) and compute_hamming (:453-461): each
+ * filter's 32 64-bit words are folded twice (w[j] |= w[j + 16] for j < 16,
+ * then w[j] |= w[j + 8] for j < 8), the 64 folded bytes replace the filter's
+ * first 64 bytes (the other 192 stay), and ham[i] becomes their popcount. */
+void orc_fold_fast(uint8_t *bf, uint32_t bf_count, uint16_t *ham)
+{
+    for (uint32_t i = 0; i < bf_count; i++) {
+        uint64_t w[32];
+        memcpy(w, bf + (uint64_t)i * ORC_BF, ORC_BF);
+        uint64_t rsize = 32;
+        for (int t = 0; t < 2; t++) {
+            for (uint64_t j = 0; j < rsize / 2; j++)
+                w[j] |= w[j + rsize / 2];
+            rsize /= 2;
+        }
+        memcpy(bf + (uint64_t)i * ORC_BF, w, 64);
+        uint32_t c = 0;
+        for (int j = 0; j < 8; j++)
+            c += (uint32_t)__builtin_popcountll(w[j]);
+        ham[i] = (uint16_t)c;
+    }
+}
+
+/* sdbf/sdbf_core.cc:396-455 (sdbf_max_score): 256-byte filters, or the folded
+ * 64-byte ones when the reference digest is in fast mode (:422-437) */
 double orc_max_score(const orc_digest *ref, uint32_t ref_index, const orc_digest *tgt)
 {
     double score, max_score = -1;
@@ -459,11 +506,19 @@ double orc_max_score(const orc_digest *ref, uint32_t ref_index, const orc_digest
             continue;
         uint32_t e2 = tgt->hamming[i];
         max_est = (e1 < e2) ? e1 : e2;
-        uint32_t mn = 4096 / (s1 + s2);
-        cut_off = SDBFX_CUTOFFS256[mn];
-        match = orc_bitcount_cut_256(bf_1, bf_2, cut_off, slack);
-        if (match > 0)
-            match = orc_bitcount_cut_256(bf_1, bf_2, 0, 0);
+        if (!ref->fast) {
+            uint32_t mn = 4096 / (s1 + s2);
+            cut_off = SDBFX_CUTOFFS256[mn];
+            match = orc_bitcount_cut_256(bf_1, bf_2, cut_off, slack);
+            if (match > 0)
+                match = orc_bitcount_cut_256(bf_1, bf_2, 0, 0);
+        } else {
+            uint32_t mn = 1024 / (s1 + s2);
+            cut_off = SDBFX_CUTOFFS64[mn];
+            match = orc_bitcount_cut_64(bf_1, bf_2, cut_off, slack);
+            if (match > 0)
+                match = orc_bitcount_cut_64(bf_1, bf_2, 0, 0);
+        }
         score = (match <= cut_off) ? 0 : (double)(match - cut_off) / (max_est - cut_off);
         max_score = (score > max_score) ? score : max_score;
     }

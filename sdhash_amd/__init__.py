@@ -19,7 +19,8 @@ SYMBOLS = (
     "sdbfx_ctx_create", "sdbfx_ctx_destroy", "sdbfx_last_error", "sdbfx_sync",
     "sdbfx_set_create", "sdbfx_set_destroy", "sdbfx_set_size", "sdbfx_set_filter_count",
     "sdbfx_set_digest", "sdbfx_set_digest_device", "sdbfx_set_add", "sdbfx_set_info",
-    "sdbfx_set_read", "sdbfx_set_to_text", "sdbfx_set_add_text", "sdbfx_compare_all", "sdbfx_compare_to",
+    "sdbfx_set_read", "sdbfx_set_to_text", "sdbfx_set_add_text", "sdbfx_set_fold", "sdbfx_set_folded",
+    "sdbfx_compare_all", "sdbfx_compare_to",
     "sdbfx_device_alloc", "sdbfx_device_free", "sdbfx_upload", "sdbfx_set_profiling", "sdbfx_kernel_time",
     "sdbfx_reset_kernel_times", "sdbfx_set_compare_method",
     "sdbfx_serve_create", "sdbfx_serve_destroy", "sdbfx_serve_error", "sdbfx_serve_ref_size",
@@ -79,6 +80,8 @@ def load():
     L.sdbfx_set_compare_method.argtypes = [P, C.c_int]
     L.sdbfx_set_to_text.argtypes = [P, P, P, U64, C.POINTER(U64)]
     L.sdbfx_set_add_text.argtypes = [P, P, P, U64, C.POINTER(U32)]
+    L.sdbfx_set_fold.argtypes = [P, P]
+    L.sdbfx_set_folded.argtypes = [P]
     L.sdbfx_serve_create.argtypes = [C.c_int, P, U64, U32, C.POINTER(P)]
     L.sdbfx_serve_destroy.argtypes = [P]
     L.sdbfx_serve_destroy.restype = None
@@ -94,7 +97,7 @@ def load():
         getattr(L, name).restype = getattr(L, name).restype or I32
     for name in ("sdbfx_ctx_create", "sdbfx_sync", "sdbfx_set_create", "sdbfx_set_digest",
                  "sdbfx_set_digest_device", "sdbfx_set_add", "sdbfx_set_info", "sdbfx_set_read",
-                 "sdbfx_set_to_text", "sdbfx_set_add_text",
+                 "sdbfx_set_to_text", "sdbfx_set_add_text", "sdbfx_set_fold",
                  "sdbfx_compare_all", "sdbfx_compare_to", "sdbfx_device_alloc", "sdbfx_device_free",
                  "sdbfx_upload", "sdbfx_set_profiling", "sdbfx_kernel_time",
                  "sdbfx_reset_kernel_times", "sdbfx_set_compare_method", "sdbfx_serve_create",
@@ -188,6 +191,15 @@ class DigestSet:
         out = np.empty(max(n.value, 1), np.uint8)
         self.engine._check(lib.sdbfx_set_to_text(ctx, self.handle, _ptr(out), n.value, C.byref(n)))
         return out[: n.value].tobytes()
+
+    def fold(self) -> None:
+        """Fast mode (sdbf::fast): folds every filter to 64 bytes in place on
+        the device; folded sets compare with the fast cutoffs."""
+        self.engine._check(self.engine.lib.sdbfx_set_fold(self.engine.ctx, self.handle))
+
+    @property
+    def folded(self) -> bool:
+        return self.engine.lib.sdbfx_set_folded(self.handle) == 1
 
     def add_text(self, text: bytes) -> int:
         """Appends the digests of an sdbf text (sdbf_set's file reader: lines up

@@ -507,6 +507,9 @@ sdbfx_set *sdbf_set::device_set()
             check(ctx, sdbfx_set_add(ctx, dev, s->hashname, s->orig_file_size, s->bf_count, s->last_count, 0,
                                      s->max_elem, s->buffer, nullptr));
     }
+    // fast mode folds every item (sdbf/sdbf_set.cc:243-248), later ones too
+    if (fast_)
+        check(ctx, sdbfx_set_fold(ctx, dev));
     return dev;
 }
 
@@ -524,7 +527,7 @@ std::string sdbf_set::compare_all_quiet(int32_t threshold, int32_t thread_count,
 {
     (void)thread_count;
     if (fast)
-        throw -1;
+        fast_ = true;
     std::stringstream out;
     out.fill('0');
     const size_t n = items.size();
@@ -556,7 +559,7 @@ std::string sdbf_set::compare_to_quiet(sdbf_set *other, int32_t threshold, uint3
     (void)thread_count;
     (void)sample_size;
     if (fast)
-        throw -1;
+        fast_ = other->fast_ = true;   // both sets folded (sdbf/sdbf_set.cc:322-333)
     std::stringstream out;
     out.fill('0');
     const size_t nq = items.size(), nt = other->items.size();

@@ -69,6 +69,8 @@ struct sdbfx_set {
     // built on first use: per digest whether it is built, and the event
     // after the last build
     DevBuf cord, ckey;
+    // digests [0, nfolded) are folded to 64 bytes (sdbfx_set_fold)
+    uint32_t nfolded = 0;
     mutable std::vector<uint8_t> cbuilt;
     mutable hipEvent_t cev = nullptr;
     mutable std::mutex cmu;
@@ -475,10 +477,10 @@ struct PairRef {
 // included).
 bool use_mfma(const sdbfx_ctx* ctx, const std::vector<uint64_t>& inst, const std::vector<uint64_t>& tgt)
 {
-    // the MFMA kernel addresses a target digest's filters by a 32-bit
-    // 16-byte-chunk index
+    // the MFMA kernel lists a candidate's target filter (its index in the
+    // digest) in 25 bits
     for (uint64_t t : tgt)
-        if (t >= (1ull << 28))
+        if (t >= (1ull << 25))
             return false;
     if (ctx->cmp_method != SDBFX_CMP_AUTO)
         return ctx->cmp_method == SDBFX_CMP_MFMA;
@@ -500,6 +502,17 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
     if (npairs > 0xFFFFFFFFull)
         return fail(ctx, "too many digest pairs in one call");
     const sdbfx_set* S[2] = {A, all ? A : B};
+    // fast mode: every digest of both sets folded (sdbfx_set_fold), or none
+    bool fast = false;
+    for (int si = 0; si < (all ? 1 : 2); si++) {
+        const uint32_t n = (uint32_t)S[si]->names.size();
+        if (S[si]->nfolded != 0 && S[si]->nfolded != n)
+            return fail(ctx, "set mixes folded and plain digests");
+        const bool f = n && S[si]->nfolded == n;
+        if (si == 1 && f != fast)
+            return fail(ctx, "a folded set can only be compared with a folded set");
+        fast = f;
+    }
 
     // orientation of every pair, in output order
     std::vector<PairRef> ref(npairs), tgt(npairs);
@@ -547,7 +560,8 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
         ginst.push_back(ic);
         gtgt.push_back(T->bfc[q >= na ? q - na : q]);
     }
-    const bool mfma = use_mfma(ctx, ginst, gtgt);
+    // folded filters: the 64-byte form of the VALU kernel
+    const bool mfma = !fast && use_mfma(ctx, ginst, gtgt);
     const uint32_t rows = mfma ? SDBFX_MROWS : SDBFX_CMP_ROWS;
     // target filters per work item.  The MFMA kernel's fixed costs (row
     // staging, publishing maxima) are per work item and its pruning works
@@ -670,8 +684,12 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
                (unsigned long long*)ctx->gbest.p);
     } else if (wi) {
         const uint32_t grid = (uint32_t)std::min<uint64_t>(wi, 16384);
-        LAUNCH(ctx, cmp_kernel, grid, 64, va, vb, (const CmpGroup*)ctx->groups.p, (uint32_t)groups.size(),
-               (const CmpPair*)ctx->pairs.p, wi, (uint32_t*)ctx->partial.p);
+        if (fast)
+            LAUNCH(ctx, cmp_fast_kernel, grid, 64, va, vb, (const CmpGroup*)ctx->groups.p,
+                   (uint32_t)groups.size(), (const CmpPair*)ctx->pairs.p, wi, (uint32_t*)ctx->partial.p);
+        else
+            LAUNCH(ctx, cmp_kernel, grid, 64, va, vb, (const CmpGroup*)ctx->groups.p, (uint32_t)groups.size(),
+                   (const CmpPair*)ctx->pairs.p, wi, (uint32_t*)ctx->partial.p);
     }
     if (inst)
         LAUNCH(ctx, chunk_max_kernel, (uint32_t)((inst + 255) / 256), 256, (const CmpPair*)ctx->pairs.p,
@@ -1095,6 +1113,39 @@ int sdbfx_set_add_text(sdbfx_ctx* ctx, sdbfx_set* set, const char* text, uint64_
     if (added)
         *added = (uint32_t)hs.size();
     return 0;
+}
+
+int sdbfx_set_fold(sdbfx_ctx* ctx, sdbfx_set* set)
+{
+    if (!ctx || !set)
+        return fail(ctx, "null argument");
+    const uint32_t n = (uint32_t)set->names.size();
+    if (set->nfolded >= n)
+        return 0;
+    HIPC(ctx, hipSetDevice(ctx->device));
+    // the digests appended since the last fold (folding is idempotent, as
+    // each filter's last 192 bytes stay)
+    const uint64_t f0 = set->first[set->nfolded], nf = set->nfilt - f0;
+    {
+        std::lock_guard<std::mutex> lk(set->cmu);
+        if (set->cev)
+            HIPC(ctx, hipStreamWaitEvent(ctx->stream, set->cev, 0));
+        // Hamming weights change: column orders built before are void
+        set->cbuilt.resize(n, 0);
+        for (uint32_t d = set->nfolded; d < n; d++)
+            set->cbuilt[d] = 0;
+    }
+    if (nf)
+        LAUNCH(ctx, fold_kernel, (uint32_t)((nf + 255) / 256), 256, (uint32_t*)set->filt.p, (uint16_t*)set->hamf.p,
+               f0, nf);
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    set->nfolded = n;
+    return 0;
+}
+
+int sdbfx_set_folded(const sdbfx_set* set)
+{
+    return set && !set->names.empty() && set->nfolded == set->names.size() ? 1 : 0;
 }
 
 int sdbfx_compare_all(sdbfx_ctx* ctx, const sdbfx_set* set, int32_t* scores)

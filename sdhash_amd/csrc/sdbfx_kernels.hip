@@ -63,6 +63,7 @@ __device__ __forceinline__ uint32_t pix(uint32_t r) { return PIX(r); }
 
 __constant__ uint16_t c_ranks[SDBFX_RANKS_LEN] = {SDBFX_ENTR64_RANKS_INIT};
 __constant__ uint16_t c_cut[SDBFX_CUTOFFS_LEN] = {SDBFX_CUTOFFS256_INIT};
+__constant__ uint16_t c_cut64[SDBFX_CUTOFFS_LEN] = {SDBFX_CUTOFFS64_INIT};
 __constant__ uint32_t c_entropy[65] = {SDBFX_ENTROPY64_INIT};
 
 struct __align__(16) TileSmem {
@@ -1344,15 +1345,35 @@ struct SetView {
     const uint32_t* ckey;   // position -> element count | Hamming weight << 16
 };
 
-__global__ __launch_bounds__(64) void cmp_kernel(
-    SetView A, SetView B, const CmpGroup* __restrict__ groups, uint32_t ngroups,
-    const CmpPair* __restrict__ pairs, uint64_t nwork, uint32_t* __restrict__ partial)
+// FAST: filters folded to 64 bytes (sdbf::fast), scored as sdbf_max_score's
+// fast branch (sdbf/sdbf_core.cc:422-437): cutoff CUTOFFS64[1024 / (s1 + s2)]
+// and bf_bitcount_cut_64 (sdbf/bf_utils.cc:79-110), whose only early exit is
+// 2 x (count of the first 32 bytes) + 48 < cutoff.
+// popcount(x) + acc as one v_bcnt_u32_b32: keeps the per-word counts a chain
+// of accumulating bcnts (the compiler otherwise sums them as a tree of
+// v_add3, 31 more instructions per pair)
+#if defined(HIPEMU)
+__device__ __forceinline__ uint32_t bcnt_acc(uint32_t x, uint32_t acc) { return (uint32_t)__popc(x) + acc; }
+#else
+__device__ __forceinline__ uint32_t bcnt_acc(uint32_t x, uint32_t acc)
 {
-    __shared__ uint16_t cut[1025];
+    uint32_t r;
+    asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(acc));
+    return r;
+}
+#endif
+
+template <bool FAST>
+__device__ __forceinline__ void cmp_body(
+    SetView A, SetView B, const CmpGroup* __restrict__ groups, uint32_t ngroups,
+    const CmpPair* __restrict__ pairs, uint64_t nwork, uint32_t* __restrict__ partial, uint16_t* cut)
+{
+    constexpr int W = FAST ? 16 : 64;   // words of a filter compared
     const int lane = threadIdx.x;
     for (int s = lane; s < 1025; s += 64) {
-        const uint32_t idx = s >= 28 ? 4096u / (uint32_t)s : 0u;
-        cut[s] = c_cut[idx];
+        const uint32_t idx = FAST ? min(1024u / (uint32_t)max(s, 1), (uint32_t)SDBFX_CUTOFFS_LEN - 1)
+                                  : (s >= 28 ? 4096u / (uint32_t)s : 0u);
+        cut[s] = FAST ? c_cut64[idx] : c_cut[idx];
     }
     __syncthreads();
 
@@ -1374,7 +1395,7 @@ __global__ __launch_bounds__(64) void cmp_kernel(
         const bool valid = li < g.inst_count;
         const uint64_t inst = g.inst_begin + li;
 
-        uint32_t a[64];
+        uint32_t a[W];
         uint32_t s1 = 0, e1 = 0;
         uint32_t p = g.pair_begin;
         if (valid) {
@@ -1394,7 +1415,7 @@ __global__ __launch_bounds__(64) void cmp_kernel(
             e1 = R.hamf[rf];
             const uint4* src = (const uint4*)(R.filt + rf * SDBFX_BFW);
 #pragma unroll
-            for (int q = 0; q < 16; q++) {
+            for (int q = 0; q < W / 4; q++) {
                 const uint4 v = src[q];
                 a[4 * q] = v.x;
                 a[4 * q + 1] = v.y;
@@ -1403,7 +1424,7 @@ __global__ __launch_bounds__(64) void cmp_kernel(
             }
         } else {
 #pragma unroll
-            for (int q = 0; q < 64; q++)
+            for (int q = 0; q < W; q++)
                 a[q] = 0;
         }
 
@@ -1422,22 +1443,30 @@ __global__ __launch_bounds__(64) void cmp_kernel(
             uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
 #pragma unroll
             for (int q = 0; q < 8; q++)
-                c0 += __popc(a[q] & bj[q]);
+                c0 = bcnt_acc(a[q] & bj[q], c0);
 #pragma unroll
             for (int q = 8; q < 16; q++)
-                c1 += __popc(a[q] & bj[q]);
+                c1 = bcnt_acc(a[q] & bj[q], c1);
+            if (!FAST) {
 #pragma unroll
-            for (int q = 16; q < 32; q++)
-                c2 += __popc(a[q] & bj[q]);
+                for (int q = 16; q < W / 2; q++)
+                    c2 = bcnt_acc(a[q] & bj[q], c2);
 #pragma unroll
-            for (int q = 32; q < 64; q++)
-                c3 += __popc(a[q] & bj[q]);
-            const uint32_t p32 = c0, p64 = c0 + c1, p128 = p64 + c2, p256 = p128 + c3;
+                for (int q = W / 2; q < W; q++)
+                    c3 = bcnt_acc(a[q] & bj[q], c3);
+            }
             const uint32_t sum = s1 + s2;
-            const uint32_t cutoff = sum <= 1024 ? cut[sum] : 1250u;
-            const bool pass = (8 * p32 + 48 >= cutoff) && (4 * p64 + 48 >= cutoff) &&
-                              (2 * p128 + 48 >= cutoff);
-            const uint32_t match = pass ? p256 : 0u;
+            uint32_t cutoff, match;
+            if (FAST) {
+                cutoff = sum <= 1024 ? cut[sum] : (uint32_t)c_cut64[0];
+                match = 2 * c0 + 48 >= cutoff ? c0 + c1 : 0u;
+            } else {
+                const uint32_t p32 = c0, p64 = c0 + c1, p128 = p64 + c2, p256 = p128 + c3;
+                cutoff = sum <= 1024 ? cut[sum] : 1250u;
+                const bool pass = (8 * p32 + 48 >= cutoff) && (4 * p64 + 48 >= cutoff) &&
+                                  (2 * p128 + 48 >= cutoff);
+                match = pass ? p256 : 0u;
+            }
             uint32_t num = 0, den = 1;
             if (match > cutoff) {
                 num = match - cutoff;
@@ -1451,6 +1480,47 @@ __global__ __launch_bounds__(64) void cmp_kernel(
         if (valid)
             partial[g.partial_begin + (uint64_t)li * g.nch + c] = best_d ? ((best_n << 16) | best_d) : 0xFFFFFFFFu;
     }
+}
+
+__global__ __launch_bounds__(64) void cmp_kernel(
+    SetView A, SetView B, const CmpGroup* __restrict__ groups, uint32_t ngroups,
+    const CmpPair* __restrict__ pairs, uint64_t nwork, uint32_t* __restrict__ partial)
+{
+    __shared__ uint16_t cut[1025];
+    cmp_body<false>(A, B, groups, ngroups, pairs, nwork, partial, cut);
+}
+
+__global__ __launch_bounds__(64) void cmp_fast_kernel(
+    SetView A, SetView B, const CmpGroup* __restrict__ groups, uint32_t ngroups,
+    const CmpPair* __restrict__ pairs, uint64_t nwork, uint32_t* __restrict__ partial)
+{
+    __shared__ uint16_t cut[1025];
+    cmp_body<true>(A, B, groups, ngroups, pairs, nwork, partial, cut);
+}
+
+// sdbf::fast (sdbf/sdbf_class.cc:426-439) with bloom_filter::fold(2)
+// (sdbf/bloom_filter.cc:263-289): each filter's four 64-byte quarters OR-ed
+// into its first 64 bytes (the other 192 bytes stay, so folding again changes
+// nothing), and its Hamming weight set to the popcount of those 64 bytes.
+// One thread per filter.
+__global__ void fold_kernel(uint32_t* __restrict__ filt, uint16_t* __restrict__ hamf, uint64_t f0, uint64_t nf)
+{
+    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (t >= nf)
+        return;
+    uint4* w = (uint4*)(filt + (f0 + t) * SDBFX_BFW);
+    uint4 o[4];
+    uint32_t h = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint4 x = w[q], y = w[q + 4], z = w[q + 8], v = w[q + 12];
+        o[q] = make_uint4(x.x | y.x | z.x | v.x, x.y | y.y | z.y | v.y, x.z | y.z | z.z | v.z, x.w | y.w | z.w | v.w);
+        h += __popc(o[q].x) + __popc(o[q].y) + __popc(o[q].z) + __popc(o[q].w);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        w[q] = o[q];
+    hamf[f0 + t] = (uint16_t)h;
 }
 
 // ------------------------------------------------------- compare on MFMA
@@ -1529,16 +1599,21 @@ __device__ __forceinline__ uint32_t size_code(uint32_t s)
 __device__ __forceinline__ void prefix_counts(const uint4* ra, uint32_t sw, const uint4* __restrict__ fb,
                                               uint32_t& p32, uint32_t& p64, uint32_t& p128, uint32_t& p256)
 {
-    // chunks 0-1, 2-3, 4-7, 8-15: four 16-byte chunks of each side in
-    // registers at a time
+    // chunks 0-1, 2-3, 4-7, 8-15: the target filter's 16 loads issued
+    // together (one round trip), the reference side read from LDS four
+    // chunks at a time
     uint32_t c[4] = {0, 0, 0, 0};
+    uint4 yy[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++)
+        yy[k] = fb[k];
 #pragma unroll
     for (int g = 0; g < 4; g++) {
         uint4 x[4], y[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             x[k] = ra[(4 * g + k) ^ sw];
-            y[k] = fb[4 * g + k];
+            y[k] = yy[4 * g + k];
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -1578,7 +1653,7 @@ struct MfmaWaveSmem {
     uint32_t rinfo[SDBFX_MROWS];                    // size_code(s1) | e1 << 16
     const uint32_t* rptr[SDBFX_MROWS];              // reference filter of each row
     unsigned long long best[SDBFX_MROWS];
-    uint32_t clist[MF_CAND_CAP];    // row | column << 7
+    uint32_t clist[MF_CAND_CAP];    // row | (target filter index in its digest) << 7
     uint32_t bfix[SDBFX_MROWS];     // per row: floor(best fraction * 2^16), or its initial accumulator
 };
 
@@ -1825,6 +1900,8 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
         const uint32_t* __restrict__ tf = g.tgt_set ? B.filt : A.filt;
         const uint32_t* __restrict__ tord = g.tgt_set ? B.cord : A.cord;
         const uint32_t* __restrict__ tkey = g.tgt_set ? B.ckey : A.ckey;
+        const uint16_t* __restrict__ tec = (g.tgt_set ? B.ecf : A.ecf) + g.tgt_first;
+        const uint16_t* __restrict__ tham = (g.tgt_set ? B.hamf : A.hamf) + g.tgt_first;
         // the target digest's filters as 16-byte chunks: filter L of the
         // digest, chunk q at L * 16 + q
         const uint4* __restrict__ tbase = (const uint4*)(tf + g.tgt_first * SDBFX_BFW);
@@ -2047,7 +2124,6 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
                 own += (uint32_t)__popc(cbits[wi]);
             uint32_t ncand;
             const uint32_t first = wave_excl_sum(own, lane, &ncand);
-            const uint64_t nt = j0 + t0;
             for (uint32_t round = 0; round < ncand; round += MF_CAND_CAP) {
                 uint32_t pos = first;
                 if (pos < round + MF_CAND_CAP && pos + own > round) {
@@ -2059,8 +2135,10 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
                             x ^= 1u << bit;
                             if (pos >= round && pos < round + MF_CAND_CAP) {
                                 const uint32_t blk = 2 * wi + (bit < 16 ? 1 : 0), i = (bit < 16 ? 15u : 31u) - bit;
+                                // the column's filter: this lane's, block column blk % MF_B
+                                const uint32_t cb = bit < 16 ? cbt[(2 * wi + 1) % MF_B] : cbt[(2 * wi) % MF_B];
                                 const uint32_t row = 32 * (blk / MF_B) + 8 * (i >> 2) + 4 * h + (i & 3);
-                                clist[pos - round] = row | (32 * (blk % MF_B) + r32) << 7;
+                                clist[pos - round] = row | (cb >> 4) << 7;
                             }
                             pos++;
                         }
@@ -2072,14 +2150,13 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
                 for (uint32_t base = 0; base < nlist; base += 64) {
                     if (base + lane < nlist) {
                         const uint32_t e = clist[base + lane];
-                        const uint32_t row = e & 127u, col = e >> 7;
-                        const uint64_t p = nt + col;
-                        const uint32_t L = tord[p], key = tkey[p];
+                        const uint32_t row = e & 127u, L = e >> 7;
+                        const uint32_t s2 = tec[L], e2 = tham[L];
                         const uint32_t ri = rinfo[row];
-                        const uint32_t cutoff = (uint32_t)(-1 - (int)ncut[(ri & 0xFFFFu) + size_code(key & 0xFFFFu)]);
                         uint32_t p32, p64, p128, p256;
                         prefix_counts(atile + row * 16, row & 15, tbase + (uint64_t)L * 16, p32, p64, p128, p256);
-                        merge_pair(p32, p64, p128, p256, cutoff, ri >> 16, key >> 16, &best[row]);
+                        const uint32_t cutoff = (uint32_t)(-1 - (int)ncut[(ri & 0xFFFFu) + size_code(s2)]);
+                        merge_pair(p32, p64, p128, p256, cutoff, ri >> 16, e2, &best[row]);
                     }
                 }
                 wave_lds_sync();

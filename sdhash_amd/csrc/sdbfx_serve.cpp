@@ -75,6 +75,7 @@ struct sdbfx_server {
     sdbfx_set* own_sample = nullptr;     // owned copy when the sample is not the whole set
     uint64_t batch_bytes = 0;
     uint32_t batch_max = 0;
+    bool fast = false;         // the reference set is folded (sdbfx_set_fold)
     sdbfx_ctx* cd = nullptr;   // digest context
     sdbfx_ctx* cc = nullptr;   // compare context
 
@@ -133,6 +134,9 @@ struct sdbfx_server {
             if (rc == 0)
                 rc = sdbfx_set_digest(cd, b.set, n, bufs.data(), lens.data(), names.data(), dd.data(),
                                       b.status.data());
+            // a folded reference set (fast mode) is compared with folded queries
+            if (rc == 0 && fast)
+                rc = sdbfx_set_fold(cd, b.set);
             if (rc != 0) {
                 set_error(std::string("digest: ") + sdbfx_last_error(cd));
                 for (uint32_t i = 0; i < n; i++)
@@ -301,6 +305,7 @@ int sdbfx_serve_create(int device, const sdbfx_set* ref, uint64_t batch_bytes, u
     memset(s->tuned, 0, sizeof(s->tuned));
     s->ref = ref;
     s->nref = sdbfx_set_size(ref);
+    s->fast = sdbfx_set_folded(ref) == 1;
     s->batch_bytes = batch_bytes;
     s->batch_max = batch_max;
     if (sdbfx_ctx_create(device, 0, &s->cd) != 0 || sdbfx_ctx_create(device, 0, &s->cc) != 0) {
@@ -309,7 +314,9 @@ int sdbfx_serve_create(int device, const sdbfx_set* ref, uint64_t batch_bytes, u
         delete s;
         return -1;
     }
-    if (make_sample(s) != 0) {
+    // the sample copies folded filters of a folded reference: fold it too
+    // (folding is idempotent) so that it is compared in the same mode
+    if (make_sample(s) != 0 || (s->fast && s->own_sample && sdbfx_set_fold(s->cc, s->own_sample) != 0)) {
         if (s->own_sample)
             sdbfx_set_destroy(s->own_sample);
         sdbfx_ctx_destroy(s->cd);

@@ -8,7 +8,8 @@
 // Options: -r/--deep, -g/--gen-compare, -c/--compare, -t/--threshold N,
 // -b/--block-size KB, -p/--threads N (accepted; the device does the work),
 // -z/--segment-size MB, -o/--output BASE, --separator pipe|csv|tab,
-// --hash-name NAME (stdin), --validate, --verbose, --version, -h/--help.
+// --hash-name NAME (stdin), --fast, --validate, --verbose, --version,
+// -h/--help.
 // File policy as sdhash.cc:502-618: inputs below 16 MiB are stream-hashed
 // unless -b > 0; larger ones use 16 KiB blocks unless -b 0 (or -b N KiB),
 // cut into segments named "<file>.NNNNM" (sdhash_threads.cc:300-373).
@@ -32,7 +33,7 @@ namespace {
 const uint64_t KB = 1024, MB = 1024 * 1024;
 
 struct Opts {
-    bool deep = false, gen = false, cmp = false, validate = false, verbose = false;
+    bool deep = false, gen = false, cmp = false, validate = false, verbose = false, fast = false;
     int32_t threshold = 1;
     int32_t dd = -1;               // KiB; -1 = automatic
     uint64_t segment = 128 * MB;
@@ -54,6 +55,7 @@ void usage()
                  "  -o [ --output ] BASE    write BASE.sdbf / BASE.compare\n"
                  "  --separator S           pipe, csv or tab\n"
                  "  --hash-name NAME        name of the digest of stdin\n"
+                 "  --fast                  shrink sdbf filters for speedup\n"
                  "  --validate              parse SDBF files and report their contents\n"
                  "  --verbose               progress output\n";
 }
@@ -99,6 +101,8 @@ bool parse(int argc, char **argv, Opts &o)
             o.cmp = true;
         } else if (a == "--validate") {
             o.validate = true;
+        } else if (a == "--fast") {
+            o.fast = true;   // sdhash.cc:268-273
         } else if (a == "--verbose") {
             o.verbose = true;
         } else if (is("-t", "--threshold")) {
@@ -306,17 +310,17 @@ int main(int argc, char **argv)
                 sdbf_set *s1 = new sdbf_set(o.inputs[0].c_str());
                 s1->set_separator(sep);
                 if (!o.output.empty())
-                    write_file(s1->compare_all_quiet(o.threshold, 0, false), o.output + ".compare");
+                    write_file(s1->compare_all_quiet(o.threshold, 0, o.fast), o.output + ".compare");
                 else
-                    s1->compare_all(o.threshold, false);
+                    s1->compare_all(o.threshold, o.fast);
             } else if (o.inputs.size() == 2) {
                 sdbf_set *s1 = new sdbf_set(o.inputs[0].c_str());
                 sdbf_set *s2 = new sdbf_set(o.inputs[1].c_str());
                 s1->set_separator(sep);
                 if (!o.output.empty())
-                    write_file(s1->compare_to_quiet(s2, o.threshold, 0, 0, false), o.output + ".compare");
+                    write_file(s1->compare_to_quiet(s2, o.threshold, 0, 0, o.fast), o.output + ".compare");
                 else
-                    s1->compare_to(s2, o.threshold, 0, false);
+                    s1->compare_to(s2, o.threshold, 0, o.fast);
             } else {
                 std::cerr << "sdhash: ERROR: Comparison requires 1 or 2 arguments." << std::endl;
                 return -1;
@@ -413,9 +417,9 @@ int main(int argc, char **argv)
         if (o.gen) {
             set1->set_separator(sep);
             if (!o.output.empty())
-                write_file(set1->compare_all_quiet(o.threshold, 0, false), o.output + ".compare");
+                write_file(set1->compare_all_quiet(o.threshold, 0, o.fast), o.output + ".compare");
             else
-                set1->compare_all(o.threshold, false);
+                set1->compare_all(o.threshold, o.fast);
         } else if (!o.output.empty()) {
             write_file(set1->to_string(), o.output + ".sdbf");
         }

@@ -158,6 +158,31 @@ def main(lib):
            t2.elem_counts)
     assert np.array_equal(eng.compare_to(gq2, gt), oracle.compare_to(q2, t + [t2]))
     eng.compare_method(sdhash_amd.CMP_AUTO)
+    # fast mode (sdbf::fast): folded filters and Hamming weights, fast scores,
+    # folding twice changes nothing, mixed modes refused
+    fs = eng.new_set()
+    assert np.all(fs.digest(files, names, dd_block=modes) == 0)
+    fs.fold()
+    fs.fold()
+    assert fs.folded
+    fods = [d.folded() for d in ods]
+    for i, d in enumerate(fods):
+        f, ec, hm = fs.read(i)
+        assert np.array_equal(f, d.filters) and np.array_equal(hm, d.hamming[: d.bf_count]), i
+    assert np.array_equal(eng.compare_all(fs), oracle.compare_all(fods))
+    ft, fq = eng.new_set(), eng.new_set()
+    for d, g in ((t[0], ft), *((x, fq) for x in q)):
+        g.add(d.name, d.orig_size, d.bf_count, d.last_count, d.dd_block_size, d.max_elem, d.filters, d.elem_counts)
+    ft.fold()
+    try:
+        eng.compare_to(fq, ft)
+    except sdhash_amd.SdbfxError as e:
+        assert "folded" in str(e)
+    else:
+        raise AssertionError("folded and plain sets compared")
+    fq.fold()
+    assert np.array_equal(eng.compare_to(fq, ft), oracle.compare_to([x.folded() for x in q], [t[0].folded()]))
+    print("fast ok", flush=True)
     print("compare ok", flush=True)
 
     # serving: queries batched with others and alone give the oracle's scores

@@ -196,3 +196,36 @@ def test_compare_target_set_grows(cmp_engine):
     want = oracle.compare_to(q, [t1, t2], threads=th)
     assert np.array_equal(got, want), np.argwhere(got != want)[:10]
     assert (want > 0).sum() > 100
+
+
+def test_compare_fast_mode(cmp_engine):
+    # sdhash --fast: every filter folded to 64 bytes on the device; the folded
+    # filters, Hamming weights and all scores equal the oracle's sdbf::fast
+    files = synth.file_set(93, 16, 256 * KIB)
+    img = synth.disk_image(94, 6 * MIB)
+    names = [b"f%02d" % i for i in range(len(files))] + [b"img"]
+    modes = [16384 if i % 4 == 0 else 0 for i in range(len(files))] + [16384]
+    bufs = files + [img]
+    ods = [oracle.digest_block(b, m, n) if m else oracle.digest_stream(b, n) for b, n, m in zip(bufs, names, modes)]
+    s = cmp_engine.new_set()
+    assert np.all(s.digest(bufs, names, dd_block=modes) == 0)
+    s.fold()
+    fods = [d.folded() for d in ods]
+    for i, d in enumerate(fods):
+        f, ec, hm = s.read(i)
+        assert np.array_equal(f, d.filters) and np.array_equal(hm, d.hamming[: d.bf_count]), i
+    got = cmp_engine.compare_all(s)
+    want = oracle.compare_all(fods)
+    assert np.array_equal(got, want), np.flatnonzero(got != want)[:10]
+    assert (want > 0).sum() > 10
+    # fragments against the image, both folded
+    frags = synth.fragments(95, img, 200, 8 * KIB)
+    q = [oracle.digest_stream(f, b"q%03d" % i) for i, f in enumerate(frags)]
+    gq = upload_oracle(cmp_engine, q)
+    gt = upload_oracle(cmp_engine, [ods[-1]])
+    gq.fold()
+    gt.fold()
+    got = cmp_engine.compare_to(gq, gt)
+    want = oracle.compare_to([d.folded() for d in q], [fods[-1]])
+    assert np.array_equal(got, want), np.argwhere(got != want)[:10]
+    assert (want[:100] > 0).sum() > 10   # folding loses matches: 25 of 100 here

@@ -96,3 +96,27 @@ def test_serve_poll_and_unknown_ticket(engine):
             srv.wait(t)                          # released by poll
     finally:
         srv.close()
+
+
+def test_serve_fast_mode(engine):
+    # a folded reference set (sdhash --fast): the server folds each batch of
+    # queries and scores them with the fast cutoffs
+    img, files, ods, ref = reference(engine)
+    ref.fold()
+    fods = [d.folded() for d in ods]
+    qs, dd = queries(img, files)
+    want = []
+    for q, d in zip(qs, dd):
+        if q.size < 512:
+            want.append(None)
+            continue
+        od = oracle.digest_block(q, d, b"q") if d else oracle.digest_stream(q, b"q")
+        want.append(oracle.compare_to([od.folded()], fods)[0])
+    assert sum(1 for w in want if w is not None and (w > 0).any()) > 5
+    srv = sdhash_amd.Server(ref, batch_bytes=2 * MIB, batch_max=16)
+    try:
+        res = [srv.wait(t) for t in [srv.submit(q, b"q%03d" % i, d) for i, (q, d) in enumerate(zip(qs, dd))]]
+        assert srv.error() == ""
+    finally:
+        srv.close()
+    check(res, want)

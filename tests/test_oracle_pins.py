@@ -7,6 +7,7 @@ and brute force.  CPU only."""
 import base64
 import hashlib
 import os
+import re
 
 import numpy as np
 import pytest
@@ -365,3 +366,66 @@ def test_cutoff_table_monotone_in_size():
     cut = [table[4096 // s] for s in range(28, 1025)]
     assert all(a <= b for a, b in zip(cut, cut[1:]))
     assert max(table) == 1250 and cut[-1] <= 1250
+
+
+# ---- fast mode (sdbf::fast, sdbf/sdbf_class.cc:426-439)
+
+CUTOFFS64_FP = "06a3db5f5e2e7675"   # sdbf/sdbf_conf.cc:156-165
+KIB = 1024
+
+
+def test_cutoffs64_fingerprint():
+    path = os.path.join(ROOT, "sdhash_amd", "csrc", "sdbf_tables.h")
+    src = open(path).read()
+    body = src[src.index("#define SDBFX_CUTOFFS64_INIT"):]
+    body = body[: body.index("\n\n")]
+    c = [int(x) for x in re.findall(r"\b\d+\b", body.split("\\", 1)[1])]
+    assert len(c) == 149 and fingerprint(c) == CUTOFFS64_FP
+
+
+def test_fold_is_or_of_quarters():
+    rng = np.random.Generator(np.random.PCG64(71))
+    f = (rng.random(5 * 256) < 0.3).astype(np.uint8) * rng.integers(0, 256, 5 * 256, dtype=np.uint8)
+    d = oracle.Digest(b"x", f, 5, 100, None, 5000, 0)
+    g = d.folded()
+    a = f.reshape(5, 4, 64)
+    want = a[:, 0] | a[:, 1] | a[:, 2] | a[:, 3]
+    got = g.filters.reshape(5, 256)
+    assert np.array_equal(got[:, :64], want)
+    assert np.array_equal(got[:, 64:], f.reshape(5, 256)[:, 64:])
+    assert np.array_equal(g.hamming[:5], np.unpackbits(want, axis=1).sum(axis=1))
+    assert np.array_equal(d.filters, f)   # the original is left as it was
+
+
+def test_bitcount_cut_64_brute_force():
+    rng = np.random.Generator(np.random.PCG64(72))
+    for _ in range(200):
+        a = rng.integers(0, 256, 64, dtype=np.uint8) & rng.integers(0, 256, 64, dtype=np.uint8)
+        b = rng.integers(0, 256, 64, dtype=np.uint8)
+        full = int(np.unpackbits(a & b).sum())
+        half = int(np.unpackbits(a[:32] & b[:32]).sum())
+        assert oracle.bitcount_cut_64(a, b, 0, 0) == full
+        cut = int(rng.integers(0, 200))
+        want = 0 if cut > 0 and 2 * half + 48 < cut else full
+        assert oracle.bitcount_cut_64(a, b, cut, 48) == want
+
+
+def test_fast_scores_closed_forms():
+    # a digest against an identical copy: every folded filter matches itself
+    # fully (fraction 1) -> 100; against bit-disjoint filters -> 0
+    files = synth.file_set(73, 2, 200 * KIB)
+    d = oracle.digest_stream(files[0], b"a").folded()
+    e = oracle.Digest(b"b", d.filters.copy(), d.bf_count, d.last_count, None, d.orig_size, 0,
+                      d.hamming.copy(), True)
+    assert oracle.compare_all([d, e])[0] == 100
+    # disjoint: the same filters with their folded halves moved to bits that
+    # the other side never sets
+    lo = d.filters.reshape(-1, 256).copy()
+    hi = lo.copy()
+    lo[:, :64] &= 0x0F
+    hi[:, :64] &= 0xF0
+    x = oracle.Digest(b"x", lo.reshape(-1), d.bf_count, d.last_count, None, 1, 0,
+                      np.unpackbits(lo[:, :64], axis=1).sum(axis=1).astype(np.uint16), True)
+    y = oracle.Digest(b"y", hi.reshape(-1), d.bf_count, d.last_count, None, 1, 0,
+                      np.unpackbits(hi[:, :64], axis=1).sum(axis=1).astype(np.uint16), True)
+    assert oracle.compare_all([x, y])[0] == 0
