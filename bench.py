@@ -350,7 +350,7 @@ def allpairs(eng, args) -> dict:
         sc = eng.compare_all(s)
         wall = time.perf_counter() - t
         eng.profiling(False)
-        kms = sum(eng.kernel_time(k)[0] for k in (kname, "col_order_kernel", "inst_map_kernel", "chunk_max_kernel",
+        kms = sum(eng.kernel_time(k)[0] for k in (kname, "col_order_kernel", "cmp_prep_kernel", "chunk_max_kernel",
                                                    "pair_sum_kernel"))
         if ref is None:
             ref = sc
@@ -447,13 +447,13 @@ def main() -> int:
     k = {}
     for name in ("block_digest_kernel", "stream_tile_kernel", "stream_fixup_plan_kernel", "stream_fixup_par_kernel",
                  "stream_fixup_kernel", "stream_insert_kernel", "gather_filters_kernel", "cmp_kernel",
-                 "col_order_kernel", "inst_map_kernel", "cmp_mfma_kernel", "chunk_max_kernel", "pair_sum_kernel"):
+                 "col_order_kernel", "cmp_prep_kernel", "cmp_mfma_kernel", "chunk_max_kernel", "pair_sum_kernel"):
         ms, n = eng.kernel_time(name)
         if n:
             k[name] = (ms, n)
     nfilt_t = tset.filter_count()
     pairs = sum(qset.info(i).bf_count for i in range(len(qset))) * nfilt_t
-    cmp_ms = sum(k.get(n, (0.0, 0))[0] for n in ("cmp_kernel", "col_order_kernel", "inst_map_kernel",
+    cmp_ms = sum(k.get(n, (0.0, 0))[0] for n in ("cmp_kernel", "col_order_kernel", "cmp_prep_kernel",
                                                  "cmp_mfma_kernel", "chunk_max_kernel", "pair_sum_kernel")) / args.steps
 
     # the two comparison kernels on the same sets: same scores, rate of each

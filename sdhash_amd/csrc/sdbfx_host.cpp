@@ -633,49 +633,70 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
     const SetView va = view(S[0]), vb = view(S[1]);
     if (wi && mfma) {
         // the column order of every target digest not ordered yet (after
-        // any earlier build, possibly on another stream)
+        // any earlier build, possibly on another stream).  The sets stay
+        // locked until the builds are queued and their events recorded.
+        const int nsets = all ? 1 : 2;
+        std::unique_lock<std::mutex> lk0(S[0]->cmu, std::defer_lock), lk1(S[1]->cmu, std::defer_lock);
+        if (nsets == 2)
+            std::lock(lk0, lk1);
+        else
+            lk0.lock();
         std::vector<uint64_t> cfirst[2];
         std::vector<uint32_t> ccount[2];
-        for (int si = 0; si < (all ? 1 : 2); si++) {
+        int last = -1;   // the set whose build goes into the set-up launch
+        for (int si = 0; si < nsets; si++) {
             const sdbfx_set* T = S[si];
-            std::lock_guard<std::mutex> lk(T->cmu);
             if (T->cev)
                 HIPC(ctx, hipStreamWaitEvent(ctx->stream, T->cev, 0));
             T->cbuilt.resize(T->names.size(), 0);
-            std::vector<uint64_t>& dfirst = cfirst[si];
-            std::vector<uint32_t>& dcount = ccount[si];
             for (const CmpGroup& g : groups) {
-                const uint32_t d = (uint32_t)(g.tgt_set && !all ? 1 : 0);
-                if ((int)d != si)
+                if ((int)(g.tgt_set && !all ? 1 : 0) != si)
                     continue;
                 // the digest of this group: the one whose first filter it is
                 const uint32_t idx = (uint32_t)(std::lower_bound(T->first.begin(), T->first.end(), g.tgt_first) -
                                                 T->first.begin());
                 if (idx < T->first.size() && !T->cbuilt[idx]) {
                     T->cbuilt[idx] = 1;
-                    dfirst.push_back(g.tgt_first);
-                    dcount.push_back(g.tgt_count);
+                    cfirst[si].push_back(g.tgt_first);
+                    ccount[si].push_back(g.tgt_count);
                 }
             }
-            if (!dfirst.empty()) {
-                if ((rc = upload(ctx, ctx->cofirst, dfirst)) || (rc = upload(ctx, ctx->cocount, dcount)))
-                    return rc;
-                LAUNCH(ctx, col_order_kernel, (uint32_t)dfirst.size(), 1024, (const uint16_t*)T->ecf.p,
-                       (const uint16_t*)T->hamf.p, (const uint64_t*)ctx->cofirst.p,
-                       (const uint32_t*)ctx->cocount.p, (uint32_t*)T->cord.p, (uint32_t*)T->ckey.p);
-                if (!T->cev)
-                    HIPC(ctx, hipEventCreate(&T->cev));
-                HIPC(ctx, hipEventRecord(T->cev, ctx->stream));
-            }
+            if (!cfirst[si].empty())
+                last = si;
         }
-        // per-instance maxima shared by the work items for pruning (from
-        // 0), and the row data of every instance
+        auto record = [&](const sdbfx_set* T) {
+            if (!T->cev)
+                HIPC(ctx, hipEventCreate(&T->cev));
+            HIPC(ctx, hipEventRecord(T->cev, ctx->stream));
+            return 0;
+        };
+        for (int si = 0; si < nsets; si++) {
+            if (cfirst[si].empty() || si == last)
+                continue;
+            const sdbfx_set* T = S[si];
+            if ((rc = upload(ctx, ctx->cofirst, cfirst[si])) || (rc = upload(ctx, ctx->cocount, ccount[si])))
+                return rc;
+            LAUNCH(ctx, col_order_kernel, (uint32_t)cfirst[si].size(), 1024, (const uint16_t*)T->ecf.p,
+                   (const uint16_t*)T->hamf.p, (const uint64_t*)ctx->cofirst.p, (const uint32_t*)ctx->cocount.p,
+                   (uint32_t*)T->cord.p, (uint32_t*)T->ckey.p);
+            if ((rc = record(T)))
+                return rc;
+        }
+        // one launch: the last column-order build, and per instance its row
+        // data and its shared maximum for pruning cleared
         if ((rc = ensure(ctx, ctx->gbest, inst * 8 + 256)) || (rc = ensure(ctx, ctx->rmap, inst * 8 + 256)) ||
             (rc = ensure(ctx, ctx->rcode, inst * 4 + 256)))
             return rc;
-        HIPC(ctx, hipMemsetAsync(ctx->gbest.p, 0, inst * 8, ctx->stream));
-        LAUNCH(ctx, inst_map_kernel, (uint32_t)((inst + 255) / 256), 256, va, vb, (const CmpPair*)ctx->pairs.p,
-               (uint32_t)npairs, inst, (uint64_t*)ctx->rmap.p, (uint32_t*)ctx->rcode.p);
+        const sdbfx_set* T = S[last >= 0 ? last : 0];
+        const uint32_t nd = last >= 0 ? (uint32_t)cfirst[last].size() : 0;
+        if (nd && ((rc = upload(ctx, ctx->cofirst, cfirst[last])) || (rc = upload(ctx, ctx->cocount, ccount[last]))))
+            return rc;
+        LAUNCH(ctx, cmp_prep_kernel, nd + (uint32_t)((inst + 1023) / 1024), 1024, (const uint16_t*)T->ecf.p,
+               (const uint16_t*)T->hamf.p, (const uint64_t*)ctx->cofirst.p, (const uint32_t*)ctx->cocount.p, nd,
+               (uint32_t*)T->cord.p, (uint32_t*)T->ckey.p, va, vb, (const CmpPair*)ctx->pairs.p, (uint32_t)npairs,
+               inst, (uint64_t*)ctx->rmap.p, (uint32_t*)ctx->rcode.p, (unsigned long long*)ctx->gbest.p);
+        if (nd && (rc = record(T)))
+            return rc;
         // one wave per SIMD (the 128 x 128 tile holds 256 accumulators per
         // lane): a 4-wave workgroup per CU, a multiple of the 8 XCDs
         const uint32_t grid = (uint32_t)std::min<uint64_t>(((wi + 3) / 4 + 7) & ~7ull, 256);

@@ -1701,11 +1701,32 @@ __device__ uint32_t co_stride_inverse(uint32_t m)
     }
 }
 
-__global__ __launch_bounds__(1024) void col_order_kernel(const uint16_t* __restrict__ ecf,
-                                                         const uint16_t* __restrict__ hamf,
-                                                         const uint64_t* __restrict__ dfirst,
-                                                         const uint32_t* __restrict__ dcount,
-                                                         uint32_t* __restrict__ cord, uint32_t* __restrict__ ckey)
+// Adds 1 to ctr[key] for every lane with `on` and returns the lane's place
+// (the counter's value before its increment).  Most filters of a digest share
+// one element count, so the lanes with lane 0's key (or the first active
+// lane's) take one atomic together; the others take one each.
+__device__ __forceinline__ uint32_t lds_inc_lead(uint32_t* ctr, uint32_t key, bool on)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const unsigned long long act = __ballot(on);
+    if (!act)
+        return 0;
+    const int leader = __builtin_ctzll(act);
+    const uint32_t lkey = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
+    const unsigned long long same = __ballot(on && key == lkey);
+    uint32_t base = 0;
+    if ((int)lane == leader)
+        base = atomicAdd(&ctr[lkey], (uint32_t)__popcll(same));
+    base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+    if ((same >> lane) & 1)
+        return base + lanes_below(same);
+    return on ? atomicAdd(&ctr[key], 1u) : 0u;
+}
+
+__device__ __forceinline__ void col_order_body(const uint16_t* __restrict__ ecf, const uint16_t* __restrict__ hamf,
+                                               const uint64_t* __restrict__ dfirst,
+                                               const uint32_t* __restrict__ dcount, uint32_t* __restrict__ cord,
+                                               uint32_t* __restrict__ ckey, uint32_t d)
 {
     constexpr uint32_t kRare = 0x80000000u;   // bin start of a rare code
     constexpr uint32_t kPer = 8;              // filters per thread per step, loaded first
@@ -1713,8 +1734,8 @@ __global__ __launch_bounds__(1024) void col_order_kernel(const uint16_t* __restr
     __shared__ uint32_t hist[CO_BINS + 1];
     __shared__ uint32_t wsum[16], wsumr[16];
     __shared__ uint32_t kinv;
-    const uint64_t f0 = dfirst[blockIdx.x];
-    const uint32_t n = dcount[blockIdx.x];
+    const uint64_t f0 = dfirst[d];
+    const uint32_t n = dcount[d];
     const uint32_t t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const uint32_t m = n / 128;   // full tiles
     hist[t] = 0;
@@ -1732,8 +1753,7 @@ __global__ __launch_bounds__(1024) void col_order_kernel(const uint16_t* __restr
         }
 #pragma unroll
         for (uint32_t k = 0; k < kPer; k++)
-            if (i0 + k * 1024 + t < n)
-                atomicAdd(&hist[co_bin(e[k])], 1u);
+            (void)lds_inc_lead(hist, co_bin(e[k]), i0 + k * 1024 + t < n);
     }
     __syncthreads();
     // bin starts: rare codes share ranks 0 .. R-1 (hist[CO_BINS]), the others
@@ -1766,10 +1786,10 @@ __global__ __launch_bounds__(1024) void col_order_kernel(const uint16_t* __restr
 #pragma unroll
         for (uint32_t k = 0; k < kPer; k++) {
             const uint32_t i = i0 + k * 1024 + t;
+            const uint32_t bin = co_bin(e[k]);
+            // bins of rare codes keep kRare; the counters stay below 2^31
+            const uint32_t r = lds_inc_lead(hist, (hist[bin] & kRare) ? (uint32_t)CO_BINS : bin, i < n);
             if (i < n) {
-                const uint32_t bin = co_bin(e[k]);
-                // bins of rare codes keep kRare; the counters stay below 2^31
-                const uint32_t r = atomicAdd(&hist[(hist[bin] & kRare) ? (uint32_t)CO_BINS : bin], 1u);
                 const uint32_t p = r < m * 128 ? (uint32_t)((uint64_t)(r / 128) * ki % m) * 128 + r % 128 : r;
                 cord[f0 + p] = i;
                 ckey[f0 + p] = e[k] | hm[k] << 16;
@@ -1778,12 +1798,34 @@ __global__ __launch_bounds__(1024) void col_order_kernel(const uint16_t* __restr
     }
 }
 
-// One instance per thread: where its reference filter is, and its size code
-// and Hamming weight (the row data of cmp_mfma_kernel).
-__global__ void inst_map_kernel(SetView A, SetView B, const CmpPair* __restrict__ pairs, uint32_t npairs,
-                                uint64_t ninst, uint64_t* __restrict__ rmap, uint32_t* __restrict__ rcode)
+__global__ __launch_bounds__(1024) void col_order_kernel(const uint16_t* __restrict__ ecf,
+                                                         const uint16_t* __restrict__ hamf,
+                                                         const uint64_t* __restrict__ dfirst,
+                                                         const uint32_t* __restrict__ dcount,
+                                                         uint32_t* __restrict__ cord, uint32_t* __restrict__ ckey)
 {
-    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    col_order_body(ecf, hamf, dfirst, dcount, cord, ckey, blockIdx.x);
+}
+
+// The set-up of cmp_mfma_kernel in one launch.  Workgroups [0, nd): the column
+// order of nd target digests (col_order_body).  The rest: one instance per
+// thread, its row data (where its reference filter is, its size code and
+// Hamming weight) and its shared maximum gbest cleared.
+__global__ __launch_bounds__(1024) void cmp_prep_kernel(const uint16_t* __restrict__ ecf,
+                                                        const uint16_t* __restrict__ hamf,
+                                                        const uint64_t* __restrict__ dfirst,
+                                                        const uint32_t* __restrict__ dcount, uint32_t nd,
+                                                        uint32_t* __restrict__ cord, uint32_t* __restrict__ ckey,
+                                                        SetView A, SetView B, const CmpPair* __restrict__ pairs,
+                                                        uint32_t npairs, uint64_t ninst, uint64_t* __restrict__ rmap,
+                                                        uint32_t* __restrict__ rcode,
+                                                        unsigned long long* __restrict__ gbest)
+{
+    if (blockIdx.x < nd) {
+        col_order_body(ecf, hamf, dfirst, dcount, cord, ckey, blockIdx.x);
+        return;
+    }
+    const uint64_t t = (uint64_t)(blockIdx.x - nd) * blockDim.x + threadIdx.x;
     if (t >= ninst)
         return;
     uint32_t lo = 0, hi = npairs;
@@ -1799,6 +1841,7 @@ __global__ void inst_map_kernel(SetView A, SetView B, const CmpPair* __restrict_
     const uint64_t rf = P.ref_first + (t - P.inst_begin);
     rmap[t] = rf | (uint64_t)P.ref_set << 63;
     rcode[t] = size_code(R.ecf[rf]) | (uint32_t)R.hamf[rf] << 16;
+    gbest[t] = 0ull;
 }
 
 // Four waves per workgroup, one per SIMD, each taking work items on its own;
