@@ -2413,32 +2413,36 @@ __global__ void text_decode_kernel(const char* __restrict__ text, const TextPars
         return;
     const TextParse D = tp[find_digest(tp, nd, q)];
     const uint64_t lq = q - D.q_begin;
+    // the quantum's 4 characters (block mode: quantum 0 of each filter is its
+    // ':xx:' prefix) at body_off + 4 lq, read as the one or two aligned words
+    // that hold them (the text buffer extends past the last line)
+    const uint64_t A = D.body_off + 4 * lq;
+    const uint32_t sh = (uint32_t)(A & 3);
+    const uint32_t* wp = (const uint32_t*)(text + (A - sh));
+    const uint32_t cw = sh ? (wp[0] >> (8 * sh) | wp[1] << (32 - 8 * sh)) : wp[0];
+    const uint32_t ch0 = cw & 0xFFu, ch1 = cw >> 8 & 0xFFu, ch2 = cw >> 16 & 0xFFu, ch3 = cw >> 24;
     uint64_t ob;   // output byte within the digest
     uint32_t nb;
-    const char* src;
     if (D.dd == 0) {
         const uint64_t total = (uint64_t)D.bf_count * SDBFX_BF;
         ob = 3 * lq;
         nb = (uint32_t)min((uint64_t)3, total - ob);
-        src = text + D.body_off + 4 * lq;
     } else {
         const uint64_t f = lq / 87;
         const uint32_t r = (uint32_t)(lq % 87);
-        const char* fs = text + D.body_off + f * 348;
         if (r == 0) {
-            const int h1 = hex_value((uint8_t)fs[1]), h0 = hex_value((uint8_t)fs[2]);
-            if (fs[0] != ':' || fs[3] != ':' || h1 < 0 || h0 < 0)
+            const int h1 = hex_value(ch1), h0 = hex_value(ch2);
+            if (ch0 != ':' || ch3 != ':' || h1 < 0 || h0 < 0)
                 atomicOr(err, 1u);
             ecf[D.filt_first + f] = (uint16_t)((h1 & 15) << 4 | (h0 & 15));
             return;
         }
         ob = f * SDBFX_BF + 3 * (r - 1);
         nb = min(3u, (uint32_t)SDBFX_BF - 3 * (r - 1));
-        src = fs + 4 + 4 * (r - 1);
     }
-    const int v0 = b64_value((uint8_t)src[0]), v1 = b64_value((uint8_t)src[1]);
-    const int v2 = nb > 1 ? b64_value((uint8_t)src[2]) : (src[2] == '=' ? 0 : -1);
-    const int v3 = nb > 2 ? b64_value((uint8_t)src[3]) : (src[3] == '=' ? 0 : -1);
+    const int v0 = b64_value(ch0), v1 = b64_value(ch1);
+    const int v2 = nb > 1 ? b64_value(ch2) : (ch2 == '=' ? 0 : -1);
+    const int v3 = nb > 2 ? b64_value(ch3) : (ch3 == '=' ? 0 : -1);
     if ((v0 | v1 | v2 | v3) < 0)
         atomicOr(err, 1u);
     const uint32_t v = (uint32_t)(v0 & 63) << 18 | (uint32_t)(v1 & 63) << 12 | (uint32_t)(v2 & 63) << 6 |
