@@ -13,7 +13,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_SO = os.path.join(_HERE, "liboracle.so")
+# tests/test_oracle_mutants.py points this at builds with planted mistakes
+_SO = os.environ.get("SDBF_ORACLE_SO") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 
@@ -61,6 +62,11 @@ def lib():
         L.orc_elem_count.argtypes = [P, C.c_uint64]
         L.orc_elem_count.restype = C.c_int32
         L.orc_threads_available.restype = C.c_int
+        L.orc_stream_hash_scored.argtypes = [P, P, C.c_uint64, C.c_uint64, C.POINTER(P), C.POINTER(C.c_uint32),
+                                             C.POINTER(C.c_uint32)]
+        L.orc_block_digest_scored.argtypes = [P, P, C.c_uint64, C.c_uint32, P, P]
+        L.orc_block_hash_scored.argtypes = [P, P, C.c_uint32, C.c_uint32, C.c_int32, P]
+        L.orc_block_hash_scored.restype = C.c_uint32
         _lib = L
     return _lib
 
@@ -231,6 +237,48 @@ def digest_stream_many(bufs, names=None, threads: int = 1) -> list[Digest]:
         nm = names[i] if names is not None else b"f%05d" % i
         out.append(Digest(nm, f, int(cnt[i]), int(last[i]), None, arrs[i].size, 0))
     return out
+
+
+def _scored_inputs(scores, words):
+    sc = np.ascontiguousarray(scores, dtype=np.uint16)
+    w = np.ascontiguousarray(words, dtype=np.uint32).reshape(-1)
+    assert w.size == 5 * sc.size
+    return sc, w
+
+
+def stream_hash_scored(scores, words, unit: int = 32 << 20):
+    """The stream path (unit loop, gen_chunk_hash, trim) over given
+    per-position scores and SHA-1 words (words: [n, 5]); returns
+    (filters uint8[bf_count*256], bf_count, last_count)."""
+    sc, w = _scored_inputs(scores, words)
+    p = C.c_void_p()
+    n = C.c_uint32()
+    last = C.c_uint32()
+    lib().orc_stream_hash_scored(_ptr(sc), _ptr(w), sc.size, unit, C.byref(p), C.byref(n), C.byref(last))
+    f = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint8)), shape=(n.value * 256,)).copy()
+    lib().orc_free(p)
+    return f, n.value, last.value
+
+
+def block_digest_scored(scores, words, dd: int):
+    """The block path (per-block histogram, threshold, gen_block_hash, tail
+    rule) over given per-position scores and SHA-1 words; returns
+    (filters uint8[count*256], elem_counts uint16[count])."""
+    sc, w = _scored_inputs(scores, words)
+    cnt = int(lib().orc_block_count(sc.size, dd))
+    f = np.zeros(max(cnt, 1) * 256, dtype=np.uint8)
+    ec = np.zeros(max(cnt, 1), dtype=np.uint16)
+    lib().orc_block_digest_scored(_ptr(sc), _ptr(w), sc.size, dd, _ptr(f), _ptr(ec))
+    return f[: cnt * 256], ec[:cnt]
+
+
+def block_hash_scored(scores, words, threshold: int, allowed: int):
+    """One gen_block_hash call over positions [0, len(scores) - 64) with the
+    given threshold and allowance; returns (filter uint8[256], hash_cnt)."""
+    sc, w = _scored_inputs(scores, words)
+    bf = np.zeros(256, dtype=np.uint8)
+    n = lib().orc_block_hash_scored(_ptr(sc), _ptr(w), sc.size, threshold, allowed, _ptr(bf))
+    return bf, int(n)
 
 
 def max_score(ref: Digest, i: int, tgt: Digest) -> float:

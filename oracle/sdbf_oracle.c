@@ -17,16 +17,28 @@
  *   scores state machine hand-traced known-answer vectors + closed forms on
  *                        monotone / constant / zero rank sequences
  *   bf_sha1_insert       set semantics (brute force) + FIPS-180 SHA-1 vectors
- *   stream/block hashing invariants (element counts, filter bit content =
- *                        OR of the selected features' bits) + hand checks of
- *                        the block threshold/allowance rule
+ *   stream hashing       hand-derived known answers over chosen scores and
+ *                        SHA-1 words (orc_stream_hash_scored): zero-new-bit
+ *                        skip, big-filter rejection keeping the filter bits,
+ *                        rollover at 160, restart at 8738, the per-unit
+ *                        restart count, the big filter kept across units,
+ *                        the trim
+ *   block hashing        hand-derived known answers (orc_block_digest_scored,
+ *                        orc_block_hash_scored): allowance spent only on
+ *                        counted features, the stop at 192, the threshold
+ *                        walk with its k = 15 exit, the tail rule
  *   bitcount_cut         brute-force popcount
  *   fast mode            fold = OR of the four 64-byte quarters (identity of
  *                        the two halving folds); 64-byte counts by brute
  *                        force; self-compare = 100 and disjoint = 0 closed
  *                        forms; the CUTOFFS64 table fingerprint
  *   sdbf_score           closed forms (self-compare = 100, disjoint = 0,
- *                        all-sparse target = -1) + hand-computed pairs
+ *                        all-sparse target = -1) + hand-computed pairs: the
+ *                        swap on filter count, on last element count OR name,
+ *                        the -1 sentinel's order dependence, the single-filter
+ *                        denominator, rounding half up
+ *   (tests/test_oracle_mutants.py plants one plausible mistake at a time in
+ *   a copy of this file; the known-answer tests must fail on each)
  *   base64 / to_string   RFC 4648 vectors + format grammar
  */
 #include <stdint.h>
@@ -199,6 +211,22 @@ void orc_sha1_words(const uint8_t *p, uint32_t *h)
 
 /* ---------------------------------------------------------------- stream */
 
+/* Where the SHA-1 words of the feature at a position come from: the input
+ * bytes (the digest path), or a table of five words per position (the
+ * known-answer tests, which choose each feature's bit positions by hand). */
+typedef struct {
+    const uint8_t *bytes;     /* input; the feature at i is bytes[i..i+63] */
+    const uint32_t *words;    /* or: words[5*i .. 5*i+4] */
+} feat_src;
+
+static void feat_words(const feat_src *fs, uint64_t i, uint32_t *h)
+{
+    if (fs->words)
+        memcpy(h, fs->words + 5 * i, 20);
+    else
+        orc_sha1_words(fs->bytes + i, h);
+}
+
 typedef struct {
     uint8_t *bf;
     uint32_t cap;        /* filters allocated */
@@ -207,8 +235,10 @@ typedef struct {
     uint8_t *big;        /* current big filter */
 } stream_state;
 
-/* sdbf/sdbf_core.cc:94-136 (gen_chunk_hash, info == NULL) */
-static void chunk_hash(stream_state *st, const uint8_t *file, uint64_t chunk_pos,
+/* sdbf/sdbf_core.cc:94-136 (gen_chunk_hash, info == NULL) over one unit:
+ * positions [0, chunk_size - 64) of the unit that starts at `pos` of the
+ * feature source */
+static void chunk_hash(stream_state *st, const feat_src *fs, uint64_t pos,
                        const uint16_t *scores, uint64_t chunk_size)
 {
     uint32_t h[5];
@@ -219,7 +249,7 @@ static void chunk_hash(stream_state *st, const uint8_t *file, uint64_t chunk_pos
         return;
     for (uint64_t i = 0; i < chunk_size - ORC_WIN; i++) {
         if (scores[i] > ORC_THRESHOLD) {
-            orc_sha1_words(file + chunk_pos + i, h);
+            feat_words(fs, pos + i, h);
             if (!orc_bf_sha1_insert(cur, h))
                 continue;
             if (!big_insert(st->big, h))
@@ -242,42 +272,49 @@ static void chunk_hash(stream_state *st, const uint8_t *file, uint64_t chunk_pos
 }
 
 /*
- * Stream-mode digest of one buffer: sdbf_class.cc:115-136 (ctor from a
- * buffer, dd_block_size 0) -> sdbf_core.cc:197-246 (gen_chunk_sdbf).
- * Returns 0, or -3 when len < 512.  *filters_out is malloc'ed and holds
- * *bf_count filters (after the final trim).
+ * sdbf/sdbf_core.cc:197-246 (gen_chunk_sdbf): units of `chunk` bytes plus
+ * the remainder, each scored (from the bytes' ranks, or taken from
+ * `given` for the unit's positions) and hashed in order; then the final
+ * trim.  The filter buffer holds ((len >> 11) + 1) filters, as the
+ * reference's estimate (:202-204), plus one.
  */
-int orc_digest_stream(const uint8_t *buf, uint64_t len, uint8_t **filters_out,
-                      uint32_t *bf_count, uint32_t *last_count)
+static void stream_digest_core(const feat_src *fs, const uint16_t *given, uint64_t len, uint64_t chunk,
+                               uint8_t **filters_out, uint32_t *bf_count, uint32_t *last_count)
 {
-    if (len < ORC_MIN_FILE)
-        return -3;
     uint64_t buff_size = ((len >> 11) + 1) << 8;
     if (buff_size < 256)
         buff_size = 256;
     stream_state st;
     st.cap = (uint32_t)(buff_size / ORC_BF);
+    /* real scores credit at most one position per step, so features (score
+     * > 16) are fewer than len / 17 and the estimate above holds; given
+     * scores may make every position a feature */
+    if (given)
+        st.cap = (uint32_t)(len / ORC_MAX_ELEM + 2);
     st.bf = (uint8_t *)calloc(st.cap + 1, ORC_BF);
     st.bf_count = 1;
     st.last_count = 0;
     st.big = (uint8_t *)calloc(1, ORC_BIG);
-    uint64_t chunk = ORC_CHUNK;
     uint64_t nbuf = len < chunk ? len : chunk;
-    uint16_t *ranks = (uint16_t *)malloc(nbuf * sizeof(uint16_t));
-    uint16_t *scores = (uint16_t *)malloc(nbuf * sizeof(uint16_t));
+    uint16_t *ranks = given ? NULL : (uint16_t *)malloc(nbuf * sizeof(uint16_t));
+    uint16_t *scores = given ? NULL : (uint16_t *)malloc(nbuf * sizeof(uint16_t));
     uint64_t qt = len / chunk, rem = len % chunk, pos = 0;
     int32_t histo[66];
     for (uint64_t i = 0; i < qt; i++, pos += chunk) {
-        orc_gen_ranks(buf + chunk * i, chunk, ranks);
-        memset(histo, 0, sizeof(histo));
-        orc_gen_scores(ranks, chunk, scores, histo);
+        if (!given) {
+            orc_gen_ranks(fs->bytes + chunk * i, chunk, ranks);
+            memset(histo, 0, sizeof(histo));
+            orc_gen_scores(ranks, chunk, scores, histo);
+        }
         /* the threshold loop at sdbf_core.cc:220-224 has no effect in stream mode */
-        chunk_hash(&st, buf, pos, scores, chunk);
+        chunk_hash(&st, fs, pos, given ? given + pos : scores, chunk);
     }
     if (rem > 0) {
-        orc_gen_ranks(buf + qt * chunk, rem, ranks);
-        orc_gen_scores(ranks, rem, scores, NULL);
-        chunk_hash(&st, buf, pos, scores, rem);
+        if (!given) {
+            orc_gen_ranks(fs->bytes + qt * chunk, rem, ranks);
+            orc_gen_scores(ranks, rem, scores, NULL);
+        }
+        chunk_hash(&st, fs, pos, given ? given + pos : scores, rem);
     }
     /* sdbf_core.cc:235-238 */
     if (st.bf_count > 1 && st.last_count < ORC_MAX_ELEM / 8) {
@@ -290,19 +327,48 @@ int orc_digest_stream(const uint8_t *buf, uint64_t len, uint8_t **filters_out,
     *filters_out = st.bf;
     *bf_count = st.bf_count;
     *last_count = st.last_count;
+}
+
+/*
+ * Stream-mode digest of one buffer: sdbf_class.cc:115-136 (ctor from a
+ * buffer, dd_block_size 0) -> sdbf_core.cc:197-246 (gen_chunk_sdbf).
+ * Returns 0, or -3 when len < 512.  *filters_out is malloc'ed and holds
+ * *bf_count filters (after the final trim).
+ */
+int orc_digest_stream(const uint8_t *buf, uint64_t len, uint8_t **filters_out,
+                      uint32_t *bf_count, uint32_t *last_count)
+{
+    if (len < ORC_MIN_FILE)
+        return -3;
+    feat_src fs = {buf, NULL};
+    stream_digest_core(&fs, NULL, len, ORC_CHUNK, filters_out, bf_count, last_count);
     return 0;
+}
+
+/*
+ * Known-answer entry of the stream path: the same unit loop, hashing and
+ * trim as orc_digest_stream, with the per-position scores and SHA-1 words
+ * given (scores[i] for every input position i, words[5*i..5*i+4]) and the
+ * unit size a parameter (the digest path uses 32 MiB).
+ */
+void orc_stream_hash_scored(const uint16_t *scores, const uint32_t *words, uint64_t len, uint64_t unit,
+                            uint8_t **filters_out, uint32_t *bf_count, uint32_t *last_count)
+{
+    feat_src fs = {NULL, words};
+    stream_digest_core(&fs, scores, len, unit, filters_out, bf_count, last_count);
 }
 
 /* ---------------------------------------------------------------- block */
 
-/* sdbf/sdbf_core.cc:142-191 (gen_block_hash, no index) */
-static uint32_t block_hash(const uint8_t *data, const uint16_t *scores, uint32_t max_offset,
+/* sdbf/sdbf_core.cc:142-191 (gen_block_hash, no index): the positions of
+ * one block start at `pos` of the feature source */
+static uint32_t block_hash(const feat_src *fs, uint64_t pos, const uint16_t *scores, uint32_t max_offset,
                            uint32_t threshold, int32_t allowed, uint8_t *bf)
 {
     uint32_t h[5], hash_cnt = 0;
     for (uint32_t i = 0; i < max_offset - ORC_WIN && hash_cnt < ORC_MAX_ELEM_DD; i++) {
         if (scores[i] > threshold || (scores[i] == threshold && allowed > 0)) {
-            orc_sha1_words(data + i, h);
+            feat_words(fs, pos + i, h);
             if (!orc_bf_sha1_insert(bf, h))
                 continue;
             hash_cnt++;
@@ -326,36 +392,25 @@ void orc_block_threshold(const int32_t *histo, uint32_t *k_out, int32_t *allowed
     *allowed_out = (int32_t)(ORC_MAX_ELEM_DD - sum);
 }
 
-/* one full dd-block: sdbf_core.cc:266-277 */
-static void one_block(const uint8_t *blk, uint32_t bs, uint16_t *ranks, uint16_t *scores,
-                      uint8_t *bf, uint16_t *ec)
-{
-    int32_t histo[66];
-    uint32_t k;
-    int32_t allowed;
-    orc_gen_ranks(blk, bs, ranks);
-    memset(histo, 0, sizeof(histo));
-    orc_gen_scores(ranks, bs, scores, histo);
-    orc_block_threshold(histo, &k, &allowed);
-    *ec = (uint16_t)block_hash(blk, scores, bs, k, allowed, bf);
-}
-
-/*
- * Block-mode digest: sdbf_class.cc:124-133 -> sdbf_core.cc:288-324
- * (gen_block_sdbf_mt).  filters must hold orc_block_count() * 256 zeroed
- * bytes, elem_counts orc_block_count() entries.  Blocks are independent;
- * `threads` > 1 spreads them over OpenMP threads.
- */
 uint64_t orc_block_count(uint64_t len, uint32_t dd)
 {
     return len / dd + ((len % dd) >= ORC_MIN_FILE ? 1 : 0);
 }
 
-int orc_digest_block(const uint8_t *buf, uint64_t len, uint32_t dd, uint8_t *filters,
-                     uint16_t *elem_counts, int threads)
+/*
+ * Block-mode digest: sdbf_class.cc:124-133 -> sdbf_core.cc:288-324
+ * (gen_block_sdbf_mt, with thread_gen_block_sdbf at :252-282 per full
+ * block).  Each full block: ranks, scores with their histogram (or the
+ * given scores and their histogram over [0, dd - 64), as
+ * gen_chunk_scores counts it at :83-86), threshold and allowance, then
+ * gen_block_hash.  The tail (remainder >= 512 bytes, :309-319): threshold
+ * 16 and allowance 192.  filters holds orc_block_count() * 256 zeroed bytes,
+ * elem_counts orc_block_count() entries; blocks are independent, so
+ * `threads` > 1 spreads them over OpenMP threads.
+ */
+static void block_digest_core(const feat_src *fs, const uint16_t *given, uint64_t len, uint32_t dd,
+                              uint8_t *filters, uint16_t *elem_counts, int threads)
 {
-    if (len < ORC_MIN_FILE)
-        return -3;
     uint64_t qt = len / dd, rem = len % dd;
 #ifdef _OPENMP
     if (threads < 1)
@@ -368,25 +423,73 @@ int orc_digest_block(const uint8_t *buf, uint64_t len, uint32_t dd, uint8_t *fil
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 16)
 #endif
-        for (int64_t i = 0; i < (int64_t)qt; i++)
-            one_block(buf + (uint64_t)i * dd, dd, ranks, scores, filters + (uint64_t)i * ORC_BF,
-                      elem_counts + i);
+        for (int64_t i = 0; i < (int64_t)qt; i++) {
+            const uint64_t pos = (uint64_t)i * dd;
+            int32_t histo[66];
+            uint32_t k;
+            int32_t allowed;
+            const uint16_t *sc = scores;
+            memset(histo, 0, sizeof(histo));
+            if (given) {
+                sc = given + pos;
+                for (uint32_t j = 0; j < dd - ORC_WIN; j++)
+                    histo[sc[j]]++;
+            } else {
+                orc_gen_ranks(fs->bytes + pos, dd, ranks);
+                orc_gen_scores(ranks, dd, scores, histo);
+            }
+            orc_block_threshold(histo, &k, &allowed);
+            elem_counts[i] = (uint16_t)block_hash(fs, pos, sc, dd, k, allowed, filters + (uint64_t)i * ORC_BF);
+        }
         free(ranks);
         free(scores);
     }
     (void)threads;
     if (rem >= ORC_MIN_FILE) {
-        /* sdbf_core.cc:309-319: tail block, threshold 16, allowance max_elem (192) */
         uint16_t *ranks = (uint16_t *)malloc(dd * sizeof(uint16_t));
         uint16_t *scores = (uint16_t *)malloc(dd * sizeof(uint16_t));
-        orc_gen_ranks(buf + qt * dd, rem, ranks);
-        orc_gen_scores(ranks, rem, scores, NULL);
-        elem_counts[qt] = (uint16_t)block_hash(buf + qt * dd, scores, (uint32_t)rem, ORC_THRESHOLD,
-                                               ORC_MAX_ELEM_DD, filters + qt * ORC_BF);
+        const uint16_t *sc = scores;
+        if (given) {
+            sc = given + qt * dd;
+        } else {
+            orc_gen_ranks(fs->bytes + qt * dd, rem, ranks);
+            orc_gen_scores(ranks, rem, scores, NULL);
+        }
+        elem_counts[qt] = (uint16_t)block_hash(fs, qt * dd, sc, (uint32_t)rem, ORC_THRESHOLD, ORC_MAX_ELEM_DD,
+                                               filters + qt * ORC_BF);
         free(ranks);
         free(scores);
     }
+}
+
+int orc_digest_block(const uint8_t *buf, uint64_t len, uint32_t dd, uint8_t *filters,
+                     uint16_t *elem_counts, int threads)
+{
+    if (len < ORC_MIN_FILE)
+        return -3;
+    feat_src fs = {buf, NULL};
+    block_digest_core(&fs, NULL, len, dd, filters, elem_counts, threads);
     return 0;
+}
+
+/*
+ * Known-answer entries of the block path, with per-position scores (each
+ * <= 65) and SHA-1 words given: the whole digest of `len` positions
+ * (block_digest_core as above), and one gen_block_hash call with its
+ * threshold and allowance chosen by the caller (returns hash_cnt).
+ */
+void orc_block_digest_scored(const uint16_t *scores, const uint32_t *words, uint64_t len, uint32_t dd,
+                             uint8_t *filters, uint16_t *elem_counts)
+{
+    feat_src fs = {NULL, words};
+    block_digest_core(&fs, scores, len, dd, filters, elem_counts, 1);
+}
+
+uint32_t orc_block_hash_scored(const uint16_t *scores, const uint32_t *words, uint32_t max_offset,
+                               uint32_t threshold, int32_t allowed, uint8_t *bf)
+{
+    feat_src fs = {NULL, words};
+    return block_hash(&fs, 0, scores, max_offset, threshold, allowed, bf);
 }
 
 /* ------------------------------------------------------------- compare */
