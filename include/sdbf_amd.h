@@ -1,19 +1,25 @@
 // sdbf_amd.h -- C++ classes with the interface of the reference's libsdbf
-// (sdbf/sdbf_class.h:23-121, sdbf/sdbf_set.h:15-96), implemented on the
-// gfx950 engine behind include/sdbfx.h.  Callers written against the
-// reference (sdhash-src, sdhash-server) recompile against this header.
+// (sdbf/sdbf_class.h:23-121, sdbf/sdbf_set.h:15-96, sdbf/bloom_filter.h:
+// 16-101), implemented on the gfx950 engine behind include/sdbfx.h.  Callers
+// written against the reference (sdhash-src, sdhash-server) recompile
+// against this header (sdbf_defines.h includes it under the reference's
+// header name).
 //
 // Differences from the reference, all outside the hot path:
-//   - index_info (--index / --index-search) is accepted as a parameter but
-//     must be NULL; sampling (`sample` > 0) is not supported and throws -1.
-//   - fast mode (`fast` = true) folds the set's device copy (sdbfx_set_fold),
-//     as sdbf::fast folds each item; the host sdbf objects keep their
-//     unfolded filters, so to_string after a fast comparison prints those.
-//     Once a set has been compared in fast mode it stays folded, as in the
-//     reference.
-//   - sdbf_set::bf_vector (the per-filter bloom_filter copies of
-//     vector_init, sdbf/sdbf_set.cc:409-420) is not materialised; the
-//     device-resident set replaces it.
+//   - index_info (--index / --index-search): the constructors take one
+//     without an index (index NULL, no set list), as the reference CLI
+//     passes it, and throw -1 for one that asks for indexing.
+//   - sampling (compare(other, sample), compare_to(..., sample_size, ...))
+//     gives the reference's scores: a calling digest with more than `sample`
+//     filters is scored on its first `sample` filters; the reference's
+//     random offsets are unreachable (sdbf/sdbf_core.cc:342-374).
+//   - sdbf::big_filters holds no filters: the 16 KiB de-duplication filters
+//     of stream digesting live in device memory for the duration of one
+//     digest only (the reference keeps them for --large output, which needs
+//     protobuf).
+//   - sdbf_set::bf_vector is filled by vector_init() with a host copy of
+//     every filter (sdbf/sdbf_set.cc:409-420); the comparisons themselves
+//     use the device-resident copy of the set.
 #ifndef SDBF_AMD_H
 #define SDBF_AMD_H
 
@@ -26,7 +32,63 @@
 
 #include "sdbfx.h"
 
-struct index_info;   // reference --index support: accepted only as NULL
+class bloom_filter;
+class sdbf_set;
+
+// --index / --index-search state (sdbf/index_info.h:8-15).  The reference
+// CLI passes one to every constructor; with no index (index == NULL and no
+// set list to search) it changes nothing.
+struct index_info {
+    bloom_filter *index;
+    std::vector<bloom_filter *> *indexlist;
+    std::vector<sdbf_set *> *setlist;
+    bool search_deep;
+    bool search_first;
+    bool basename;
+};
+
+// A Bloom filter (sdbf/bloom_filter.h:16-101): the reference's container for
+// sdbf_set::index and sdbf_set::bf_vector entries, with its public fields.
+// Host-side; the digest and comparison kernels hold their filters on the
+// device.
+class bloom_filter {
+public:
+    // empty filter of `size` bytes (a power of 2, >= 64; otherwise throws -1)
+    bloom_filter(uint64_t size, uint16_t hash_count, uint64_t max_elem, double max_fp);
+    // copy of a 256-byte sdbf filter (sdbf/bloom_filter.cc:125-144)
+    bloom_filter(uint8_t *data, uint64_t size, int id, int bf_elem_ct, uint16_t hamming);
+    ~bloom_filter();
+    bool insert_sha1(uint32_t *sha1);
+    bool query_sha1(uint32_t *sha1);
+    uint64_t elem_count();
+    double est_fp_rate();
+    double bits_per_elem();
+    std::string name() const;
+    void set_name(std::string name);
+    void fold(uint32_t times);
+    int add(bloom_filter *other);
+    int bloom_id();
+    void set_bloom_id(int id);
+    void compute_hamming();
+
+    static const uint32_t BIT_MASKS_32[];
+    static const uint32_t BITS[];
+
+    uint8_t *bf;
+    uint16_t hamming;
+    uint32_t hamminglg;
+    uint64_t bf_size;
+    uint64_t bit_mask;
+    uint64_t max_elem;
+    uint16_t hash_count;
+
+private:
+    bool query_and_set(uint32_t *sha1, bool mode_set);
+    double max_fp = 0;
+    uint64_t bf_elem_count = 0;
+    std::string setname;
+    int bl_id = 0;
+};
 
 // Engine-wide configuration, the counterpart of sdbf_conf
 // (sdbf/sdbf_conf.h:16-58): the constants the path uses and the device
@@ -71,20 +133,35 @@ public:
     std::string get_index_results() const { return std::string(); }
     uint8_t *clone_filter(uint32_t position);
     uint32_t filter_count();
+    // fast mode (sdbf/sdbf_class.cc:426-439): folds every filter to 64 bytes
+    // in place; comparisons with this digest as the reference side then
+    // score on the folded filters
+    void fast();
 
     static sdbf_conf *config;
     static int32_t get_elem_count(sdbf *mine, uint64_t index);
 
     // Digests many buffers in one device pass (the batch form of the
     // buffer constructor).  Entries of `out` are NULL for inputs that are
-    // too small (the constructor's -3).
+    // too small (the constructor's -3).  orig_sizes (optional) gives each
+    // digest's input_size() when it differs from the digested length, as
+    // for the stream constructor, whose digest covers msize bytes of which
+    // it read fewer.
     static void hash_batch(const std::vector<std::string> &names, const std::vector<const uint8_t *> &bufs,
-                           const std::vector<uint64_t> &lens, uint32_t dd_block_size, std::vector<sdbf *> &out);
+                           const std::vector<uint64_t> &lens, uint32_t dd_block_size, std::vector<sdbf *> &out,
+                           const std::vector<uint64_t> *orig_sizes = nullptr);
 
 public:
     uint8_t *buffer = nullptr;     // bf_count * 256 filter bytes
     uint16_t *hamming = nullptr;   // Hamming weight of every filter
     uint32_t max_elem = 0;
+    std::vector<bloom_filter *> *big_filters = new std::vector<bloom_filter *>();   // always empty (see above)
+
+    // fields the reference keeps private, for the engine's own code
+    uint32_t last_count_() const { return last_count; }
+    uint32_t dd_block_size_() const { return dd_block_size; }
+    const uint16_t *elem_counts_() const { return elem_counts; }
+    bool fast_mode() const { return fastmode; }
 
 private:
     sdbf() = default;
@@ -100,6 +177,7 @@ private:
     uint16_t *elem_counts = nullptr;
     uint32_t dd_block_size = 0;
     uint64_t orig_file_size = 0;
+    bool fastmode = false;
 };
 
 class sdbf_set {
@@ -108,6 +186,7 @@ class sdbf_set {
 
 public:
     sdbf_set();
+    sdbf_set(bloom_filter *index);  // empty set with an index (sdbf/sdbf_set.cc:40-46)
     sdbf_set(const char *fname);   // load a file of sdbf lines
     ~sdbf_set();
     sdbf *at(uint32_t pos);
@@ -127,7 +206,11 @@ public:
     std::string name() const;
     void set_name(std::string name);
     void set_separator(char sep);
-    void vector_init() {}
+    void vector_init();
+
+public:
+    bloom_filter *index = nullptr;
+    std::vector<bloom_filter *> *bf_vector = nullptr;
 
 private:
     sdbfx_set *device_set();   // the items uploaded to the device (cached)
@@ -137,7 +220,6 @@ private:
     char sep = '|';
     sdbfx_set *dev = nullptr;
     size_t dev_items = 0;
-    bool fast_ = false;        // compared in fast mode: the device copy is folded
 };
 
 std::ostream &operator<<(std::ostream &os, const sdbf &s);

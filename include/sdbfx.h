@@ -12,6 +12,8 @@
  *   -1  cannot process (bad arguments, device failure, out of memory)
  *   -2  invalid digest format
  *   -3  input too small (< 512 bytes)
+ *   -4  the call's scratch would exceed the context's memory budget
+ *       (SDBFX_EBUDGET); nothing was done
  *
  * The reference binds this path through C++ classes (sdbf, sdbf_set); each
  * entry point below names the reference function it replaces.
@@ -40,14 +42,25 @@ typedef struct {
 
 /* ---- context ------------------------------------------------------- */
 
+#define SDBFX_EBUDGET (-4)
+
 /* Opens device `device`; mem_budget (bytes, 0 = no limit) bounds the
- * scratch the context allocates.  Replaces the process-global
- * sdbf::config set-up (sdbf/sdbf_class.cc:25, sdhash-src/sdhash.cc:227). */
+ * scratch device memory the context holds (staging, work lists, partial
+ * results; not the digest sets, which the caller owns): a call that would
+ * grow it past the budget returns SDBFX_EBUDGET and changes nothing.
+ * Scratch is kept between calls (sdbfx_ctx_trim frees it).  Replaces the
+ * process-global sdbf::config set-up (sdbf/sdbf_class.cc:25,
+ * sdhash-src/sdhash.cc:227); the reference's only bound is exit(-1) when an
+ * allocation fails (sdbf/error.cc:33-38). */
 int sdbfx_ctx_create(int device, uint64_t mem_budget, sdbfx_ctx **out);
 void sdbfx_ctx_destroy(sdbfx_ctx *ctx);
 const char *sdbfx_last_error(const sdbfx_ctx *ctx);
 /* waits for all work of the context */
 int sdbfx_sync(sdbfx_ctx *ctx);
+/* scratch bytes the context holds now */
+uint64_t sdbfx_ctx_scratch_bytes(const sdbfx_ctx *ctx);
+/* frees all scratch of the context (after waiting for its work) */
+int sdbfx_ctx_trim(sdbfx_ctx *ctx);
 
 /* ---- digest sets ---------------------------------------------------- */
 
@@ -128,20 +141,34 @@ int sdbfx_set_add_text(sdbfx_ctx *ctx, sdbfx_set *set, const char *text, uint64_
 
 /*
  * Fast mode: folds every digest of the set to 64-byte filters in place, as
- * sdbf::fast does (sdbf/sdbf_class.cc:426-439, bloom_filter::fold(2) at
- * sdbf/bloom_filter.cc:263-289): each filter's four 64-byte quarters are
- * OR-ed into its first 64 bytes and its Hamming weight becomes their
- * popcount.  Comparisons of folded sets then score as sdbf_max_score's fast
- * branch (sdbf/sdbf_core.cc:422-437: CUTOFFS64, bf_bitcount_cut_64), which
- * the sdhash --fast option selects (sdbf_set.cc:240-248, 318-333).  Folding
- * again is a no-op; digests added later stay plain until the next call.  A
- * set that mixes folded and plain digests, or a folded set against a plain
- * one, is an error in comparisons.
+ * sdbf::fast does for each item (sdbf/sdbf_class.cc:426-439, bloom_filter::
+ * fold(2) at sdbf/bloom_filter.cc:263-289): each filter's four 64-byte
+ * quarters are OR-ed into its first 64 bytes (the other 192 bytes stay) and
+ * its Hamming weight becomes their popcount.  Folding again is a no-op;
+ * digests added later stay plain until folded.  Replaces the fast() loops of
+ * sdbf_set::compare_all (sdbf/sdbf_set.cc:243-248).
+ *
+ * A set may hold folded and plain digests.  As in sdbf_max_score
+ * (sdbf/sdbf_core.cc:422-437), each oriented pair is scored by its
+ * reference digest's mode: a folded reference digest on 64 bytes with
+ * CUTOFFS64 and bf_bitcount_cut_64, against the target's filters and Hamming
+ * weights as they are (folded or not); a plain reference digest on 256
+ * bytes, also against a folded target, where the reference's formula uses
+ * the target's folded Hamming weight (scores above 100 can result).  A pair
+ * whose score is infinite there (the denominator max_est - cut_off is 0) makes
+ * the comparison return -1, since the reference aborts on it
+ * (boost::math::round raises on infinity).
  */
 int sdbfx_set_fold(sdbfx_ctx *ctx, sdbfx_set *set);
 
+/* Folds digest idx only (sdbf::fast on one item, sdbf/sdbf_class.cc:426-439). */
+int sdbfx_set_fold_digest(sdbfx_ctx *ctx, sdbfx_set *set, uint32_t idx);
+
 /* 1 when every digest of a non-empty set is folded, else 0. */
 int sdbfx_set_folded(const sdbfx_set *set);
+
+/* 1 when digest idx is folded (sdbf::fastmode), else 0. */
+int sdbfx_set_digest_folded(const sdbfx_set *set, uint32_t idx);
 
 /* ---- comparison ----------------------------------------------------- */
 
@@ -159,6 +186,24 @@ int sdbfx_compare_all(sdbfx_ctx *ctx, const sdbfx_set *set, int32_t *scores);
  * sdbf_set::compare_to / compare_to_quiet (sdbf/sdbf_set.cc:318-394).
  */
 int sdbfx_compare_to(sdbfx_ctx *ctx, const sdbfx_set *q, const sdbfx_set *t, int32_t *scores);
+
+/* As sdbfx_compare_to for the query digests [q_first, q_first + q_count)
+ * only: scores[(i - q_first) * nt + j]. */
+int sdbfx_compare_to_range(sdbfx_ctx *ctx, const sdbfx_set *q, uint32_t q_first, uint32_t q_count,
+                           const sdbfx_set *t, int32_t *scores);
+
+/*
+ * As sdbfx_compare_to with sdbf_score's `sample` (sdhash -s N,
+ * sdbf/sdbf_core.cc:342-374, via sdbf_set::compare_to(other, t, sample, f)):
+ * a query digest with more than `sample` filters takes part with its first
+ * `sample` filters, and the swap rule sees that count and the element count
+ * of filter sample - 1.  The reference's random offsets (:370-372) are never
+ * drawn, because after the swap rule the reference side never has more than
+ * `sample` filters, so the result is deterministic.  sample 0 is
+ * sdbfx_compare_to.
+ */
+int sdbfx_compare_to_sampled(sdbfx_ctx *ctx, const sdbfx_set *q, const sdbfx_set *t, uint32_t sample,
+                             int32_t *scores);
 
 /*
  * Selects the comparison kernel for later sdbfx_compare_* calls of the
@@ -191,8 +236,12 @@ int sdbfx_set_compare_method(sdbfx_ctx *ctx, int method);
  * compareTwo path of sdhash-server (sdhash-srv-main.cc:224-402).
  */
 typedef struct sdbfx_server sdbfx_server;
+/* mem_budget (bytes, 0 = no limit) bounds the server's scratch device
+ * memory, half for digesting and half for comparing: a batch that would
+ * exceed it is split and its later queries deferred to the next batch; a
+ * query that does not fit on its own is refused (status SDBFX_EBUDGET). */
 int sdbfx_serve_create(int device, const sdbfx_set *ref, uint64_t batch_bytes, uint32_t batch_max,
-                       sdbfx_server **out);
+                       uint64_t mem_budget, sdbfx_server **out);
 /* Waits for the batches already past the queue; queued requests are dropped. */
 void sdbfx_serve_destroy(sdbfx_server *s);
 const char *sdbfx_serve_error(sdbfx_server *s);
@@ -201,9 +250,10 @@ uint32_t sdbfx_serve_ref_size(const sdbfx_server *s);
 int sdbfx_serve_submit(sdbfx_server *s, const uint8_t *buf, uint64_t len, const char *name, uint32_t dd_block,
                        uint64_t *ticket);
 /* *done = 1 when the result is ready; then scores (sdbfx_serve_ref_size
- * ints), status (0, or -3 for a query under 512 bytes, -1 on a device
- * error) and the submit-to-result latency are written and the ticket is
- * released.  Any output pointer may be NULL. */
+ * ints), status (0, -3 for a query under 512 bytes, SDBFX_EBUDGET for one
+ * refused by the memory budget, -1 on a device error) and the
+ * submit-to-result latency are written and the ticket is released.  Any
+ * output pointer may be NULL. */
 int sdbfx_serve_poll(sdbfx_server *s, uint64_t ticket, int32_t *scores, int32_t *status, uint64_t *latency_ns,
                      int *done);
 /* As poll, blocking until the result is ready. */
@@ -212,6 +262,8 @@ int sdbfx_serve_wait(sdbfx_server *s, uint64_t ticket, int32_t *scores, int32_t 
  * MFMA kernel.  The server picks the comparison kernel per batch size class
  * by timing both on the first batch of each class. */
 int sdbfx_serve_stats(sdbfx_server *s, uint64_t *batches, uint64_t *queries, uint64_t *mfma_batches);
+/* Queries deferred to a later batch and refused, by the memory budget. */
+int sdbfx_serve_admission(sdbfx_server *s, uint64_t *deferred, uint64_t *refused);
 
 /* ---- device memory for device-resident inputs ------------------------ */
 

@@ -53,6 +53,9 @@ def lib():
         L.orc_score.restype = C.c_int
         L.orc_compare_all.argtypes = [P, C.c_uint32, P, C.c_int]
         L.orc_compare_to.argtypes = [P, C.c_uint32, P, C.c_uint32, P, C.c_int]
+        L.orc_compare_to_sampled.argtypes = [P, C.c_uint32, P, C.c_uint32, C.c_uint32, P, C.c_int]
+        L.orc_score_sampled.argtypes = [P, P, C.c_uint32]
+        L.orc_score_sampled.restype = C.c_int
         L.orc_digest_stream_many.argtypes = [C.c_uint32, P, P, P, P, P, P, C.c_int]
         L.orc_free.argtypes = [P]
         L.orc_b64encode.argtypes = [P, C.c_uint64, P]
@@ -285,8 +288,10 @@ def max_score(ref: Digest, i: int, tgt: Digest) -> float:
     return float(lib().orc_max_score(C.byref(ref.cstruct()), i, C.byref(tgt.cstruct())))
 
 
-def score(a: Digest, b: Digest) -> int:
-    return int(lib().orc_score(C.byref(a.cstruct()), C.byref(b.cstruct())))
+def score(a: Digest, b: Digest, sample: int = 0) -> int:
+    """a.compare(b, sample) (sdbf_score); INT32_MIN where the reference aborts
+    on an infinite score."""
+    return int(lib().orc_score_sampled(C.byref(a.cstruct()), C.byref(b.cstruct()), sample))
 
 
 def _array(ds):
@@ -305,10 +310,10 @@ def compare_all(ds: list[Digest], threads: int = 1) -> np.ndarray:
     return out
 
 
-def compare_to(q: list[Digest], t: list[Digest], threads: int = 1) -> np.ndarray:
+def compare_to(q: list[Digest], t: list[Digest], threads: int = 1, sample: int = 0) -> np.ndarray:
     out = np.zeros((len(q), len(t)), dtype=np.int32)
     if len(q) and len(t):
-        lib().orc_compare_to(_array(q), len(q), _array(t), len(t), _ptr(out), threads)
+        lib().orc_compare_to_sampled(_array(q), len(q), _array(t), len(t), sample, _ptr(out), threads)
     return out
 
 

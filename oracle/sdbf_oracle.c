@@ -628,13 +628,21 @@ double orc_max_score(const orc_digest *ref, uint32_t ref_index, const orc_digest
     return max_score;
 }
 
-/* sdbf/sdbf_core.cc:331-389 (sdbf_score, sample 0) with the boost 1.49
- * round of external/boost/math/special_functions/round.hpp:20-26 */
-int orc_score(const orc_digest *a, const orc_digest *b)
+/* sdbf/sdbf_core.cc:331-389 (sdbf_score) with the boost 1.49 round of
+ * external/boost/math/special_functions/round.hpp:20-26.  With sample > 0 a
+ * calling digest of more than `sample` filters counts as `sample` filters in
+ * the swap rule (:342-359) and is scored on its first `sample` filters: the
+ * loop's random offset (:370-372) needs bf_count_1 > sample, which the code
+ * before it never leaves true, so ref_index is always i. */
+int orc_score_sampled(const orc_digest *a, const orc_digest *b, uint32_t sample)
 {
     const orc_digest *s1 = a, *s2 = b;
     double score_sum = -1;
-    uint32_t n1 = s1->bf_count;
+    uint32_t n1;
+    if (sample > 0 && s1->bf_count > sample)
+        n1 = sample;
+    else
+        n1 = s1->bf_count;
     if ((n1 > s2->bf_count) ||
         (n1 == s2->bf_count &&
          ((elem_count(s1, n1 - 1) > elem_count(s2, s2->bf_count - 1)) ||
@@ -644,9 +652,12 @@ int orc_score(const orc_digest *a, const orc_digest *b)
         s2 = t;
         n1 = s1->bf_count;
     }
+    uint32_t rand_offset = 1;
     int sparse = 0;
     for (uint32_t i = 0; i < n1; i++) {
-        double m = orc_max_score(s1, i, s2);
+        if (sample > 0 && n1 > sample)
+            abort();   /* unreachable: see above */
+        double m = orc_max_score(s1, i * rand_offset, s2);
         score_sum = (score_sum < 0) ? m : score_sum + m;
         if (elem_count(s1, i) < ORC_MIN_ELEM)
             sparse++;
@@ -659,8 +670,15 @@ int orc_score(const orc_digest *a, const orc_digest *b)
     if (score_sum < 0)
         return -1;
     double v = 100.0 * score_sum / (double)denom;
+    /* an infinite fraction (a plain reference filter against a folded
+     * target whose Hamming weight equals the cutoff: max_est - cut_off is 0)
+     * makes boost::math::round raise in the reference; reported as INT_MIN */
+    if (isinf(v))
+        return (int)0x80000000u;
     return (int)floor(v + 0.5);
 }
+
+int orc_score(const orc_digest *a, const orc_digest *b) { return orc_score_sampled(a, b, 0); }
 
 /*
  * All pairs i<j of a set, in set order (sdbf_set.cc:240-266): out[k] holds
@@ -681,7 +699,21 @@ void orc_compare_all(const orc_digest *set, uint32_t n, int32_t *out, int thread
     }
 }
 
-/* query x target (sdbf_set.cc:318-348): out[i*nt + j] = q[i].compare(t[j]) */
+/* query x target (sdbf_set.cc:318-348): out[i*nt + j] = q[i].compare(t[j], sample) */
+void orc_compare_to_sampled(const orc_digest *q, uint32_t nq, const orc_digest *t, uint32_t nt,
+                            uint32_t sample, int32_t *out, int threads)
+{
+    (void)threads;
+#ifdef _OPENMP
+    if (threads < 1)
+        threads = 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+#endif
+    for (int64_t i = 0; i < (int64_t)nq; i++)
+        for (uint32_t j = 0; j < nt; j++)
+            out[(uint64_t)i * nt + j] = orc_score_sampled(&q[i], &t[j], sample);
+}
+
 void orc_compare_to(const orc_digest *q, uint32_t nq, const orc_digest *t, uint32_t nt,
                     int32_t *out, int threads)
 {

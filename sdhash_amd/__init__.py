@@ -16,15 +16,17 @@ LIB_PATH = os.path.join(_HERE, "libsdbfx.so")
 
 # C ABI entry points (all declared in include/sdbfx.h)
 SYMBOLS = (
-    "sdbfx_ctx_create", "sdbfx_ctx_destroy", "sdbfx_last_error", "sdbfx_sync",
+    "sdbfx_ctx_create", "sdbfx_ctx_destroy", "sdbfx_last_error", "sdbfx_sync", "sdbfx_ctx_scratch_bytes",
+    "sdbfx_ctx_trim",
     "sdbfx_set_create", "sdbfx_set_destroy", "sdbfx_set_size", "sdbfx_set_filter_count",
     "sdbfx_set_digest", "sdbfx_set_digest_device", "sdbfx_set_add", "sdbfx_set_info",
     "sdbfx_set_read", "sdbfx_set_to_text", "sdbfx_set_add_text", "sdbfx_set_fold", "sdbfx_set_folded",
-    "sdbfx_compare_all", "sdbfx_compare_to",
+    "sdbfx_set_fold_digest", "sdbfx_set_digest_folded",
+    "sdbfx_compare_all", "sdbfx_compare_to", "sdbfx_compare_to_sampled", "sdbfx_compare_to_range",
     "sdbfx_device_alloc", "sdbfx_device_free", "sdbfx_upload", "sdbfx_set_profiling", "sdbfx_kernel_time",
     "sdbfx_reset_kernel_times", "sdbfx_set_compare_method",
     "sdbfx_serve_create", "sdbfx_serve_destroy", "sdbfx_serve_error", "sdbfx_serve_ref_size",
-    "sdbfx_serve_submit", "sdbfx_serve_poll", "sdbfx_serve_wait", "sdbfx_serve_stats",
+    "sdbfx_serve_submit", "sdbfx_serve_poll", "sdbfx_serve_wait", "sdbfx_serve_stats", "sdbfx_serve_admission",
 )
 
 # comparison kernels (sdbfx_set_compare_method)
@@ -34,7 +36,12 @@ _lib = None
 
 
 class SdbfxError(RuntimeError):
-    pass
+    def __init__(self, msg, status=-1):
+        super().__init__(msg)
+        self.status = status
+
+
+EBUDGET = -4   # a call's scratch would exceed the context's memory budget
 
 
 class _Info(C.Structure):
@@ -57,6 +64,9 @@ def load():
     L.sdbfx_last_error.argtypes = [P]
     L.sdbfx_last_error.restype = C.c_char_p
     L.sdbfx_sync.argtypes = [P]
+    L.sdbfx_ctx_scratch_bytes.argtypes = [P]
+    L.sdbfx_ctx_scratch_bytes.restype = U64
+    L.sdbfx_ctx_trim.argtypes = [P]
     L.sdbfx_set_create.argtypes = [P, C.POINTER(P)]
     L.sdbfx_set_destroy.argtypes = [P]
     L.sdbfx_set_destroy.restype = None
@@ -71,6 +81,8 @@ def load():
     L.sdbfx_set_read.argtypes = [P, P, U32, P, P, P]
     L.sdbfx_compare_all.argtypes = [P, P, P]
     L.sdbfx_compare_to.argtypes = [P, P, P, P]
+    L.sdbfx_compare_to_sampled.argtypes = [P, P, P, U32, P]
+    L.sdbfx_compare_to_range.argtypes = [P, P, U32, U32, P, P]
     L.sdbfx_device_alloc.argtypes = [P, U64, C.POINTER(P)]
     L.sdbfx_device_free.argtypes = [P, P]
     L.sdbfx_upload.argtypes = [P, P, P, U64]
@@ -82,7 +94,9 @@ def load():
     L.sdbfx_set_add_text.argtypes = [P, P, P, U64, C.POINTER(U32)]
     L.sdbfx_set_fold.argtypes = [P, P]
     L.sdbfx_set_folded.argtypes = [P]
-    L.sdbfx_serve_create.argtypes = [C.c_int, P, U64, U32, C.POINTER(P)]
+    L.sdbfx_set_fold_digest.argtypes = [P, P, U32]
+    L.sdbfx_set_digest_folded.argtypes = [P, U32]
+    L.sdbfx_serve_create.argtypes = [C.c_int, P, U64, U32, U64, C.POINTER(P)]
     L.sdbfx_serve_destroy.argtypes = [P]
     L.sdbfx_serve_destroy.restype = None
     L.sdbfx_serve_error.argtypes = [P]
@@ -93,15 +107,17 @@ def load():
     L.sdbfx_serve_poll.argtypes = [P, U64, P, C.POINTER(I32), C.POINTER(U64), C.POINTER(C.c_int)]
     L.sdbfx_serve_wait.argtypes = [P, U64, P, C.POINTER(I32), C.POINTER(U64)]
     L.sdbfx_serve_stats.argtypes = [P, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64)]
+    L.sdbfx_serve_admission.argtypes = [P, C.POINTER(U64), C.POINTER(U64)]
     for name in SYMBOLS:
         getattr(L, name).restype = getattr(L, name).restype or I32
-    for name in ("sdbfx_ctx_create", "sdbfx_sync", "sdbfx_set_create", "sdbfx_set_digest",
+    for name in ("sdbfx_ctx_create", "sdbfx_sync", "sdbfx_ctx_trim", "sdbfx_set_create", "sdbfx_set_digest",
                  "sdbfx_set_digest_device", "sdbfx_set_add", "sdbfx_set_info", "sdbfx_set_read",
-                 "sdbfx_set_to_text", "sdbfx_set_add_text", "sdbfx_set_fold",
-                 "sdbfx_compare_all", "sdbfx_compare_to", "sdbfx_device_alloc", "sdbfx_device_free",
+                 "sdbfx_set_to_text", "sdbfx_set_add_text", "sdbfx_set_fold", "sdbfx_set_fold_digest",
+                 "sdbfx_compare_all", "sdbfx_compare_to", "sdbfx_compare_to_sampled", "sdbfx_compare_to_range", "sdbfx_device_alloc", "sdbfx_device_free",
                  "sdbfx_upload", "sdbfx_set_profiling", "sdbfx_kernel_time",
                  "sdbfx_reset_kernel_times", "sdbfx_set_compare_method", "sdbfx_serve_create",
-                 "sdbfx_serve_submit", "sdbfx_serve_poll", "sdbfx_serve_wait", "sdbfx_serve_stats"):
+                 "sdbfx_serve_submit", "sdbfx_serve_poll", "sdbfx_serve_wait", "sdbfx_serve_stats",
+                 "sdbfx_serve_admission"):
         getattr(L, name).restype = I32
     _lib = L
     return L
@@ -197,9 +213,16 @@ class DigestSet:
         the device; folded sets compare with the fast cutoffs."""
         self.engine._check(self.engine.lib.sdbfx_set_fold(self.engine.ctx, self.handle))
 
+    def fold_digest(self, i: int) -> None:
+        """sdbf::fast on digest i only."""
+        self.engine._check(self.engine.lib.sdbfx_set_fold_digest(self.engine.ctx, self.handle, i))
+
     @property
     def folded(self) -> bool:
         return self.engine.lib.sdbfx_set_folded(self.handle) == 1
+
+    def digest_folded(self, i: int) -> bool:
+        return self.engine.lib.sdbfx_set_digest_folded(self.handle, i) == 1
 
     def add_text(self, text: bytes) -> int:
         """Appends the digests of an sdbf text (sdbf_set's file reader: lines up
@@ -278,7 +301,14 @@ class Engine:
 
     def _check(self, rc: int) -> None:
         if rc != 0:
-            raise SdbfxError(self.lib.sdbfx_last_error(self.ctx).decode())
+            raise SdbfxError(self.lib.sdbfx_last_error(self.ctx).decode(), rc)
+
+    def scratch_bytes(self) -> int:
+        return int(self.lib.sdbfx_ctx_scratch_bytes(self.ctx))
+
+    def trim(self) -> None:
+        """Frees the context's scratch memory."""
+        self._check(self.lib.sdbfx_ctx_trim(self.ctx))
 
     def new_set(self) -> DigestSet:
         return DigestSet(self)
@@ -292,9 +322,13 @@ class Engine:
         self._check(self.lib.sdbfx_compare_all(self.ctx, s.handle, _ptr(out)))
         return out[: n * (n - 1) // 2]
 
-    def compare_to(self, q: DigestSet, t: DigestSet) -> np.ndarray:
+    def compare_to(self, q: DigestSet, t: DigestSet, sample: int = 0) -> np.ndarray:
+        """scores[i, j] = q[i].compare(t[j], sample) (sdhash -c, -s sample)."""
         out = np.zeros((max(len(q), 1), max(len(t), 1)), np.int32)
-        self._check(self.lib.sdbfx_compare_to(self.ctx, q.handle, t.handle, _ptr(out)))
+        if sample:
+            self._check(self.lib.sdbfx_compare_to_sampled(self.ctx, q.handle, t.handle, sample, _ptr(out)))
+        else:
+            self._check(self.lib.sdbfx_compare_to(self.ctx, q.handle, t.handle, _ptr(out)))
         return out[: len(q), : len(t)]
 
     def sync(self) -> None:
@@ -336,11 +370,11 @@ class Server:
     the query's scores against every reference digest."""
 
     def __init__(self, ref: DigestSet, device: int = 0, batch_bytes: int = 256 << 20,
-                 batch_max: int = 1024):
+                 batch_max: int = 1024, mem_budget: int = 0):
         self.lib = load()
         self.ref = ref                      # must outlive the server
         p = C.c_void_p()
-        if self.lib.sdbfx_serve_create(device, ref.handle, batch_bytes, batch_max, C.byref(p)) != 0:
+        if self.lib.sdbfx_serve_create(device, ref.handle, batch_bytes, batch_max, mem_budget, C.byref(p)) != 0:
             raise SdbfxError("cannot create the server")
         self.handle = p
         self.nref = int(self.lib.sdbfx_serve_ref_size(p))
@@ -379,6 +413,12 @@ class Server:
         b, q, m = C.c_uint64(), C.c_uint64(), C.c_uint64()
         self.lib.sdbfx_serve_stats(self.handle, C.byref(b), C.byref(q), C.byref(m))
         return b.value, q.value, m.value
+
+    def admission(self) -> tuple[int, int]:
+        """(queries deferred to a later batch, queries refused) by the memory budget."""
+        d, r = C.c_uint64(), C.c_uint64()
+        self.lib.sdbfx_serve_admission(self.handle, C.byref(d), C.byref(r))
+        return d.value, r.value
 
     def error(self) -> str:
         return self.lib.sdbfx_serve_error(self.handle).decode()

@@ -10,6 +10,7 @@
 
 #include <cstdlib>
 #include <cstring>
+#include <stdexcept>
 #include <fstream>
 #include <iomanip>
 #include <sstream>
@@ -103,6 +104,13 @@ char *dupname(const char *s)
     return d;
 }
 
+// an index_info that asks for an index insert or search (sdbf_core.cc:110,
+// 151-153, 165)
+bool indexing(const index_info *info)
+{
+    return info && (info->index || (info->setlist && !info->setlist->empty()));
+}
+
 void check(sdbfx_ctx *ctx, int rc)
 {
     if (rc != 0) {
@@ -113,6 +121,121 @@ void check(sdbfx_ctx *ctx, int rc)
 }
 
 }  // namespace
+
+// ----------------------------------------------------------------- bloom_filter
+// sdbf/bloom_filter.cc:20-33, 44-64, 125-144, 263-359
+
+const uint32_t bloom_filter::BITS[] = {0x01, 0x02, 0x04, 0x08, 0x10, 0x20, 0x40, 0x80};
+
+// BIT_MASKS_32[k] = 2^k - 1 for k >= 8, the low masks for k < 8
+const uint32_t bloom_filter::BIT_MASKS_32[] = {
+    0x01,       0x03,       0x07,       0x0F,       0x1F,       0x3F,       0x7F,       0xFF,
+    0x01FF,     0x03FF,     0x07FF,     0x0FFF,     0x1FFF,     0x3FFF,     0x7FFF,     0xFFFF,
+    0x01FFFF,   0x03FFFF,   0x07FFFF,   0x0FFFFF,   0x1FFFFF,   0x3FFFFF,   0x7FFFFF,   0xFFFFFF,
+    0x01FFFFFF, 0x03FFFFFF, 0x07FFFFFF, 0x0FFFFFFF, 0x1FFFFFFF, 0x3FFFFFFF, 0x7FFFFFFF, 0xFFFFFFFF};
+
+static uint16_t log_size_of(uint64_t size)
+{
+    uint16_t n = 0;
+    for (uint64_t t = size; t; t >>= 1)
+        n++;
+    return n;
+}
+
+bloom_filter::bloom_filter(uint64_t size, uint16_t hash_count_, uint64_t max_elem_, double max_fp_)
+    : hamming(0), hamminglg(0), bf_size(size), max_elem(max_elem_), hash_count(hash_count_), max_fp(max_fp_)
+{
+    if (!size || (size & (size - 1)) || size < 64 || log_size_of(size) + 1 >= 32)
+        throw -1;
+    bit_mask = BIT_MASKS_32[log_size_of(size) + 1];
+    bf = (uint8_t *)calloc(size, 1);
+}
+
+bloom_filter::bloom_filter(uint8_t *data, uint64_t size, int id, int bf_elem_ct, uint16_t hamming_)
+    : hamming(hamming_), hamminglg(0), bf_size(size), max_elem(192), hash_count(5)
+{
+    bit_mask = 2047;
+    bf = (uint8_t *)malloc(size);
+    memcpy(bf, data, size);
+    bf_elem_count = (uint64_t)bf_elem_ct;
+    bl_id = id;
+}
+
+bloom_filter::~bloom_filter() { free(bf); }
+
+bool bloom_filter::query_and_set(uint32_t *sha1, bool mode_set)
+{
+    uint32_t bit_cnt = 0;
+    for (uint32_t i = 0; i < hash_count; i++) {
+        const uint64_t pos = sha1[i] & bit_mask;
+        const uint8_t bit = (uint8_t)BITS[pos & 7];
+        if (bf[pos >> 3] & bit)
+            bit_cnt++;
+        else if (mode_set)
+            bf[pos >> 3] |= bit;
+        else
+            return false;
+    }
+    if (!mode_set)
+        return bit_cnt == hash_count;
+    if (bit_cnt < hash_count) {
+        bf_elem_count++;
+        return true;
+    }
+    return false;
+}
+
+bool bloom_filter::insert_sha1(uint32_t *sha1) { return query_and_set(sha1, true); }
+bool bloom_filter::query_sha1(uint32_t *sha1) { return query_and_set(sha1, false); }
+uint64_t bloom_filter::elem_count() { return bf_elem_count; }
+
+// the reference returns -1 (no estimate, sdbf/bloom_filter.cc:163)
+double bloom_filter::est_fp_rate() { return -1.0; }
+
+double bloom_filter::bits_per_elem() { return (double)(bf_size << 3) / (double)bf_elem_count; }
+std::string bloom_filter::name() const { return setname; }
+void bloom_filter::set_name(std::string n) { setname = n; }
+int bloom_filter::bloom_id() { return bl_id; }
+void bloom_filter::set_bloom_id(int id) { bl_id = id; }
+
+// halves the filter `times` times by OR-ing its upper half onto its lower
+// half; as in sdbf/bloom_filter.cc:263-289 a halving that reaches 32 words
+// (256 bytes) ends the folding
+void bloom_filter::fold(uint32_t times)
+{
+    uint64_t rsize = bf_size / 8;
+    uint64_t *w = (uint64_t *)bf;
+    for (uint32_t i = 0; i < times; i++) {
+        for (uint64_t j = 0; j < rsize / 2; j++)
+            w[j] |= w[j + rsize / 2];
+        rsize /= 2;
+        if (rsize == 32)
+            break;
+    }
+    bf_size = rsize * 8;
+    bit_mask = BIT_MASKS_32[log_size_of(bf_size) + 1];
+    uint8_t *nb = (uint8_t *)malloc(bf_size);
+    memcpy(nb, bf, bf_size);
+    free(bf);
+    bf = nb;
+}
+
+int bloom_filter::add(bloom_filter *other)
+{
+    if (bf_size != other->bf_size)
+        return 1;
+    for (uint64_t j = 0; j < bf_size; j++)
+        bf[j] |= other->bf[j];
+    return 0;
+}
+
+void bloom_filter::compute_hamming()
+{
+    hamming = 0;
+    hamminglg = 0;
+    for (uint64_t j = 0; j < bf_size; j++)
+        hamminglg += (uint32_t)__builtin_popcount(bf[j]);
+}
 
 // ----------------------------------------------------------------- sdbf_conf
 
@@ -183,7 +306,8 @@ void sdbf::hash_buffer(const char *name, const uint8_t *buf, uint64_t len, uint3
 }
 
 void sdbf::hash_batch(const std::vector<std::string> &names, const std::vector<const uint8_t *> &bufs,
-                      const std::vector<uint64_t> &lens, uint32_t dd, std::vector<sdbf *> &out)
+                      const std::vector<uint64_t> &lens, uint32_t dd, std::vector<sdbf *> &out,
+                      const std::vector<uint64_t> *orig_sizes)
 {
     std::lock_guard<std::mutex> g(config->lock);
     sdbfx_ctx *ctx = config->ctx();
@@ -207,6 +331,8 @@ void sdbf::hash_batch(const std::vector<std::string> &names, const std::vector<c
             continue;
         sdbf *s = new sdbf();
         s->from_device(set, k++, nm[i]);
+        if (orig_sizes)
+            s->orig_file_size = (*orig_sizes)[i];
         out[i] = s;
     }
     sdbfx_set_destroy(set);
@@ -230,7 +356,7 @@ sdbf::sdbf(const char *filename, uint32_t dd)
 // digest covers all msize bytes (zero filled past the read)
 sdbf::sdbf(const char *name, std::istream *ifs, uint32_t dd, uint64_t msize, index_info *info)
 {
-    if (info)
+    if (indexing(info))
         throw -1;
     std::vector<uint8_t> data(msize, 0);
     ifs->read((char *)data.data(), (std::streamsize)msize);
@@ -244,7 +370,7 @@ sdbf::sdbf(const char *name, std::istream *ifs, uint32_t dd, uint64_t msize, ind
 // sdbf/sdbf_class.cc:115-136
 sdbf::sdbf(const char *name, char *str, uint32_t dd, uint64_t length, index_info *info)
 {
-    if (info)
+    if (indexing(info))
         throw -1;
     if (length < 512)
         throw -3;
@@ -328,6 +454,7 @@ sdbf::sdbf(const std::string &str)
 
 sdbf::~sdbf()
 {
+    delete big_filters;
     free(buffer);
     free(hamming);
     free(elem_counts);
@@ -355,33 +482,62 @@ int32_t sdbf::get_elem_count(sdbf *mine, uint64_t index)
     return mine->elem_counts[index];
 }
 
-// sdbf/sdbf_class.cc:266-271 (sample is not supported: all filters are used)
+// uploads one digest to a device set, folded on the device when the host
+// object is in fast mode (its host filters are folded already; folding is
+// idempotent, and it sets the device's Hamming weights to the folded ones)
+static int add_to_device(sdbfx_ctx *ctx, sdbfx_set *set, sdbf *s)
+{
+    const uint32_t idx = sdbfx_set_size(set);
+    int rc = sdbfx_set_add(ctx, set, s->name(), s->input_size(), s->filter_count(), s->last_count_(),
+                           s->dd_block_size_(), s->max_elem, s->buffer, s->elem_counts_());
+    if (rc == 0 && s->fast_mode())
+        rc = sdbfx_set_fold_digest(ctx, set, idx);
+    return rc;
+}
+
+// sdbf/sdbf_class.cc:266-271 -> sdbf_score (sdbf/sdbf_core.cc:331-389),
+// `sample` included
 int32_t sdbf::compare(sdbf *other, uint32_t sample)
 {
-    (void)sample;
     std::lock_guard<std::mutex> g(config->lock);
     sdbfx_ctx *ctx = config->ctx();
-    sdbfx_set *set = nullptr;
-    check(ctx, sdbfx_set_create(ctx, &set));
-    sdbf *two[2] = {this, other};
-    for (sdbf *s : two) {
-        int rc;
-        if (s->elem_counts)
-            rc = sdbfx_set_add(ctx, set, s->hashname, s->orig_file_size, s->bf_count, 0, s->dd_block_size,
-                               s->max_elem, s->buffer, s->elem_counts);
-        else
-            rc = sdbfx_set_add(ctx, set, s->hashname, s->orig_file_size, s->bf_count, s->last_count, 0,
-                               s->max_elem, s->buffer, nullptr);
-        if (rc != 0) {
-            sdbfx_set_destroy(set);
-            check(ctx, rc);
-        }
-    }
+    sdbfx_set *q = nullptr, *t = nullptr;
+    check(ctx, sdbfx_set_create(ctx, &q));
+    int rc = sdbfx_set_create(ctx, &t);
+    if (rc == 0)
+        rc = add_to_device(ctx, q, this);
+    if (rc == 0)
+        rc = add_to_device(ctx, t, other);
     int32_t score = -1;
-    const int rc = sdbfx_compare_all(ctx, set, &score);
-    sdbfx_set_destroy(set);
+    if (rc == 0)
+        rc = sdbfx_compare_to_sampled(ctx, q, t, sample, &score);
+    sdbfx_set_destroy(q);
+    sdbfx_set_destroy(t);
     check(ctx, rc);
     return score;
+}
+
+// sdbf/sdbf_class.cc:426-439 with bloom_filter::fold(2) and compute_hamming
+// (sdbf/bloom_filter.cc:263-289, 453-461): each filter's 32 64-bit words
+// folded twice (w[j] |= w[j + 16], j < 16; then w[j] |= w[j + 8], j < 8), the
+// 64 folded bytes written over the filter's first 64 bytes, its Hamming
+// weight their popcount
+void sdbf::fast()
+{
+    for (uint32_t i = 0; i < bf_count; i++) {
+        uint64_t w[32];
+        memcpy(w, buffer + (size_t)i * bf_size, 256);
+        for (int j = 0; j < 16; j++)
+            w[j] |= w[j + 16];
+        for (int j = 0; j < 8; j++)
+            w[j] |= w[j + 8];
+        memcpy(buffer + (size_t)i * bf_size, w, 64);
+        uint32_t h = 0;
+        for (int j = 0; j < 8; j++)
+            h += (uint32_t)__builtin_popcountll(w[j]);
+        hamming[i] = (uint16_t)h;
+    }
+    fastmode = true;
 }
 
 // sdbf/sdbf_class.cc:294-334
@@ -414,10 +570,13 @@ std::ostream &operator<<(std::ostream &os, const sdbf *s) { return os << s->to_s
 
 // ----------------------------------------------------------------- sdbf_set
 
-sdbf_set::sdbf_set() {}
+sdbf_set::sdbf_set() : bf_vector(new std::vector<bloom_filter *>()) {}
+
+// sdbf/sdbf_set.cc:40-46
+sdbf_set::sdbf_set(bloom_filter *idx) : index(idx), bf_vector(new std::vector<bloom_filter *>()) {}
 
 // sdbf/sdbf_set.cc:51-74
-sdbf_set::sdbf_set(const char *fname)
+sdbf_set::sdbf_set(const char *fname) : bf_vector(new std::vector<bloom_filter *>())
 {
     struct stat st;
     if (fname && stat(fname, &st) == 0 && S_ISREG(st.st_mode)) {
@@ -430,12 +589,29 @@ sdbf_set::sdbf_set(const char *fname)
             items.push_back(new sdbf(line));
         }
     }
+    vector_init();
 }
 
+// sdbf/sdbf_set.cc:80-86: the set's digests are not deleted, bf_vector is
 sdbf_set::~sdbf_set()
 {
     if (dev)
         sdbfx_set_destroy(dev);
+    for (bloom_filter *b : *bf_vector)
+        delete b;
+    delete bf_vector;
+}
+
+// sdbf/sdbf_set.cc:409-420: a host copy of every filter of every digest
+void sdbf_set::vector_init()
+{
+    for (uint32_t i = 0; i < items.size(); i++)
+        for (uint32_t n = 0; n < items[i]->filter_count(); n++) {
+            bloom_filter *b = new bloom_filter(items[i]->buffer + (size_t)n * 256, 256, (int)i,
+                                               sdbf::get_elem_count(items[i], n), items[i]->hamming[n]);
+            b->set_name(items[i]->name());
+            bf_vector->push_back(b);
+        }
 }
 
 sdbf *sdbf_set::at(uint32_t pos) { return pos < items.size() ? items[pos] : nullptr; }
@@ -462,13 +638,9 @@ uint64_t sdbf_set::input_size()
     return n;
 }
 
-uint64_t sdbf_set::filter_count()
-{
-    uint64_t n = 0;
-    for (sdbf *s : items)
-        n += s->filter_count();
-    return n;
-}
+// sdbf/sdbf_set.cc:401-404: the size of bf_vector (filled by vector_init,
+// which the file constructor calls)
+uint64_t sdbf_set::filter_count() { return bf_vector->size(); }
 
 int sdbf_set::empty() { return items.empty() ? 1 : 0; }
 
@@ -498,18 +670,12 @@ sdbfx_set *sdbf_set::device_set()
         check(ctx, sdbfx_set_create(ctx, &dev));
         dev_items = 0;
     }
-    for (; dev_items < items.size(); dev_items++) {
-        sdbf *s = items[dev_items];
-        if (s->elem_counts)
-            check(ctx, sdbfx_set_add(ctx, dev, s->hashname, s->orig_file_size, s->bf_count, 0, s->dd_block_size,
-                                     s->max_elem, s->buffer, s->elem_counts));
-        else
-            check(ctx, sdbfx_set_add(ctx, dev, s->hashname, s->orig_file_size, s->bf_count, s->last_count, 0,
-                                     s->max_elem, s->buffer, nullptr));
-    }
-    // fast mode folds every item (sdbf/sdbf_set.cc:243-248), later ones too
-    if (fast_)
-        check(ctx, sdbfx_set_fold(ctx, dev));
+    for (; dev_items < items.size(); dev_items++)
+        check(ctx, add_to_device(ctx, dev, items[dev_items]));
+    // items folded (sdbf::fast) after their upload
+    for (uint32_t i = 0; i < items.size(); i++)
+        if (items[i]->fast_mode() && !sdbfx_set_digest_folded(dev, i))
+            check(ctx, sdbfx_set_fold_digest(ctx, dev, i));
     return dev;
 }
 
@@ -526,8 +692,10 @@ static void put_line(std::ostream &out, const char *a, const char *b, char sep, 
 std::string sdbf_set::compare_all_quiet(int32_t threshold, int32_t thread_count, bool fast)
 {
     (void)thread_count;
+    // sdbf/sdbf_set.cc:287-292: every item folded
     if (fast)
-        fast_ = true;
+        for (sdbf *s : items)
+            s->fast();
     std::stringstream out;
     out.fill('0');
     const size_t n = items.size();
@@ -557,9 +725,17 @@ std::string sdbf_set::compare_to_quiet(sdbf_set *other, int32_t threshold, uint3
                                        int32_t thread_count, bool fast)
 {
     (void)thread_count;
-    (void)sample_size;
-    if (fast)
-        fast_ = other->fast_ = true;   // both sets folded (sdbf/sdbf_set.cc:322-333)
+    // sdbf/sdbf_set.cc:367-376: fast mode folds this->items.at(i) for i below
+    // other's size, then for i below this set's size -- so only this set is
+    // folded, and a larger other set makes at() throw std::out_of_range
+    // (inside an OpenMP region in the reference, which terminates)
+    if (fast) {
+        if (other->items.size() > items.size())
+            throw std::out_of_range("sdbf_set::compare_to: fast mode folds this->items.at(i) for every i "
+                                    "below the other set's size");
+        for (sdbf *s : items)
+            s->fast();
+    }
     std::stringstream out;
     out.fill('0');
     const size_t nq = items.size(), nt = other->items.size();
@@ -570,7 +746,7 @@ std::string sdbf_set::compare_to_quiet(sdbf_set *other, int32_t threshold, uint3
         std::lock_guard<std::mutex> g(sdbf::config->lock);
         sdbfx_set *q = device_set();
         sdbfx_set *t = other->device_set();
-        check(sdbf::config->ctx(), sdbfx_compare_to(sdbf::config->ctx(), q, t, scores.data()));
+        check(sdbf::config->ctx(), sdbfx_compare_to_sampled(sdbf::config->ctx(), q, t, sample_size, scores.data()));
     }
     for (size_t i = 0; i < nq; i++)
         for (size_t j = 0; j < nt; j++)

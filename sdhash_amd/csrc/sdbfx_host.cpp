@@ -38,7 +38,8 @@ struct DevBuf {
 struct sdbfx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    uint64_t budget = 0;
+    uint64_t budget = 0;   // bytes of scratch (the DevBufs below) allowed, 0 = no limit
+    uint64_t used = 0;     // bytes of scratch held
     std::string err;
     bool prof = false;
     int cmp_method = SDBFX_CMP_AUTO;
@@ -69,14 +70,23 @@ struct sdbfx_set {
     // built on first use: per digest whether it is built, and the event
     // after the last build
     DevBuf cord, ckey;
-    // digests [0, nfolded) are folded to 64 bytes (sdbfx_set_fold)
-    uint32_t nfolded = 0;
+    // per digest: 1 when folded to 64 bytes (sdbf::fast, sdbfx_set_fold)
+    std::vector<uint8_t> fold;
     mutable std::vector<uint8_t> cbuilt;
     mutable hipEvent_t cev = nullptr;
     mutable std::mutex cmu;
 };
 
 namespace {
+
+// every scratch buffer of a context
+std::vector<DevBuf*> scratch_bufs(sdbfx_ctx* ctx)
+{
+    return {&ctx->stage, &ctx->blocks, &ctx->tiles, &ctx->tstate, &ctx->units, &ctx->files, &ctx->cand,
+            &ctx->sfilt, &ctx->sham, &ctx->counts, &ctx->gsrc, &ctx->gdst, &ctx->gcnt, &ctx->glast, &ctx->groups,
+            &ctx->pairs, &ctx->partial, &ctx->imax, &ctx->scores, &ctx->gbest, &ctx->rmap, &ctx->rcode,
+            &ctx->text, &ctx->tdesc, &ctx->thdr, &ctx->cofirst, &ctx->cocount};
+}
 
 int fail(sdbfx_ctx* ctx, const std::string& msg)
 {
@@ -92,21 +102,30 @@ int fail(sdbfx_ctx* ctx, const std::string& msg)
             return fail(ctx, std::string(#call) + ": " + hipGetErrorString(e_));          \
     } while (0)
 
-// grows a device buffer to at least `bytes` (contents not kept)
+// grows a scratch buffer of the context to at least `bytes` (contents not
+// kept), within the context's budget: SDBFX_EBUDGET when the scratch held
+// would exceed it (nothing is freed or allocated then)
 int ensure(sdbfx_ctx* ctx, DevBuf& b, size_t bytes)
 {
     if (b.cap >= bytes && b.p)
         return 0;
+    size_t cap = std::max<size_t>(bytes, 256);
+    cap = (cap + 4095) & ~(size_t)4095;
+    if (ctx->budget && ctx->used - b.cap + cap > ctx->budget) {
+        ctx->err = "scratch of " + std::to_string(ctx->used - b.cap + cap) + " bytes exceeds the context's budget of " +
+                   std::to_string(ctx->budget);
+        return SDBFX_EBUDGET;
+    }
     if (b.p) {
         HIPC(ctx, hipStreamSynchronize(ctx->stream));
         HIPC(ctx, hipFree(b.p));
+        ctx->used -= b.cap;
         b.p = nullptr;
         b.cap = 0;
     }
-    size_t cap = std::max<size_t>(bytes, 256);
-    cap = (cap + 4095) & ~(size_t)4095;
     HIPC(ctx, hipMalloc(&b.p, cap));
     b.cap = cap;
+    ctx->used += cap;
     return 0;
 }
 
@@ -364,6 +383,7 @@ int digest_pass(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t i0, uint32_t i1, const 
             continue;
         const bool blk = dd && dd[i];
         set->names.push_back(names && names[i] ? names[i] : "");
+        set->fold.push_back(0);
         set->orig.push_back(lens[i]);
         set->first.push_back(first[i - i0]);
         set->bfc.push_back((uint32_t)cnt[i - i0]);
@@ -427,6 +447,23 @@ int digest_common(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t n, const uint8_t* d_b
             return fail(ctx, "dd block size must be a multiple of 16");
         status[i] = lens[i] < SDBFX_MIN_FILE ? -3 : 0;
     }
+    // a failing call leaves the set as it was (inputs are digested in passes
+    // of up to kPassBytes, each appending its digests)
+    const size_t nd0 = set->names.size();
+    const uint64_t nf0 = set->nfilt;
+    auto rollback = [&](int rc) {
+        set->names.resize(nd0);
+        set->fold.resize(nd0);
+        set->orig.resize(nd0);
+        set->first.resize(nd0);
+        set->bfc.resize(nd0);
+        set->last.resize(nd0);
+        set->dd.resize(nd0);
+        set->maxe.resize(nd0);
+        set->last_ec.resize(nd0);
+        set->nfilt = nf0;
+        return rc;
+    };
     uint32_t i0 = 0;
     std::vector<uint64_t> hoff;
     while (i0 < n) {
@@ -447,17 +484,18 @@ int digest_common(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t n, const uint8_t* d_b
                 pos += (lens[i] + SDBFX_PAD - 1) / SDBFX_PAD * SDBFX_PAD;
             }
             if (int rc = ensure(ctx, ctx->stage, pos + SDBFX_PAD))
-                return rc;
-            HIPC(ctx, hipMemsetAsync(ctx->stage.p, 0, SDBFX_PAD, ctx->stream));
+                return rollback(rc);
+            if (hipMemsetAsync(ctx->stage.p, 0, SDBFX_PAD, ctx->stream) != hipSuccess)
+                return rollback(fail(ctx, "hipMemsetAsync failed"));
             for (uint32_t i = i0; i < i1; i++)
                 if (lens[i])
                     if (int rc = upload_host(ctx, (uint8_t*)ctx->stage.p + hoff[i], host_bufs[i], lens[i]))
-                        return rc;
+                        return rollback(rc);
             base = (const uint8_t*)ctx->stage.p;
             o = hoff.data();
         }
         if (int rc = digest_pass(ctx, set, i0, i1, base, o, lens, names, dd, status))
-            return rc;
+            return rollback(rc);
         i0 = i1;
     }
     return 0;
@@ -465,8 +503,9 @@ int digest_common(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t n, const uint8_t* d_b
 
 // elem count of the last filter (sdbf::get_elem_count(s, bf_count-1))
 struct PairRef {
-    uint32_t set;   // 0 = A, 1 = B
+    uint32_t set;     // 0 = A, 1 = B
     uint32_t idx;
+    uint32_t count;   // filters taken from the digest (all, or a sample's first ones)
 };
 
 // Kernel choice for SDBFX_CMP_AUTO from the padded work of each tiling, per
@@ -492,60 +531,36 @@ bool use_mfma(const sdbfx_ctx* ctx, const std::vector<uint64_t>& inst, const std
     return mfma < valu;
 }
 
-int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool all, int32_t* out)
-{
-    const uint32_t na = (uint32_t)A->names.size();
-    const uint32_t nb = all ? na : (uint32_t)B->names.size();
-    const uint64_t npairs = all ? (uint64_t)na * (na ? na - 1 : 0) / 2 : (uint64_t)na * nb;
-    if (npairs == 0)
-        return 0;
-    if (npairs > 0xFFFFFFFFull)
-        return fail(ctx, "too many digest pairs in one call");
-    const sdbfx_set* S[2] = {A, all ? A : B};
-    // fast mode: every digest of both sets folded (sdbfx_set_fold), or none
-    bool fast = false;
-    for (int si = 0; si < (all ? 1 : 2); si++) {
-        const uint32_t n = (uint32_t)S[si]->names.size();
-        if (S[si]->nfolded != 0 && S[si]->nfolded != n)
-            return fail(ctx, "set mixes folded and plain digests");
-        const bool f = n && S[si]->nfolded == n;
-        if (si == 1 && f != fast)
-            return fail(ctx, "a folded set can only be compared with a folded set");
-        fast = f;
-    }
+// Scoring mode of one oriented pair.  sdbf_max_score takes the cutoff table
+// and bit count from the reference digest's fastmode flag only
+// (sdbf/sdbf_core.cc:422-437), so a pair whose reference digest is folded
+// is scored on 64 bytes (CMP_MODE_FAST) whatever its target, and a plain
+// reference against a folded target is scored on all 256 bytes with the
+// target's folded Hamming weight (CMP_MODE_MIXED): there the denominator
+// max_est - cut_off is an unsigned 32-bit difference that can wrap or be 0,
+// and the fraction can exceed 1.
+enum { CMP_MODE_PLAIN = 0, CMP_MODE_FAST = 1, CMP_MODE_MIXED = 2 };
 
-    // orientation of every pair, in output order
-    std::vector<PairRef> ref(npairs), tgt(npairs);
-    {
-        uint64_t k = 0;
-        for (uint32_t i = 0; i < na; i++) {
-            const uint32_t j0 = all ? i + 1 : 0;
-            for (uint32_t j = j0; j < nb; j++, k++) {
-                PairRef x{0, i}, y{all ? 0u : 1u, j};
-                const sdbfx_set* X = S[x.set];
-                const sdbfx_set* Y = S[y.set];
-                const uint32_t n1 = X->bfc[x.idx], n2 = Y->bfc[y.idx];
-                const bool swap = n1 > n2 ||
-                                  (n1 == n2 && (X->last_ec[x.idx] > Y->last_ec[y.idx] ||
-                                                strcmp(X->names[x.idx].c_str(), Y->names[y.idx].c_str()) > 0));
-                ref[k] = swap ? y : x;
-                tgt[k] = swap ? x : y;
-            }
-        }
-    }
+// One comparison launch sequence over the oriented pairs `sel` (indices into
+// ref / tgt / out), all of one scoring mode.
+int compare_pairs(sdbfx_ctx* ctx, const sdbfx_set* const S[2], bool all, uint32_t na, uint32_t nb,
+                  const std::vector<PairRef>& ref, const std::vector<PairRef>& tgt, const std::vector<uint32_t>& sel,
+                  int mode, int32_t* out)
+{
+    const uint64_t npairs = sel.size();
     // group pairs by target digest (counting sort on a dense key)
     const uint32_t nkeys = na + (all ? 0 : nb);
     auto key = [&](const PairRef& r) { return r.set ? na + r.idx : r.idx; };
     std::vector<uint32_t> kc(nkeys + 1, 0);
-    for (uint64_t k = 0; k < npairs; k++)
-        kc[key(tgt[k]) + 1]++;
+    for (uint64_t s = 0; s < npairs; s++)
+        kc[key(tgt[sel[s]]) + 1]++;
     for (uint32_t q = 0; q < nkeys; q++)
         kc[q + 1] += kc[q];
     std::vector<uint32_t> order(npairs);
     {
         std::vector<uint32_t> pos(kc.begin(), kc.end() - 1);
-        for (uint64_t k = 0; k < npairs; k++)
-            order[pos[key(tgt[k])]++] = (uint32_t)k;
+        for (uint64_t s = 0; s < npairs; s++)
+            order[pos[key(tgt[sel[s]])]++] = sel[s];
     }
     std::vector<uint64_t> ginst, gtgt;
     for (uint32_t q = 0; q < nkeys; q++) {
@@ -555,13 +570,12 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
         uint64_t ic = 0;
         for (uint32_t s = kc[q]; s < kc[q + 1]; s++) {
             const PairRef r = ref[order[s]];
-            ic += S[r.set]->bfc[r.idx];
+            ic += r.count;
         }
         ginst.push_back(ic);
         gtgt.push_back(T->bfc[q >= na ? q - na : q]);
     }
-    // folded filters: the 64-byte form of the VALU kernel
-    const bool mfma = !fast && use_mfma(ctx, ginst, gtgt);
+    const bool mfma = mode == CMP_MODE_PLAIN && use_mfma(ctx, ginst, gtgt);
     const uint32_t rows = mfma ? SDBFX_MROWS : SDBFX_CMP_ROWS;
     // target filters per work item.  The MFMA kernel's fixed costs (row
     // staging, publishing maxima) are per work item and its pruning works
@@ -602,7 +616,7 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
             const sdbfx_set* R = S[r.set];
             CmpPair& p = pairs[s];
             p.ref_first = R->first[r.idx];
-            p.ref_count = R->bfc[r.idx];
+            p.ref_count = r.count;
             p.nch = g.nch;
             p.inst_begin = inst;
             p.partial_begin = g.partial_begin + (inst - g.inst_begin) * g.nch;
@@ -623,8 +637,12 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
 
     int rc;
     if ((rc = upload(ctx, ctx->groups, groups)) || (rc = upload(ctx, ctx->pairs, pairs)) ||
-        (rc = ensure(ctx, ctx->partial, partial_n * 4 + 256)) || (rc = ensure(ctx, ctx->imax, inst * 4 + 256)) ||
+        (rc = ensure(ctx, ctx->partial, partial_n * (mode == CMP_MODE_MIXED ? 8 : 4) + 256)) ||
+        (rc = ensure(ctx, ctx->imax, inst * (mode == CMP_MODE_MIXED ? 8 : 4) + 256)) ||
         (rc = ensure(ctx, ctx->scores, npairs * 4 + 256)))
+        return rc;
+    if (wi && mfma && ((rc = ensure(ctx, ctx->gbest, inst * 8 + 256)) || (rc = ensure(ctx, ctx->rmap, inst * 8 + 256)) ||
+                       (rc = ensure(ctx, ctx->rcode, inst * 4 + 256))))
         return rc;
     auto view = [](const sdbfx_set* X) {
         return SetView{(const uint32_t*)X->filt.p, (const uint16_t*)X->ecf.p, (const uint16_t*)X->hamf.p,
@@ -684,9 +702,6 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
         }
         // one launch: the last column-order build, and per instance its row
         // data and its shared maximum for pruning cleared
-        if ((rc = ensure(ctx, ctx->gbest, inst * 8 + 256)) || (rc = ensure(ctx, ctx->rmap, inst * 8 + 256)) ||
-            (rc = ensure(ctx, ctx->rcode, inst * 4 + 256)))
-            return rc;
         const sdbfx_set* T = S[last >= 0 ? last : 0];
         const uint32_t nd = last >= 0 ? (uint32_t)cfirst[last].size() : 0;
         if (nd && ((rc = upload(ctx, ctx->cofirst, cfirst[last])) || (rc = upload(ctx, ctx->cocount, ccount[last]))))
@@ -705,25 +720,112 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
                (unsigned long long*)ctx->gbest.p);
     } else if (wi) {
         const uint32_t grid = (uint32_t)std::min<uint64_t>(wi, 16384);
-        if (fast)
+        if (mode == CMP_MODE_FAST)
             LAUNCH(ctx, cmp_fast_kernel, grid, 64, va, vb, (const CmpGroup*)ctx->groups.p,
-                   (uint32_t)groups.size(), (const CmpPair*)ctx->pairs.p, wi, (uint32_t*)ctx->partial.p);
+                   (uint32_t)groups.size(), (const CmpPair*)ctx->pairs.p, wi, ctx->partial.p);
+        else if (mode == CMP_MODE_MIXED)
+            LAUNCH(ctx, cmp_mixed_kernel, grid, 64, va, vb, (const CmpGroup*)ctx->groups.p,
+                   (uint32_t)groups.size(), (const CmpPair*)ctx->pairs.p, wi, ctx->partial.p);
         else
             LAUNCH(ctx, cmp_kernel, grid, 64, va, vb, (const CmpGroup*)ctx->groups.p, (uint32_t)groups.size(),
-                   (const CmpPair*)ctx->pairs.p, wi, (uint32_t*)ctx->partial.p);
+                   (const CmpPair*)ctx->pairs.p, wi, ctx->partial.p);
     }
-    if (inst)
-        LAUNCH(ctx, chunk_max_kernel, (uint32_t)((inst + 255) / 256), 256, (const CmpPair*)ctx->pairs.p,
-               (uint32_t)npairs, inst, (const uint32_t*)ctx->partial.p, (uint32_t*)ctx->imax.p);
-    LAUNCH(ctx, pair_sum_kernel, (uint32_t)((npairs + 255) / 256), 256, (const CmpPair*)ctx->pairs.p,
-           (uint32_t)npairs, va.ecf, vb.ecf, (const uint32_t*)ctx->imax.p, (int32_t*)ctx->scores.p);
+    if (mode == CMP_MODE_MIXED) {
+        if (inst)
+            LAUNCH(ctx, chunk_max_wide_kernel, (uint32_t)((inst + 255) / 256), 256, (const CmpPair*)ctx->pairs.p,
+                   (uint32_t)npairs, inst, (const uint64_t*)ctx->partial.p, (uint64_t*)ctx->imax.p);
+        LAUNCH(ctx, pair_sum_wide_kernel, (uint32_t)((npairs + 255) / 256), 256, (const CmpPair*)ctx->pairs.p,
+               (uint32_t)npairs, va.ecf, vb.ecf, (const uint64_t*)ctx->imax.p, (int32_t*)ctx->scores.p);
+    } else {
+        if (inst)
+            LAUNCH(ctx, chunk_max_kernel, (uint32_t)((inst + 255) / 256), 256, (const CmpPair*)ctx->pairs.p,
+                   (uint32_t)npairs, inst, (const uint32_t*)ctx->partial.p, (uint32_t*)ctx->imax.p);
+        LAUNCH(ctx, pair_sum_kernel, (uint32_t)((npairs + 255) / 256), 256, (const CmpPair*)ctx->pairs.p,
+               (uint32_t)npairs, va.ecf, vb.ecf, (const uint32_t*)ctx->imax.p, (int32_t*)ctx->scores.p);
+    }
     std::vector<int32_t> sorted(npairs);
     HIPC(ctx, hipMemcpyAsync(sorted.data(), ctx->scores.p, npairs * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPC(ctx, hipStreamSynchronize(ctx->stream));
-    for (uint64_t s = 0; s < npairs; s++)
+    for (uint64_t s = 0; s < npairs; s++) {
+        if (sorted[s] == SDBFX_SCORE_NONFINITE)
+            return fail(ctx, "a pair's score is not finite (a plain reference filter against a folded target "
+                             "filter whose Hamming weight equals the cutoff): the reference aborts on it "
+                             "(boost::math::round of infinity)");
         out[order[s]] = sorted[s];
+    }
     return 0;
 }
+
+// digests [a0, a1) of A against B (or, with `all`, every pair i < j of A)
+int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool all, uint32_t sample, uint32_t a0,
+                 uint32_t a1, int32_t* out)
+{
+    const uint32_t na = (uint32_t)A->names.size();
+    const uint32_t nb = all ? na : (uint32_t)B->names.size();
+    const uint64_t npairs = all ? (uint64_t)na * (na ? na - 1 : 0) / 2 : (uint64_t)(a1 - a0) * nb;
+    if (npairs == 0)
+        return 0;
+    if (npairs > 0xFFFFFFFFull)
+        return fail(ctx, "too many digest pairs in one call");
+    const sdbfx_set* const S[2] = {A, all ? A : B};
+
+    // Sampling (sdbf_score's `sample`, sdbf_core.cc:342-347): a calling
+    // digest with more than `sample` filters takes part with its first
+    // `sample` ones, and the swap rule sees that count and the element count
+    // of filter sample - 1.  (The reference's random offsets, :370-372, are
+    // never drawn: after the swap rule the reference side never has more than
+    // `sample` filters.)  The calling digest is the row digest x below.
+    std::vector<uint16_t> ecA;   // element counts of A's block-mode digests, when sampled
+    auto ec_at = [&](uint32_t idx, uint32_t f) -> uint32_t {
+        if (!A->dd[idx])
+            return f + 1 < A->bfc[idx] ? A->maxe[idx] : A->last[idx];
+        return ecA[A->first[idx] + f];
+    };
+    if (sample) {
+        bool need = false;
+        for (uint32_t i = a0; i < a1; i++)
+            need |= A->dd[i] && A->bfc[i] > sample;
+        if (need) {
+            ecA.resize(A->nfilt);
+            HIPC(ctx, hipMemcpyAsync(ecA.data(), A->ecf.p, A->nfilt * 2, hipMemcpyDeviceToHost, ctx->stream));
+            HIPC(ctx, hipStreamSynchronize(ctx->stream));
+        }
+    }
+    // orientation of every pair, in output order (sdbf_core.cc:349-359)
+    std::vector<PairRef> ref(npairs), tgt(npairs);
+    std::vector<uint32_t> sel[3];
+    {
+        uint64_t k = 0;
+        for (uint32_t i = a0; i < a1; i++) {
+            const uint32_t j0 = all ? i + 1 : 0;
+            const bool sampled = sample && A->bfc[i] > sample;
+            const uint32_t n1 = sampled ? sample : A->bfc[i];
+            const uint32_t e1 = sampled ? ec_at(i, n1 - 1) : A->last_ec[i];
+            for (uint32_t j = j0; j < nb; j++, k++) {
+                PairRef x{0, i, n1}, y{all ? 0u : 1u, j, 0};
+                const sdbfx_set* X = S[x.set];
+                const sdbfx_set* Y = S[y.set];
+                const uint32_t n2 = Y->bfc[y.idx];
+                y.count = n2;
+                const bool swap = n1 > n2 ||
+                                  (n1 == n2 && (e1 > Y->last_ec[y.idx] ||
+                                                strcmp(X->names[x.idx].c_str(), Y->names[y.idx].c_str()) > 0));
+                if (swap)
+                    x.count = X->bfc[x.idx];
+                ref[k] = swap ? y : x;
+                tgt[k] = swap ? x : y;
+                const bool rf = S[ref[k].set]->fold[ref[k].idx] != 0, tf = S[tgt[k].set]->fold[tgt[k].idx] != 0;
+                sel[rf ? CMP_MODE_FAST : tf ? CMP_MODE_MIXED : CMP_MODE_PLAIN].push_back((uint32_t)k);
+            }
+        }
+    }
+    for (int mode = 0; mode < 3; mode++)
+        if (!sel[mode].empty())
+            if (int rc = compare_pairs(ctx, S, all, na, nb, ref, tgt, sel[mode], mode, out))
+                return rc;
+    return 0;
+}
+
 
 }  // namespace
 
@@ -758,12 +860,7 @@ void sdbfx_ctx_destroy(sdbfx_ctx* ctx)
         return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    DevBuf* bufs[] = {&ctx->stage, &ctx->blocks, &ctx->tiles, &ctx->tstate, &ctx->units, &ctx->files,
-                      &ctx->cand, &ctx->sfilt, &ctx->sham, &ctx->counts, &ctx->gsrc, &ctx->gdst,
-                      &ctx->gcnt, &ctx->glast, &ctx->groups, &ctx->pairs, &ctx->partial, &ctx->imax, &ctx->scores,
-                      &ctx->gbest, &ctx->rmap, &ctx->rcode, &ctx->text, &ctx->tdesc, &ctx->thdr,
-                      &ctx->cofirst, &ctx->cocount};
-    for (DevBuf* b : bufs)
+    for (DevBuf* b : scratch_bufs(ctx))
         if (b->p)
             (void)hipFree(b->p);
     for (int k = 0; k < 2; k++) {
@@ -789,6 +886,24 @@ int sdbfx_sync(sdbfx_ctx* ctx)
     if (!ctx)
         return -1;
     HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+uint64_t sdbfx_ctx_scratch_bytes(const sdbfx_ctx* ctx) { return ctx ? ctx->used : 0; }
+
+int sdbfx_ctx_trim(sdbfx_ctx* ctx)
+{
+    if (!ctx)
+        return -1;
+    HIPC(ctx, hipSetDevice(ctx->device));
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    for (DevBuf* b : scratch_bufs(ctx))
+        if (b->p) {
+            HIPC(ctx, hipFree(b->p));
+            b->p = nullptr;
+            b->cap = 0;
+        }
+    ctx->used = 0;
     return 0;
 }
 
@@ -887,6 +1002,7 @@ int sdbfx_set_add(sdbfx_ctx* ctx, sdbfx_set* set, const char* name, uint64_t ori
                              hipMemcpyHostToDevice, ctx->stream));
     HIPC(ctx, hipStreamSynchronize(ctx->stream));
     set->names.push_back(name ? name : "");
+    set->fold.push_back(0);
     set->orig.push_back(orig_size);
     set->first.push_back(set->nfilt);
     set->bfc.push_back(bf_count);
@@ -1122,6 +1238,7 @@ int sdbfx_set_add_text(sdbfx_ctx* ctx, sdbfx_set* set, const char* text, uint64_
         return bad("base64 or element counts");
     for (size_t d = 0; d < hs.size(); d++) {
         set->names.push_back(hs[d].name);
+        set->fold.push_back(0);
         set->orig.push_back(hs[d].orig);
         set->first.push_back(tp[d].filt_first);
         set->bfc.push_back(hs[d].bfc);
@@ -1136,37 +1253,80 @@ int sdbfx_set_add_text(sdbfx_ctx* ctx, sdbfx_set* set, const char* text, uint64_
     return 0;
 }
 
-int sdbfx_set_fold(sdbfx_ctx* ctx, sdbfx_set* set)
+}  // extern "C"
+
+namespace {
+
+// Folds digests [d0, d1) of the set that are not folded yet (sdbf::fast,
+// sdbf/sdbf_class.cc:426-439): runs of consecutive plain digests are folded
+// by one launch each.
+int fold_range(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t d0, uint32_t d1)
 {
-    if (!ctx || !set)
-        return fail(ctx, "null argument");
-    const uint32_t n = (uint32_t)set->names.size();
-    if (set->nfolded >= n)
-        return 0;
     HIPC(ctx, hipSetDevice(ctx->device));
-    // the digests appended since the last fold (folding is idempotent, as
-    // each filter's last 192 bytes stay)
-    const uint64_t f0 = set->first[set->nfolded], nf = set->nfilt - f0;
     {
         std::lock_guard<std::mutex> lk(set->cmu);
         if (set->cev)
             HIPC(ctx, hipStreamWaitEvent(ctx->stream, set->cev, 0));
         // Hamming weights change: column orders built before are void
-        set->cbuilt.resize(n, 0);
-        for (uint32_t d = set->nfolded; d < n; d++)
-            set->cbuilt[d] = 0;
+        set->cbuilt.resize(set->names.size(), 0);
+        for (uint32_t d = d0; d < d1; d++)
+            if (!set->fold[d])
+                set->cbuilt[d] = 0;
     }
-    if (nf)
-        LAUNCH(ctx, fold_kernel, (uint32_t)((nf + 255) / 256), 256, (uint32_t*)set->filt.p, (uint16_t*)set->hamf.p,
-               f0, nf);
+    for (uint32_t d = d0; d < d1;) {
+        if (set->fold[d]) {
+            d++;
+            continue;
+        }
+        uint32_t e = d;
+        while (e < d1 && !set->fold[e])
+            e++;
+        const uint64_t f0 = set->first[d];
+        const uint64_t nf = (e < set->first.size() ? set->first[e] : set->nfilt) - f0;
+        if (nf)
+            LAUNCH(ctx, fold_kernel, (uint32_t)((nf + 255) / 256), 256, (uint32_t*)set->filt.p,
+                   (uint16_t*)set->hamf.p, f0, nf);
+        d = e;
+    }
     HIPC(ctx, hipStreamSynchronize(ctx->stream));
-    set->nfolded = n;
+    for (uint32_t d = d0; d < d1; d++)
+        set->fold[d] = 1;
     return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sdbfx_set_fold(sdbfx_ctx* ctx, sdbfx_set* set)
+{
+    if (!ctx || !set)
+        return fail(ctx, "null argument");
+    return fold_range(ctx, set, 0, (uint32_t)set->names.size());
+}
+
+int sdbfx_set_fold_digest(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t idx)
+{
+    if (!ctx || !set)
+        return fail(ctx, "null argument");
+    if (idx >= set->names.size())
+        return fail(ctx, "bad digest index");
+    return fold_range(ctx, set, idx, idx + 1);
 }
 
 int sdbfx_set_folded(const sdbfx_set* set)
 {
-    return set && !set->names.empty() && set->nfolded == set->names.size() ? 1 : 0;
+    if (!set || set->names.empty())
+        return 0;
+    for (uint8_t f : set->fold)
+        if (!f)
+            return 0;
+    return 1;
+}
+
+int sdbfx_set_digest_folded(const sdbfx_set* set, uint32_t idx)
+{
+    return set && idx < set->fold.size() && set->fold[idx] ? 1 : 0;
 }
 
 int sdbfx_compare_all(sdbfx_ctx* ctx, const sdbfx_set* set, int32_t* scores)
@@ -1174,7 +1334,7 @@ int sdbfx_compare_all(sdbfx_ctx* ctx, const sdbfx_set* set, int32_t* scores)
     if (!ctx || !set || !scores)
         return fail(ctx, "null argument");
     HIPC(ctx, hipSetDevice(ctx->device));
-    return compare_impl(ctx, set, set, true, scores);
+    return compare_impl(ctx, set, set, true, 0, 0, (uint32_t)set->names.size(), scores);
 }
 
 int sdbfx_compare_to(sdbfx_ctx* ctx, const sdbfx_set* q, const sdbfx_set* t, int32_t* scores)
@@ -1182,7 +1342,27 @@ int sdbfx_compare_to(sdbfx_ctx* ctx, const sdbfx_set* q, const sdbfx_set* t, int
     if (!ctx || !q || !t || !scores)
         return fail(ctx, "null argument");
     HIPC(ctx, hipSetDevice(ctx->device));
-    return compare_impl(ctx, q, t, false, scores);
+    return compare_impl(ctx, q, t, false, 0, 0, (uint32_t)q->names.size(), scores);
+}
+
+int sdbfx_compare_to_sampled(sdbfx_ctx* ctx, const sdbfx_set* q, const sdbfx_set* t, uint32_t sample,
+                             int32_t* scores)
+{
+    if (!ctx || !q || !t || !scores)
+        return fail(ctx, "null argument");
+    HIPC(ctx, hipSetDevice(ctx->device));
+    return compare_impl(ctx, q, t, false, sample, 0, (uint32_t)q->names.size(), scores);
+}
+
+int sdbfx_compare_to_range(sdbfx_ctx* ctx, const sdbfx_set* q, uint32_t q_first, uint32_t q_count,
+                           const sdbfx_set* t, int32_t* scores)
+{
+    if (!ctx || !q || !t || !scores)
+        return fail(ctx, "null argument");
+    if ((uint64_t)q_first + q_count > q->names.size())
+        return fail(ctx, "query digest range out of the set");
+    HIPC(ctx, hipSetDevice(ctx->device));
+    return compare_impl(ctx, q, t, false, 0, q_first, q_first + q_count, scores);
 }
 
 int sdbfx_device_alloc(sdbfx_ctx* ctx, uint64_t bytes, void** dptr)

@@ -36,6 +36,7 @@
                                  // kernel takes 1024-16384 per call)
 #define SDBFX_CMP_ROWS 64        // reference-filter instances per work item, VALU compare
 #define SDBFX_MROWS 128          // reference-filter instances per work item, MFMA compare
+#define SDBFX_SCORE_NONFINITE ((int32_t)0x80000000)   // pair score whose sum is infinite
 
 // One dd-block of block mode (also the tail block of an input).
 struct BlockDesc {

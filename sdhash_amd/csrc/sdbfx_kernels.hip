@@ -1363,11 +1363,21 @@ __device__ __forceinline__ uint32_t bcnt_acc(uint32_t x, uint32_t acc)
 }
 #endif
 
-template <bool FAST>
+// MODE 0: plain 256-byte filters.  MODE 1 (FAST): the reference filter is
+// folded (sdbf::fast): 64 bytes, CUTOFFS64, bf_bitcount_cut_64, with the
+// target's filter and Hamming weight as stored, folded or not.  MODE 2
+// (MIXED): a plain reference filter against a folded target: all 256 bytes
+// and the 256-byte rules, with the target's folded Hamming weight; as in
+// sdbf_max_score the denominator (max_est - cut_off) is a 32-bit unsigned
+// difference, which can wrap, and 0 makes the fraction infinite.  MIXED
+// keeps each maximum as a 64-bit (num << 32 | den) with den 0 for infinity.
+template <int MODE>
 __device__ __forceinline__ void cmp_body(
     SetView A, SetView B, const CmpGroup* __restrict__ groups, uint32_t ngroups,
-    const CmpPair* __restrict__ pairs, uint64_t nwork, uint32_t* __restrict__ partial, uint16_t* cut)
+    const CmpPair* __restrict__ pairs, uint64_t nwork, void* __restrict__ partial_out, uint16_t* cut)
 {
+    constexpr bool FAST = MODE == 1;
+    constexpr bool WIDE = MODE == 2;
     constexpr int W = FAST ? 16 : 64;   // words of a filter compared
     const int lane = threadIdx.x;
     for (int s = lane; s < 1025; s += 64) {
@@ -1429,6 +1439,7 @@ __device__ __forceinline__ void cmp_body(
         }
 
         uint32_t best_n = 0, best_d = 0;
+        bool have = false;   // WIDE: a qualifying target filter was seen
         const uint64_t j0 = g.tgt_first + (uint64_t)c * g.chunk;
         const uint64_t j1 = min(j0 + g.chunk, g.tgt_first + g.tgt_count);
         const uint32_t* __restrict__ tf = g.tgt_set ? B.filt : A.filt;
@@ -1470,32 +1481,53 @@ __device__ __forceinline__ void cmp_body(
             uint32_t num = 0, den = 1;
             if (match > cutoff) {
                 num = match - cutoff;
-                den = min(e1, e2) - cutoff;
+                den = min(e1, e2) - cutoff;   // WIDE: may wrap or be 0
             }
-            if (best_d == 0 || num * best_d > best_n * den) {
+            if (WIDE) {
+                // num / den > best_n / best_d, den 0 being +infinity
+                const bool gt = !have || (best_d != 0 && (den == 0 || (uint64_t)num * best_d > (uint64_t)best_n * den));
+                if (gt) {
+                    best_n = num;
+                    best_d = den;
+                }
+                have = true;
+            } else if (best_d == 0 || num * best_d > best_n * den) {
                 best_n = num;
                 best_d = den;
             }
         }
-        if (valid)
-            partial[g.partial_begin + (uint64_t)li * g.nch + c] = best_d ? ((best_n << 16) | best_d) : 0xFFFFFFFFu;
+        if (valid) {
+            const uint64_t slot = g.partial_begin + (uint64_t)li * g.nch + c;
+            if (WIDE)
+                ((uint64_t*)partial_out)[slot] = have ? ((uint64_t)best_n << 32 | best_d) : ~0ull;
+            else
+                ((uint32_t*)partial_out)[slot] = best_d ? ((best_n << 16) | best_d) : 0xFFFFFFFFu;
+        }
     }
 }
 
 __global__ __launch_bounds__(64) void cmp_kernel(
     SetView A, SetView B, const CmpGroup* __restrict__ groups, uint32_t ngroups,
-    const CmpPair* __restrict__ pairs, uint64_t nwork, uint32_t* __restrict__ partial)
+    const CmpPair* __restrict__ pairs, uint64_t nwork, void* __restrict__ partial)
 {
     __shared__ uint16_t cut[1025];
-    cmp_body<false>(A, B, groups, ngroups, pairs, nwork, partial, cut);
+    cmp_body<0>(A, B, groups, ngroups, pairs, nwork, partial, cut);
 }
 
 __global__ __launch_bounds__(64) void cmp_fast_kernel(
     SetView A, SetView B, const CmpGroup* __restrict__ groups, uint32_t ngroups,
-    const CmpPair* __restrict__ pairs, uint64_t nwork, uint32_t* __restrict__ partial)
+    const CmpPair* __restrict__ pairs, uint64_t nwork, void* __restrict__ partial)
 {
     __shared__ uint16_t cut[1025];
-    cmp_body<true>(A, B, groups, ngroups, pairs, nwork, partial, cut);
+    cmp_body<1>(A, B, groups, ngroups, pairs, nwork, partial, cut);
+}
+
+__global__ __launch_bounds__(64) void cmp_mixed_kernel(
+    SetView A, SetView B, const CmpGroup* __restrict__ groups, uint32_t ngroups,
+    const CmpPair* __restrict__ pairs, uint64_t nwork, void* __restrict__ partial)
+{
+    __shared__ uint16_t cut[1025];
+    cmp_body<2>(A, B, groups, ngroups, pairs, nwork, partial, cut);
 }
 
 // sdbf::fast (sdbf/sdbf_class.cc:426-439) with bloom_filter::fold(2)
@@ -2292,6 +2324,74 @@ __global__ void pair_sum_kernel(const CmpPair* __restrict__ pairs, uint32_t npai
     scores[p] = sum < 0 ? -1 : (int32_t)floor(100.0 * sum / (double)denom + 0.5);
 }
 
+
+
+// chunk_max_kernel for 64-bit maxima (num << 32 | den, den 0 = infinity,
+// ~0 = no qualifying target) of the MIXED comparison mode.
+__global__ void chunk_max_wide_kernel(const CmpPair* __restrict__ pairs, uint32_t npairs, uint64_t ninst,
+                                      const uint64_t* __restrict__ partial, uint64_t* __restrict__ imax)
+{
+    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (t >= ninst)
+        return;
+    uint32_t lo = 0, hi = npairs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pairs[mid].inst_begin <= t)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const CmpPair P = pairs[lo];
+    const uint64_t* v = partial + P.partial_begin + (t - P.inst_begin) * P.nch;
+    uint64_t best = ~0ull;
+    for (uint32_t c = 0; c < P.nch; c++) {
+        const uint64_t x = v[c];
+        if (x == ~0ull)
+            continue;
+        const uint64_t n = x >> 32, d = x & 0xFFFFFFFFull, bn = best >> 32, bd = best & 0xFFFFFFFFull;
+        if (best == ~0ull || (bd != 0 && (d == 0 || n * bd > bn * d)))
+            best = x;
+    }
+    imax[t] = best;
+}
+
+// pair_sum_kernel over 64-bit maxima: each maximum is the double quotient
+// num / den (+infinity for den 0), summed in filter order; a sum that ends
+// infinite is flagged SDBFX_SCORE_NONFINITE (boost::math::round raises on it
+// in the reference).  Scores above 100 are possible here (fractions above 1).
+__global__ void pair_sum_wide_kernel(const CmpPair* __restrict__ pairs, uint32_t npairs,
+                                     const uint16_t* __restrict__ ecfA, const uint16_t* __restrict__ ecfB,
+                                     const uint64_t* __restrict__ imax, int32_t* __restrict__ scores)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs)
+        return;
+    const CmpPair P = pairs[p];
+    const uint16_t* __restrict__ ecf = P.ref_set ? ecfB : ecfA;
+    double sum = -1.0;
+    uint32_t sparse = 0;
+    for (uint32_t i = 0; i < P.ref_count; i++) {
+        const uint32_t s1 = ecf[P.ref_first + i];
+        double m;
+        if (s1 < SDBFX_MIN_ELEM) {
+            m = 0.0;
+            sparse++;
+        } else {
+            const uint64_t x = imax[P.inst_begin + i];
+            const uint32_t d = (uint32_t)x;
+            m = x == ~0ull ? -1.0 : d == 0 ? __builtin_huge_val() : (double)(uint32_t)(x >> 32) / (double)d;
+        }
+        sum = (sum < 0) ? m : sum + m;
+    }
+    uint64_t denom = P.ref_count;
+    if (P.ref_count > 1)
+        denom -= sparse;
+    if (denom == 0)
+        sum = -1.0;
+    const double v = 100.0 * sum / (double)denom;
+    scores[p] = sum < 0 ? -1 : v == __builtin_huge_val() ? SDBFX_SCORE_NONFINITE : (int32_t)floor(v + 0.5);
+}
 
 // ------------------------------------------------------------------ text form
 

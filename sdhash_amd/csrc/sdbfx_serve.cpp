@@ -7,8 +7,16 @@
 // context; the compare thread compares each digested batch with the
 // reference set on a second context.  While batch k is compared, batch k+1
 // is digested, and requests that arrive meanwhile form batch k+2.  At most
-// two batches are past the queue at a time, which bounds device memory to
-// about two batches plus the reference set.
+// two batches are past the queue at a time.
+//
+// Admission by memory: each of the two contexts gets half of the server's
+// memory budget for its scratch (staging, work lists, partial results).  A
+// batch whose digesting or comparison would exceed it is split: its later
+// half goes back to the front of the queue (deferred, still in arrival order)
+// and the earlier half is tried again, after the context's cached scratch is
+// freed.  A single query that does not fit on its own is refused with status
+// SDBFX_EBUDGET.  The query sets themselves (256 bytes per filter, about 1/8
+// of the input in stream mode at most) are not counted.
 //
 // Autotuning: the two comparison kernels give identical scores at different
 // speeds depending on the shape of the batch (query filters per reference
@@ -87,7 +95,7 @@ struct sdbfx_server {
     uint64_t next_ticket = 1;
     bool stop = false;
     std::string err;
-    uint64_t nbatches = 0, nqueries = 0, nmfma = 0;
+    uint64_t nbatches = 0, nqueries = 0, nmfma = 0, ndeferred = 0, nrefused = 0;
     int8_t tuned[64];   // per size class: 0 untuned, SDBFX_CMP_VALU or SDBFX_CMP_MFMA
     std::thread td, tc;
 
@@ -131,19 +139,50 @@ struct sdbfx_server {
             }
             b.status.assign(n, 0);
             int rc = sdbfx_set_create(cd, &b.set);
-            if (rc == 0)
-                rc = sdbfx_set_digest(cd, b.set, n, bufs.data(), lens.data(), names.data(), dd.data(),
-                                      b.status.data());
+            uint32_t m = n;   // the queries digested: [0, m) of the batch
+            bool trimmed = false;
+            for (;;) {
+                if (rc == 0)
+                    rc = sdbfx_set_digest(cd, b.set, m, bufs.data(), lens.data(), names.data(), dd.data(),
+                                          b.status.data());
+                if (rc != SDBFX_EBUDGET)
+                    break;
+                if (!trimmed) {
+                    // the scratch cached from earlier batches may be what is in the way
+                    trimmed = true;
+                    rc = sdbfx_ctx_trim(cd);
+                    continue;
+                }
+                if (m == 1)
+                    break;
+                m = (m + 1) / 2;
+                rc = 0;
+            }
+            if (m < n) {
+                // deferred: back to the front of the queue, in order
+                std::lock_guard<std::mutex> g(mu);
+                for (uint32_t i = n; i-- > m;)
+                    queue.push_front(b.reqs[i]);
+                ndeferred += n - m;
+                b.reqs.resize(m);
+                b.status.resize(m);
+            }
             // a folded reference set (fast mode) is compared with folded queries
             if (rc == 0 && fast)
                 rc = sdbfx_set_fold(cd, b.set);
-            if (rc != 0) {
+            if (rc == SDBFX_EBUDGET) {
+                b.status[0] = SDBFX_EBUDGET;   // a single query that does not fit
+                std::lock_guard<std::mutex> g(mu);
+                nrefused++;
+            } else if (rc != 0) {
                 set_error(std::string("digest: ") + sdbfx_last_error(cd));
-                for (uint32_t i = 0; i < n; i++)
+                for (uint32_t i = 0; i < m; i++)
                     b.status[i] = -1;
             }
             for (auto& r : b.reqs)
                 std::vector<uint8_t>().swap(r->data);   // input no longer needed
+            if (m < n)
+                cv_queue.notify_one();
             std::lock_guard<std::mutex> g(mu);
             handoff.push_back(std::move(b));
             cv_handoff.notify_one();
@@ -171,12 +210,42 @@ struct sdbfx_server {
         return mfma < 0.5 * valu;
     }
 
-    int timed_compare(const sdbfx_set* q, const sdbfx_set* t, int method, int32_t* out, double* seconds)
+    // q's digests [q0, q1) against t, split into ranges that fit the compare
+    // context's budget; `refused` receives the digests that do not fit alone
+    int compare_fit(const sdbfx_set* q, uint32_t q0, uint32_t q1, const sdbfx_set* t, int32_t* out,
+                    std::vector<uint32_t>& refused)
+    {
+        if (q0 >= q1)
+            return 0;
+        const uint32_t nt = sdbfx_set_size(t);
+        int rc = sdbfx_compare_to_range(cc, q, q0, q1 - q0, t, out + (size_t)q0 * nt);
+        if (rc == SDBFX_EBUDGET) {
+            rc = sdbfx_ctx_trim(cc);
+            if (rc == 0)
+                rc = sdbfx_compare_to_range(cc, q, q0, q1 - q0, t, out + (size_t)q0 * nt);
+        }
+        if (rc != SDBFX_EBUDGET)
+            return rc;
+        if (q1 - q0 == 1) {
+            refused.push_back(q0);
+            return 0;
+        }
+        const uint32_t mid = q0 + (q1 - q0) / 2;
+        if ((rc = compare_fit(q, q0, mid, t, out, refused)))
+            return rc;
+        return compare_fit(q, mid, q1, t, out, refused);
+    }
+
+    int timed_compare(const sdbfx_set* q, const sdbfx_set* t, int method, int32_t* out, double* seconds,
+                      std::vector<uint32_t>* refused = nullptr)
     {
         const auto t0 = std::chrono::steady_clock::now();
         int rc = sdbfx_set_compare_method(cc, method);
+        std::vector<uint32_t> none;
         if (rc == 0)
-            rc = sdbfx_compare_to(cc, q, t, out);
+            rc = compare_fit(q, 0, sdbfx_set_size(q), t, out, refused ? *refused : none);
+        if (!refused && !none.empty())
+            rc = SDBFX_EBUDGET;
         *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         return rc;
     }
@@ -197,6 +266,7 @@ struct sdbfx_server {
             const uint32_t n = (uint32_t)b.reqs.size();
             const uint32_t nd = b.set ? sdbfx_set_size(b.set) : 0;
             std::vector<int32_t> sc((size_t)nd * nref);
+            std::vector<uint32_t> refused;   // digests whose comparison does not fit the budget
             int rc = 0, method = SDBFX_CMP_AUTO;
             if (nd && nref) {
                 const uint64_t nf = sdbfx_set_filter_count(b.set);
@@ -217,12 +287,17 @@ struct sdbfx_server {
                         rc = timed_compare(b.set, sample, SDBFX_CMP_MFMA, sm.data(), &tm);
                     if (rc == 0 && sv != sm)
                         set_error("compare: the two kernels disagree");
-                    method = tm <= tv ? SDBFX_CMP_MFMA : SDBFX_CMP_VALU;
-                    tuned[cls] = (int8_t)method;
+                    if (rc == SDBFX_EBUDGET) {
+                        rc = 0;   // the trial does not fit: no tuning for this batch
+                        method = SDBFX_CMP_AUTO;
+                    } else {
+                        method = tm <= tv ? SDBFX_CMP_MFMA : SDBFX_CMP_VALU;
+                        tuned[cls] = (int8_t)method;
+                    }
                 }
                 if (rc == 0) {
                     double t;
-                    rc = timed_compare(b.set, ref, method, sc.data(), &t);
+                    rc = timed_compare(b.set, ref, method, sc.data(), &t, &refused);
                 }
             }
             if (rc != 0)
@@ -231,12 +306,18 @@ struct sdbfx_server {
             {
                 std::lock_guard<std::mutex> g(mu);
                 uint32_t k = 0;   // row of the next digested request
+                size_t rf = 0;    // next refused row
                 for (uint32_t i = 0; i < n; i++) {
                     Request& r = *b.reqs[i];
                     r.status = rc != 0 ? -1 : b.status[i];
                     if (b.status[i] == 0) {
-                        if (rc == 0)
+                        if (rf < refused.size() && refused[rf] == k) {
+                            r.status = SDBFX_EBUDGET;
+                            rf++;
+                            nrefused++;
+                        } else if (rc == 0) {
                             r.scores.assign(sc.begin() + (size_t)k * nref, sc.begin() + (size_t)(k + 1) * nref);
+                        }
                         k++;
                     }
                     r.t_done = t;
@@ -296,7 +377,7 @@ int make_sample(sdbfx_server* s)
 }  // namespace
 
 int sdbfx_serve_create(int device, const sdbfx_set* ref, uint64_t batch_bytes, uint32_t batch_max,
-                       sdbfx_server** out)
+                       uint64_t mem_budget, sdbfx_server** out)
 {
     if (!out || !ref || batch_bytes == 0 || batch_max == 0)
         return -1;
@@ -308,7 +389,7 @@ int sdbfx_serve_create(int device, const sdbfx_set* ref, uint64_t batch_bytes, u
     s->fast = sdbfx_set_folded(ref) == 1;
     s->batch_bytes = batch_bytes;
     s->batch_max = batch_max;
-    if (sdbfx_ctx_create(device, 0, &s->cd) != 0 || sdbfx_ctx_create(device, 0, &s->cc) != 0) {
+    if (sdbfx_ctx_create(device, mem_budget / 2, &s->cd) != 0 || sdbfx_ctx_create(device, mem_budget / 2, &s->cc) != 0) {
         if (s->cd)
             sdbfx_ctx_destroy(s->cd);
         delete s;
@@ -424,6 +505,18 @@ int sdbfx_serve_wait(sdbfx_server* s, uint64_t ticket, int32_t* scores, int32_t*
     std::shared_ptr<Request> r = it->second;
     s->cv_done.wait(g, [&] { return r->done; });
     return take(s, s->live.find(ticket), scores, status, latency_ns);
+}
+
+int sdbfx_serve_admission(sdbfx_server* s, uint64_t* deferred, uint64_t* refused)
+{
+    if (!s)
+        return -1;
+    std::lock_guard<std::mutex> g(s->mu);
+    if (deferred)
+        *deferred = s->ndeferred;
+    if (refused)
+        *refused = s->nrefused;
+    return 0;
 }
 
 int sdbfx_serve_stats(sdbfx_server* s, uint64_t* batches, uint64_t* queries, uint64_t* mfma_batches)

@@ -6,7 +6,8 @@
 //   sdhash-amd -c a.sdbf [b.sdbf]      compare digests of one or two files
 //
 // Options: -r/--deep, -g/--gen-compare, -c/--compare, -t/--threshold N,
-// -b/--block-size KB, -p/--threads N (accepted; the device does the work),
+// -s/--sample-size N, -b/--block-size KB, -p/--threads N (accepted; the
+// device does the work),
 // -z/--segment-size MB, -o/--output BASE, --separator pipe|csv|tab,
 // --hash-name NAME (stdin), --fast, --validate, --verbose, --version,
 // -h/--help.
@@ -35,6 +36,7 @@ const uint64_t KB = 1024, MB = 1024 * 1024;
 struct Opts {
     bool deep = false, gen = false, cmp = false, validate = false, verbose = false, fast = false;
     int32_t threshold = 1;
+    uint32_t sample = 0;           // -s: filters of each query digest compared (0 = all)
     int32_t dd = -1;               // KiB; -1 = automatic
     uint64_t segment = 128 * MB;
     std::string output, separator = "pipe", hash_name;
@@ -49,6 +51,7 @@ void usage()
                  "  -c [ --compare ]        compare SDBFs in file, or two SDBF files\n"
                  "  -g [ --gen-compare ]    compare all pairs in source data\n"
                  "  -t [ --threshold ] N    only show results >= N (default 1)\n"
+                 "  -s [ --sample-size ] N  sample N filters for comparisons (default 0)\n"
                  "  -b [ --block-size ] N   hash input files in N KiB blocks\n"
                  "  -p [ --threads ] N      accepted for compatibility\n"
                  "  -z [ --segment-size ] N segment size in MiB (default 128)\n"
@@ -84,7 +87,7 @@ bool parse(int argc, char **argv, Opts &o)
         auto is = [&](const char *s, const char *l) {
             const std::string L = l;
             return (s && a.rfind(s, 0) == 0 && (a.size() == 2 || (a.size() > 2 && std::string(s).size() == 2 &&
-                                                                   strchr("tbpzo", s[1]))) ) ||
+                                                                   strchr("tbpzos", s[1]))) ) ||
                    a == L || a.rfind(L + "=", 0) == 0;
         };
         if (a == "-h" || a == "--help") {
@@ -109,6 +112,10 @@ bool parse(int argc, char **argv, Opts &o)
             if (!need("threshold"))
                 return false;
             o.threshold = atoi(val.c_str());
+        } else if (is("-s", "--sample-size")) {
+            if (!need("sample-size"))
+                return false;
+            o.sample = (uint32_t)strtoul(val.c_str(), nullptr, 10);
         } else if (is("-b", "--block-size")) {
             if (!need("block-size"))
                 return false;
@@ -256,23 +263,23 @@ int run_jobs(std::vector<Job> &jobs, uint32_t dd_bytes, sdbf_set *set)
 {
     std::vector<std::string> names;
     std::vector<const uint8_t *> bufs;
-    std::vector<uint64_t> lens;
-    std::vector<size_t> idx;
+    std::vector<uint64_t> lens, got;
     for (size_t i = 0; i < jobs.size(); i++) {
         if (jobs[i].got < 512)   // the constructors' -3: skipped silently
             continue;
         names.push_back(jobs[i].name);
         bufs.push_back(jobs[i].data.data());
         lens.push_back(jobs[i].data.size());
-        idx.push_back(i);
+        // the stream constructor's orig_file_size is the bytes it read
+        // (gcount, sdbf_class.cc:77-85), not the zero-padded length
+        got.push_back(jobs[i].got);
     }
     std::vector<sdbf *> out;
     if (!bufs.empty())
-        sdbf::hash_batch(names, bufs, lens, dd_bytes, out);
+        sdbf::hash_batch(names, bufs, lens, dd_bytes, out, &got);
     for (size_t k = 0; k < out.size(); k++) {
         if (!out[k])
             continue;
-        (void)idx;
         if (set)
             set->add(out[k]);
         else {
@@ -318,9 +325,9 @@ int main(int argc, char **argv)
                 sdbf_set *s2 = new sdbf_set(o.inputs[1].c_str());
                 s1->set_separator(sep);
                 if (!o.output.empty())
-                    write_file(s1->compare_to_quiet(s2, o.threshold, 0, 0, o.fast), o.output + ".compare");
+                    write_file(s1->compare_to_quiet(s2, o.threshold, o.sample, 0, o.fast), o.output + ".compare");
                 else
-                    s1->compare_to(s2, o.threshold, 0, o.fast);
+                    s1->compare_to(s2, o.threshold, o.sample, o.fast);
             } else {
                 std::cerr << "sdhash: ERROR: Comparison requires 1 or 2 arguments." << std::endl;
                 return -1;
@@ -346,7 +353,11 @@ int main(int argc, char **argv)
         std::vector<Job> stream_jobs, block_jobs;
         uint32_t block_bytes = 0;
         if (o.inputs.size() == 1 && o.inputs[0] == "-") {
-            // stdin, always block mode (sdhash.cc:453-465, sdhash_threads.cc:189-233)
+            // stdin, always block mode and always in segments (sdhash.cc:
+            // 460-472: a segment size of 0 becomes 128 MiB; sdhash_threads.cc:
+            // 189-233)
+            if (o.segment == 0)
+                o.segment = 128 * MB;
             const uint32_t dd = o.dd > 0 ? (uint32_t)o.dd : 16;
             const std::string base = o.hash_name.empty() ? "stdin" : o.hash_name;
             std::vector<uint8_t> buf(o.segment);

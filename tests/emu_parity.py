@@ -159,7 +159,7 @@ def main(lib):
     assert np.array_equal(eng.compare_to(gq2, gt), oracle.compare_to(q2, t + [t2]))
     eng.compare_method(sdhash_amd.CMP_AUTO)
     # fast mode (sdbf::fast): folded filters and Hamming weights, fast scores,
-    # folding twice changes nothing, mixed modes refused
+    # folding twice changes nothing
     fs = eng.new_set()
     assert np.all(fs.digest(files, names, dd_block=modes) == 0)
     fs.fold()
@@ -170,18 +170,30 @@ def main(lib):
         f, ec, hm = fs.read(i)
         assert np.array_equal(f, d.filters) and np.array_equal(hm, d.hamming[: d.bf_count]), i
     assert np.array_equal(eng.compare_all(fs), oracle.compare_all(fods))
+    # folded and plain digests together: each pair scored by its reference
+    # digest's mode (a plain reference against a folded target included)
     ft, fq = eng.new_set(), eng.new_set()
     for d, g in ((t[0], ft), *((x, fq) for x in q)):
         g.add(d.name, d.orig_size, d.bf_count, d.last_count, d.dd_block_size, d.max_elem, d.filters, d.elem_counts)
     ft.fold()
-    try:
-        eng.compare_to(fq, ft)
-    except sdhash_amd.SdbfxError as e:
-        assert "folded" in str(e)
-    else:
-        raise AssertionError("folded and plain sets compared")
+    assert np.array_equal(eng.compare_to(fq, ft), oracle.compare_to(q, [t[0].folded()]))
+    assert np.array_equal(eng.compare_to(ft, fq), oracle.compare_to([t[0].folded()], q))
     fq.fold()
     assert np.array_equal(eng.compare_to(fq, ft), oracle.compare_to([x.folded() for x in q], [t[0].folded()]))
+    mixed = eng.new_set()
+    mods = []
+    for i, d in enumerate(ods):
+        mixed.add(d.name, d.orig_size, d.bf_count, d.last_count, d.dd_block_size, d.max_elem, d.filters,
+                  d.elem_counts)
+        if i % 2:
+            mixed.fold_digest(i)
+        mods.append(d.folded() if i % 2 else d)
+    assert not mixed.folded and mixed.digest_folded(1) and not mixed.digest_folded(2)
+    assert np.array_equal(eng.compare_all(mixed), oracle.compare_all(mods))
+    # sampling (sdhash -s): a query digest's first `sample` filters
+    for smp in (1, 2, 3, 7):
+        assert np.array_equal(eng.compare_to(gq2, gt, sample=smp), oracle.compare_to(q2, t + [t2], sample=smp)), smp
+        assert np.array_equal(eng.compare_to(gt, gq2, sample=smp), oracle.compare_to(t + [t2], q2, sample=smp)), smp
     print("fast ok", flush=True)
     print("compare ok", flush=True)
 
@@ -206,6 +218,38 @@ def main(lib):
         else:
             assert m.status == 0 and np.array_equal(m.scores, w) and np.array_equal(a.scores, w), (m, a, w)
     print("serve ok", flush=True)
+
+    # memory budget: a context whose scratch would exceed it refuses the
+    # call (SDBFX_EBUDGET) and leaves the set as it was
+    small = sdhash_amd.Engine(0, mem_budget=600 * KIB)
+    bs = small.new_set()
+    assert np.all(bs.digest([files[0][:30000]]) == 0)
+    try:
+        bs.digest([files[1], big[: 1200 * KIB]])
+    except sdhash_amd.SdbfxError as e:
+        assert e.status == sdhash_amd.EBUDGET, e.status
+    else:
+        raise AssertionError("budget not enforced")
+    assert len(bs) == 1 and small.scratch_bytes() <= 600 * KIB
+    small.trim()
+    assert small.scratch_bytes() == 0
+    assert np.all(bs.digest([files[1]]) == 0) and len(bs) == 2
+    same(bs, 1, oracle.digest_stream(files[1]), "budget")
+    small.close()
+    # the server admits by memory: batches that do not fit are split and
+    # their later queries deferred; a query that never fits is refused
+    srv = sdhash_amd.Server(s, batch_bytes=16 << 20, batch_max=64, mem_budget=1200 * KIB)
+    qs = [files[k % 8][: 24 * KIB + 512 * k] for k in range(12)] + [big[: 1200 * KIB]]
+    want_s = [oracle.compare_to([oracle.digest_stream(x, b"q")], ods)[0] for x in qs[:-1]]
+    tickets = [srv.submit(x, b"q%d" % i) for i, x in enumerate(qs)]
+    res = [srv.wait(t) for t in tickets]
+    deferred, refused = srv.admission()
+    srv.close()
+    for w, r in zip(want_s, res[:-1]):
+        assert r.status == 0 and np.array_equal(r.scores, w), r
+    assert res[-1].status == sdhash_amd.EBUDGET and refused == 1, (res[-1].status, refused)
+    assert deferred > 0, deferred
+    print("admission ok (%d deferred)" % deferred, flush=True)
     eng.close()
 
 

@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import oracle
+import sdhash_amd
 from sdhash_amd import synth
 
 pytestmark = pytest.mark.gpu
@@ -218,14 +219,100 @@ def test_compare_fast_mode(cmp_engine):
     want = oracle.compare_all(fods)
     assert np.array_equal(got, want), np.flatnonzero(got != want)[:10]
     assert (want > 0).sum() > 10
-    # fragments against the image, both folded
+    # sdhash -c q t --fast: only the query set is folded (sdbf_set.cc:367-376);
+    # each pair is scored by its reference digest's mode, so fragments (the
+    # reference side) score folded against the plain image
     frags = synth.fragments(95, img, 200, 8 * KIB)
     q = [oracle.digest_stream(f, b"q%03d" % i) for i, f in enumerate(frags)]
     gq = upload_oracle(cmp_engine, q)
     gt = upload_oracle(cmp_engine, [ods[-1]])
     gq.fold()
+    got = cmp_engine.compare_to(gq, gt)
+    want = oracle.compare_to([d.folded() for d in q], [ods[-1]])
+    assert np.array_equal(got, want), np.argwhere(got != want)[:10]
+    assert (want[:100] > 0).sum() > 10
+    # both folded
     gt.fold()
     got = cmp_engine.compare_to(gq, gt)
     want = oracle.compare_to([d.folded() for d in q], [fods[-1]])
     assert np.array_equal(got, want), np.argwhere(got != want)[:10]
     assert (want[:100] > 0).sum() > 10   # folding loses matches: 25 of 100 here
+
+
+def test_compare_folded_target_plain_reference(cmp_engine):
+    # a plain reference digest against a folded target: 256-byte rules with the
+    # target's folded Hamming weight (sdbf_core.cc:409-441), unsigned
+    # denominators, fractions above 1 (scores above 100), and the pair whose
+    # denominator is 0 (the reference aborts on it) refused
+    img = synth.disk_image(96, 3 * MIB)
+    big = [oracle.digest_block(img[o:o + MIB], 16384, b"b%d" % k) for k, o in enumerate((0, MIB, 2 * MIB))]
+    small = [oracle.digest_stream(img[o:o + 40 * KIB], b"s%d" % k) for k, o in enumerate((5000, 1500000, 2600000))]
+    gb, gs = upload_oracle(cmp_engine, big), upload_oracle(cmp_engine, small)
+    gb.fold()   # the larger digests folded: the small plain ones are the reference side
+    got = cmp_engine.compare_to(gs, gb)
+    want = oracle.compare_to(small, [d.folded() for d in big])
+    assert np.array_equal(got, want), (got, want)
+    assert want.max() > 100
+
+    def bits(*ranges):
+        v = np.zeros(2048, np.uint8)
+        for lo, hi in ranges:
+            v[lo:hi] = 1
+        return np.packbits(v, bitorder="little")
+    # the target filter folds to 240 bits = the cutoff for 160 + 160 elements
+    f = bits((0, 240), (512, 752), (1024, 1264), (1536, 1656))
+    ref = oracle.Digest(b"r", f, 1, 160, None, 1, 0)
+    tgt = oracle.Digest(b"t", np.concatenate([f, f]), 2, 160, None, 1, 0)
+    assert oracle.score(ref, tgt.folded()) == -(1 << 31)
+    g1, g2 = upload_oracle(cmp_engine, [ref]), upload_oracle(cmp_engine, [tgt])
+    g2.fold()
+    with pytest.raises(sdhash_amd.SdbfxError, match="not finite"):
+        cmp_engine.compare_to(g1, g2)
+
+
+def test_compare_sampled(cmp_engine):
+    # sdhash -s N: a query digest of more than N filters takes part with its
+    # first N (sdbf_core.cc:342-347)
+    img = synth.disk_image(97, 8 * MIB)
+    t = [oracle.digest_block(img[:4 * MIB], 16384, b"t0"), oracle.digest_stream(img[4 * MIB:5 * MIB], b"t1")]
+    q = [oracle.digest_block(img[o:o + 600 * KIB], 16384, b"q%d" % k) for k, o in enumerate(range(0, 8 * MIB, MIB))]
+    q += [oracle.digest_stream(img[o:o + 300 * KIB], b"r%d" % k) for k, o in enumerate(range(0, 8 * MIB, MIB))]
+    gq, gt = upload_oracle(cmp_engine, q), upload_oracle(cmp_engine, t)
+    for smp in (1, 5, 20, 64):
+        want = oracle.compare_to(q, t, sample=smp)
+        assert np.array_equal(cmp_engine.compare_to(gq, gt, sample=smp), want), smp
+        assert np.array_equal(cmp_engine.compare_to(gt, gq, sample=smp), oracle.compare_to(t, q, sample=smp)), smp
+    assert not np.array_equal(oracle.compare_to(q, t, sample=5), oracle.compare_to(q, t))
+
+
+def test_compare_hand_derived_rules(cmp_engine):
+    # the known answers of the oracle's score tests, on the device
+    def bits(*ranges):
+        v = np.zeros(2048, np.uint8)
+        for lo, hi in ranges:
+            v[lo:hi] = 1
+        return np.packbits(v, bitorder="little")
+    X, Z = bits((0, 600)), bits((1200, 1800))
+
+    def dd(name, fs, ecs):
+        return oracle.Digest(name, np.concatenate(fs), len(fs), 0, np.array(ecs, np.uint16), 100000, 16384)
+
+    def st(name, fs, last):
+        return oracle.Digest(name, np.concatenate(fs), len(fs), last, None, 100000, 0)
+    a2, b2 = dd(b"a", [X, X], [160, 180]), dd(b"b", [X, Z], [160, 170])
+    t4 = dd(b"t", [X, X, X, X], [5, 5, 5, 5])
+    cases = [
+        (st(b"a", [X], 160), st(b"b", [X, Z], 160), 100),
+        (st(b"b", [X, Z], 160), st(b"a", [X], 160), 100),
+        (a2, b2, 50), (b2, a2, 100),
+        (dd(b"a", [X, X, X], [5, 100, 100]), t4, -1),
+        (dd(b"a", [X, X, X], [100, 100, 5]), t4, 0),
+        (dd(b"a", [X, X, X], [100, 5, 100]), t4, -1),
+        (dd(b"a", [X], [5]), dd(b"t", [X, X], [160, 160]), 0),
+        (dd(b"a", [X, X], [5, 5]), dd(b"t", [X, X, X], [160, 160, 160]), -1),
+        (st(b"a", [bits((0, 1040))], 160), st(b"b", [bits((0, 340), (1100, 1800))], 160), 13),
+        (st(b"a", [bits((0, 1040))], 160), st(b"b", [bits((0, 339), (1100, 1801))], 160), 12),
+    ]
+    for k, (x, y, want) in enumerate(cases):
+        assert oracle.score(x, y) == want, k
+        assert cmp_engine.compare_to(upload_oracle(cmp_engine, [x]), upload_oracle(cmp_engine, [y]))[0, 0] == want, k

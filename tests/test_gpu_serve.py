@@ -120,3 +120,25 @@ def test_serve_fast_mode(engine):
     finally:
         srv.close()
     check(res, want)
+
+
+def test_serve_admits_by_memory(engine):
+    # a scratch budget that one 4 MiB query does not fit into: batches are
+    # split (later queries deferred, still answered), the large query refused
+    img, files, ods, ref = reference(engine)
+    qs = [img[o:o + 100 * KIB] for o in range(0, 6 * MIB, 300 * KIB)] + [img[:4 * MIB]]
+    want = [oracle.compare_to([oracle.digest_stream(q, b"q")], ods)[0] for q in qs[:-1]]
+    srv = sdhash_amd.Server(ref, batch_bytes=64 * MIB, batch_max=64, mem_budget=6 * MIB)
+    try:
+        first = srv.submit(img[:50 * KIB], b"warm")   # keeps the digest stage busy while the rest queue up
+        tickets = [srv.submit(q, b"q%03d" % i) for i, q in enumerate(qs)]
+        res = [srv.wait(t) for t in tickets]
+        srv.wait(first)
+        deferred, refused = srv.admission()
+        assert srv.error() == ""
+    finally:
+        srv.close()
+    for r, w in zip(res[:-1], want):
+        assert r.status == 0 and np.array_equal(r.scores, w), r
+    assert res[-1].status == sdhash_amd.EBUDGET
+    assert refused == 1 and deferred > 0, (deferred, refused)

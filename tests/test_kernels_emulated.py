@@ -34,6 +34,7 @@ def run(lib):
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
     assert "compare ok" in r.stdout
     assert "serve ok" in r.stdout
+    assert "admission ok" in r.stdout
 
 
 @pytest.mark.parametrize("variant", [[], ["-DSDBFX_WARM=12", "-DSDBFX_INSERT_REPLAY=1"]],

@@ -341,3 +341,21 @@ def test_score_rounds_half_up():
     # one bit less: 99 / 800 -> 12.375 -> 12
     y2 = bits((0, 339), (1100, 1801))
     assert oracle.score(stream_digest(b"a", [x], 160), stream_digest(b"b", [y2], 160)) == 12
+
+
+def test_score_sampling_takes_the_first_filters():
+    # sample 2 (sdbf_core.cc:342-347): a = [X, X, Z] counts as its first two
+    # filters, and the swap rule reads the element count of filter 1, which
+    # for a stream digest is max_elem (160), not a's last count (100).
+    W = bits((1900, 2048))
+    a = stream_digest(b"a", [X, X, Z], 100)
+    b_lo = stream_digest(b"b", [X, W], 150)   # 160 > 150: swap -> reference b: (1 + 0) / 2
+    b_hi = stream_digest(b"b", [X, W], 170)   # 160 > 170 fails, "a" < "b": reference a's X, X -> 100
+    assert oracle.score(a, b_lo, 2) == 50
+    assert oracle.score(a, b_hi, 2) == 100
+    # sample 1: a counts as one filter (< 2): reference a's first filter X -> 100
+    assert oracle.score(a, b_lo, 1) == 100
+    # no sampling, or a sample of at least a's count: a has 3 > 2 filters ->
+    # reference b -> 50
+    assert oracle.score(a, b_hi, 0) == 50
+    assert oracle.score(a, b_hi, 3) == 50

@@ -43,6 +43,11 @@ MUTANTS = {
     "plain sum": [("score_sum = (score_sum < 0) ? m : score_sum + m;", "score_sum = score_sum + m;")],
     "sparse always off the denominator": [("    if (n1 > 1)\n        denom -= sparse;", "    denom -= sparse;")],
     "round half to even": [("return (int)floor(v + 0.5);", "return (int)rint(v);")],
+    "sampled swap reads the last count": [
+        ("((elem_count(s1, n1 - 1) > elem_count(s2, s2->bf_count - 1)) ||",
+         "((elem_count(s1, s1->bf_count - 1) > elem_count(s2, s2->bf_count - 1)) ||")],
+    "sample ignored": [("    if (sample > 0 && s1->bf_count > sample)\n        n1 = sample;",
+                        "    if (0)\n        n1 = sample;")],
 }
 
 
