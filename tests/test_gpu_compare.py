@@ -230,7 +230,8 @@ def test_compare_fast_mode(cmp_engine):
     got = cmp_engine.compare_to(gq, gt)
     want = oracle.compare_to([d.folded() for d in q], [ods[-1]])
     assert np.array_equal(got, want), np.argwhere(got != want)[:10]
-    assert (want[:100] > 0).sum() > 10
+    # (folded 64 bytes against a plain filter's first 64 bytes: nearly
+    # every score is 0, as in the reference)
     # both folded
     gt.fold()
     got = cmp_engine.compare_to(gq, gt)
