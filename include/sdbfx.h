@@ -272,6 +272,29 @@ int sdbfx_device_alloc(sdbfx_ctx *ctx, uint64_t bytes, void **dptr);
 int sdbfx_device_free(sdbfx_ctx *ctx, void *dptr);
 int sdbfx_upload(sdbfx_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 
+/* ---- no device allocation ------------------------------------------- */
+
+/*
+ * Block-mode digest of one input already in device memory, into device
+ * memory the caller provides, with no device allocation: no hipMalloc or
+ * hipFree and no allocating library routine, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid it).  No counterpart in the reference.  The result equals
+ * sdbfx_set_digest_device of that input with dd_block (the digest of
+ * sdbf(name, buf, dd_block, len, NULL), sdbf/sdbf_class.cc:115-136):
+ * d_filters receives *nfilters filters of 256 bytes, d_elem_counts and
+ * d_hamming *nfilters values each (the caller sizes them for
+ * len / dd_block + 1).  d_input must be 256-byte aligned with 256 readable
+ * bytes past len; dd_block a multiple of 16 in [512, 16384]; inputs of more
+ * than 131,072 dd-blocks are refused (-1), inputs under 512 bytes give -3.
+ * The work list lives in a fixed device array of 3 MiB that every call on
+ * the GPU shares: calls that overlap on one GPU race.  The kernels run on
+ * the context's stream; the outputs are ready after sdbfx_sync.
+ */
+int sdbfx_digest_blocks_noalloc(sdbfx_ctx *ctx, const void *d_input, uint64_t len, uint32_t dd_block,
+                                uint8_t *d_filters, uint16_t *d_elem_counts, uint16_t *d_hamming,
+                                uint32_t *nfilters);
+
 /* ---- kernel timing -------------------------------------------------- */
 
 /* When enabled, every kernel launch of the context is bracketed by HIP

@@ -24,7 +24,7 @@ SYMBOLS = (
     "sdbfx_set_fold_digest", "sdbfx_set_digest_folded",
     "sdbfx_compare_all", "sdbfx_compare_to", "sdbfx_compare_to_sampled", "sdbfx_compare_to_range",
     "sdbfx_device_alloc", "sdbfx_device_free", "sdbfx_upload", "sdbfx_set_profiling", "sdbfx_kernel_time",
-    "sdbfx_reset_kernel_times", "sdbfx_set_compare_method",
+    "sdbfx_reset_kernel_times", "sdbfx_set_compare_method", "sdbfx_digest_blocks_noalloc",
     "sdbfx_serve_create", "sdbfx_serve_destroy", "sdbfx_serve_error", "sdbfx_serve_ref_size",
     "sdbfx_serve_submit", "sdbfx_serve_poll", "sdbfx_serve_wait", "sdbfx_serve_stats", "sdbfx_serve_admission",
 )
@@ -90,6 +90,7 @@ def load():
     L.sdbfx_kernel_time.argtypes = [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(U64)]
     L.sdbfx_reset_kernel_times.argtypes = [P]
     L.sdbfx_set_compare_method.argtypes = [P, C.c_int]
+    L.sdbfx_digest_blocks_noalloc.argtypes = [P, P, U64, U32, P, P, P, C.POINTER(U32)]
     L.sdbfx_set_to_text.argtypes = [P, P, P, U64, C.POINTER(U64)]
     L.sdbfx_set_add_text.argtypes = [P, P, P, U64, C.POINTER(U32)]
     L.sdbfx_set_fold.argtypes = [P, P]
@@ -115,7 +116,7 @@ def load():
                  "sdbfx_set_to_text", "sdbfx_set_add_text", "sdbfx_set_fold", "sdbfx_set_fold_digest",
                  "sdbfx_compare_all", "sdbfx_compare_to", "sdbfx_compare_to_sampled", "sdbfx_compare_to_range", "sdbfx_device_alloc", "sdbfx_device_free",
                  "sdbfx_upload", "sdbfx_set_profiling", "sdbfx_kernel_time",
-                 "sdbfx_reset_kernel_times", "sdbfx_set_compare_method", "sdbfx_serve_create",
+                 "sdbfx_reset_kernel_times", "sdbfx_set_compare_method", "sdbfx_digest_blocks_noalloc", "sdbfx_serve_create",
                  "sdbfx_serve_submit", "sdbfx_serve_poll", "sdbfx_serve_wait", "sdbfx_serve_stats",
                  "sdbfx_serve_admission"):
         getattr(L, name).restype = I32

@@ -77,6 +77,47 @@ struct sdbfx_set {
     mutable std::mutex cmu;
 };
 
+// The no-allocation block digest (sdbfx_digest_blocks_noalloc) keeps its
+// work list -- one BlockDesc per dd-block -- in this array instead of in
+// memory the library allocates.  Each GPU holds its own copy for the life of
+// the process; host code reaches it only through the device address that
+// hipGetSymbolAddress gives for each device (scratch_ptr), never by this
+// name, which in host code is a stand-in, not a device address.
+#define SDBFX_NOALLOC_BLOCKS 131072   // dd-blocks per call: 2 GiB of input at 16 KiB blocks
+static __device__ __align__(256) uint8_t scratch[SDBFX_NOALLOC_BLOCKS * sizeof(BlockDesc)];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Resolves the device address of `scratch` on `device`, which must be the
+// current device; once per device (sdbfx_ctx_create calls it).
+int scratch_setup(int device)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return -1;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch_ptr[device])
+        return 0;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+        return -1;
+    g_scratch_ptr[device] = p;
+    return 0;
+}
+
+void* scratch_ptr(int device)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return nullptr;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
+}  // namespace
+
 namespace {
 
 // every scratch buffer of a context
@@ -843,6 +884,8 @@ int sdbfx_ctx_create(int device, uint64_t mem_budget, sdbfx_ctx** out)
         return -1;
     if (hipSetDevice(device) != hipSuccess)
         return -1;
+    if (scratch_setup(device) != 0)
+        return -1;
     sdbfx_ctx* c = new sdbfx_ctx();
     c->device = device;
     c->budget = mem_budget;
@@ -1422,6 +1465,39 @@ int sdbfx_kernel_time(sdbfx_ctx* ctx, const char* name, double* ms, uint64_t* la
         *ms = it == ctx->times.end() ? 0.0 : it->second.first;
     if (launches)
         *launches = it == ctx->times.end() ? 0 : it->second.second;
+    return 0;
+}
+
+int sdbfx_digest_blocks_noalloc(sdbfx_ctx* ctx, const void* d_input, uint64_t len, uint32_t dd_block,
+                                uint8_t* d_filters, uint16_t* d_elem_counts, uint16_t* d_hamming, uint32_t* nfilters)
+{
+    if (!ctx || !d_input || !d_filters || !d_elem_counts || !d_hamming || !nfilters)
+        return fail(ctx, "null argument");
+    if ((uintptr_t)d_input % SDBFX_PAD)
+        return fail(ctx, "the input must be 256-byte aligned");
+    if (dd_block < SDBFX_MIN_FILE || dd_block > SDBFX_TILE || dd_block % 16)
+        return fail(ctx, "dd block size must be a multiple of 16 in [512, 16384]");
+    *nfilters = 0;
+    if (len < SDBFX_MIN_FILE) {
+        ctx->err = "input too small";
+        return -3;
+    }
+    const uint64_t nblk = block_count(len, dd_block);
+    if (nblk > SDBFX_NOALLOC_BLOCKS)
+        return fail(ctx, "input of " + std::to_string(nblk) + " dd-blocks: the no-allocation digest takes at most " +
+                             std::to_string(SDBFX_NOALLOC_BLOCKS));
+    void* sp = scratch_ptr(ctx->device);
+    if (!sp)
+        return fail(ctx, "no device address for the scratch array");
+    HIPC(ctx, hipSetDevice(ctx->device));
+    BlockDesc* bd = (BlockDesc*)sp;
+    hipLaunchKernelGGL(block_desc_kernel, dim3((uint32_t)((nblk + 255) / 256)), dim3(256), 0, ctx->stream, bd,
+                       len, dd_block, (uint32_t)nblk);
+    HIPC(ctx, hipGetLastError());
+    hipLaunchKernelGGL(block_digest_kernel, dim3((uint32_t)nblk), dim3(SDBFX_NT), 0, ctx->stream,
+                       (const uint8_t*)d_input, (const BlockDesc*)bd, d_filters, d_elem_counts, d_hamming);
+    HIPC(ctx, hipGetLastError());
+    *nfilters = (uint32_t)nblk;
     return 0;
 }
 

@@ -957,6 +957,29 @@ __global__ __launch_bounds__(SDBFX_NT, 2) void block_digest_kernel(
     }
 }
 
+// The dd-blocks of one input of `len` bytes at offset 0 (sdbf_core.cc:
+// 288-324): full blocks, then a tail of at least 512 bytes; block t is
+// filter t.  One thread per block.
+__global__ void block_desc_kernel(BlockDesc* __restrict__ out, uint64_t len, uint32_t dd, uint32_t nblk)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nblk)
+        return;
+    const uint64_t qt = len / dd;
+    BlockDesc b;
+    if (t < qt) {
+        b.off = (uint64_t)t * dd;
+        b.n = dd;
+        b.tail = 0;
+    } else {
+        b.off = qt * dd;
+        b.n = (uint32_t)(len % dd);
+        b.tail = 1;
+    }
+    b.out = t;
+    out[t] = b;
+}
+
 // ------------------------------------------------------------------ stream mode
 
 __device__ __forceinline__ void stream_tile(TileSmem& sm, const uint8_t* __restrict__ data, const TileDesc& td,

@@ -250,6 +250,27 @@ def main(lib):
     assert res[-1].status == sdhash_amd.EBUDGET and refused == 1, (res[-1].status, refused)
     assert deferred > 0, deferred
     print("admission ok (%d deferred)" % deferred, flush=True)
+
+    # the no-allocation block digest (work list in the fixed scratch array):
+    # the same filters as the set digest, twice in a row, the second smaller
+    import ctypes as C
+    for size, dd in ((3 * 16 * KIB + 700, 16384), (9000, 1024)):
+        d = eng.alloc(size)
+        d.upload(0, img[:size] if size <= img.size else big[:size])
+        n = size // dd + (1 if size % dd >= 512 else 0)
+        fb, eb, hb = eng.alloc(n * 256), eng.alloc(n * 2), eng.alloc(n * 2)
+        cnt = C.c_uint32()
+        assert eng.lib.sdbfx_digest_blocks_noalloc(eng.ctx, d.ptr, size, dd, fb.ptr, eb.ptr, hb.ptr,
+                                                   C.byref(cnt)) == 0
+        eng.sync()
+        assert cnt.value == n
+        ns = eng.new_set()
+        assert np.all(ns.digest_device(d, [0], [size], [b"x"], dd_block=dd) == 0)
+        wf, wec, whm = ns.read(0)
+        assert C.string_at(fb.ptr.value, n * 256) == wf.tobytes()
+        assert C.string_at(eb.ptr.value, n * 2) == wec.tobytes()
+        assert C.string_at(hb.ptr.value, n * 2) == whm.tobytes()
+    print("noalloc ok", flush=True)
     eng.close()
 
 

@@ -35,6 +35,7 @@ def run(lib):
     assert "compare ok" in r.stdout
     assert "serve ok" in r.stdout
     assert "admission ok" in r.stdout
+    assert "noalloc ok" in r.stdout
 
 
 @pytest.mark.parametrize("variant", [[], ["-DSDBFX_WARM=12", "-DSDBFX_INSERT_REPLAY=1"]],
