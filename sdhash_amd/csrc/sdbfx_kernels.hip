@@ -15,7 +15,7 @@
 //   3. scores  : the popularity state machine of sdbf_core.cc:51-89.  Its state
 //                before step i is the remembered minimum position m (the rank
 //                at m is implied), so 256 lanes each run one segment of steps
-//                after a speculative 128-step warm-up.  Every lane's start
+//                after a speculative SDBFX_WARM-step warm-up.  Every lane's start
 //                state is checked against its predecessor's end state and
 //                mismatching segments are undone and re-run until all agree
 //                (fix-point).  The first segment of a stream tile that does not
@@ -394,6 +394,77 @@ __device__ __forceinline__ uint32_t sm_run(TileSmem& sm, uint32_t base, uint32_t
     return m;
 }
 
+// Corrects a run of steps [i0, i1) that sm_run made from state mo when the
+// true start state is mn: both trajectories advance together over the same
+// words, and where a step's credits differ the old one is withdrawn and the
+// new one added.  Once a lane's two trajectories reach the same canonical
+// state their futures are identical (credits included), so the lane stops;
+// the loop ends when every lane of the wave has stopped.  Returns the
+// canonical state after step i1-1 of the new trajectory (the old run's end
+// state when they merged).  With HAND, the canonical state before step
+// hand_i+1 of the new trajectory is stored to sm.misc[2] if that step is
+// re-run.  i0 - base is a multiple of 4.  Called by every lane of the wave;
+// lanes with `run` false do nothing.
+template <bool HAND>
+__device__ __forceinline__ uint32_t sm_dual(TileSmem& sm, uint32_t base, uint32_t t0, uint32_t T, uint32_t i0,
+                                            uint32_t i1, uint32_t mo, uint32_t mn, uint32_t old_end, uint32_t hand_i,
+                                            bool run)
+{
+    const uint16_t* P = sm.P;
+    // (the states of lanes that do not run may be unset)
+    uint32_t ro = run && mo ? rank_at(P, base, mo) : 0u;
+    uint32_t rn = run && mn ? rank_at(P, base, mn) : 0u;
+    auto credit = [&](uint32_t pos, uint32_t v) {
+        const uint32_t rel = pos - t0;
+        if (pos >= t0 && rel < T)
+            atomicAdd(&sm.u.s.score[rel >> 2], v << (8 * (rel & 3)));
+    };
+    bool live = run && i0 < i1, merged = false;
+    uint32_t i = i0;
+    while (__ballot(live)) {
+        if (live) {
+            const uint32_t x = pix(i - base);
+            const uint64_t W = *(const uint64_t*)(P + x);
+            const uint64_t R = *(const uint64_t*)(P + x + 68);   // pix(i + 64 - base)
+            uint32_t rf[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                rf[j] = P[pix(i + j + (((uint32_t)(W >> (16 * j)) & 0xFFFFu) >> 10) - base)] & 0x3FFu;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t ii = i + j;
+                if (ii < i1) {
+                    const uint32_t w = (uint32_t)(W >> (16 * j)) & 0xFFFFu;
+                    const uint32_t r64 = (uint32_t)(R >> (16 * j)) & 0x3FFu;
+                    const uint32_t mf = ii + (w >> 10);
+                    const bool co = ii > 0 && ro > 0 && r64 >= ro && ii < mo;
+                    const bool cn = ii > 0 && rn > 0 && r64 >= rn && ii < mn;
+                    mo = co ? (r64 == ro ? ii + 64 : mo) : mf;
+                    mn = cn ? (r64 == rn ? ii + 64 : mn) : mf;
+                    const bool ho = co || rf[j] != 0, hn = cn || rf[j] != 0;
+                    ro = co ? ro : rf[j];
+                    rn = cn ? rn : rf[j];
+                    if (mo != mn || ho != hn) {
+                        if (ho)
+                            credit(mo, 0xFFFFFFFFu);   // -1 in the position's byte
+                        if (hn)
+                            credit(mn, 1u);
+                    }
+                    if (HAND && ii == hand_i)
+                        sm.misc[2] = canon(P, base, ii + 1, mn);
+                }
+            }
+            i += 4;
+            const uint32_t ko = (mo > i && ro > 0) ? mo : 0u, kn = (mn > i && rn > 0) ? mn : 0u;
+            if (ko == kn && i <= i1)
+                merged = true;
+            if (merged || i >= i1)
+                live = false;
+        }
+    }
+    return merged ? old_end : canon(P, base, i1, mn);
+}
+
 __device__ __forceinline__ uint32_t byte_of(const uint4& v, int j)
 {
     const uint32_t w = (j < 4) ? v.x : (j < 8) ? v.y : (j < 12) ? v.z : v.w;
@@ -512,8 +583,8 @@ __device__ __forceinline__ uint32_t tile_core(TileSmem& sm, const uint8_t* __res
                             e += sm.dtab[((wi_[j] >> (8 * (bi & 3))) & 0xFFu) + 1] -
                                  sm.dtab[(wo_[j] >> (8 * (bo & 3))) & 0xFFu];
                         }
-                        const uint32_t r = sm.rtab[e >> 10];
-                        const uint32_t rv = p0 + 16 * c + j < pv ? r : 0u;
+                        // windows from pv on get their rank 0 below
+                        const uint32_t rv = sm.rtab[e >> 10];
                         rk[j >> 1] = (j & 1) ? (rk[j >> 1] | rv << 16) : rv;
                     }
                     uint16_t* pc = Pc + 16 * c + 4 * ((r0 + 16 * c) >> 6) - 4 * (r0 >> 6);   // pix(r0 + 16c)
@@ -523,6 +594,8 @@ __device__ __forceinline__ uint32_t tile_core(TileSmem& sm, const uint8_t* __res
                 }
             }
         }
+        // positions without a window: rank 0 (this also clears what the last
+        // chunk wrote past pv, which is always inside the lane's own run)
         for (uint32_t p = max(pv, p0); p < base + r1; p++)
             sm.P[pix(p - base)] = 0;
     }
@@ -694,13 +767,17 @@ __device__ __forceinline__ uint32_t tile_core(TileSmem& sm, const uint8_t* __res
             mis = pred != sm.start_m[tid];
         }
         __syncthreads();
-        if (mis) {
-            sm_run<-1, true, false>(sm, base, t0, T, s, e, sm.start_m[tid], 0);
-            const uint32_t m = hand_i != 0xFFFFFFFFu
-                                   ? sm_run<1, true, true>(sm, base, t0, T, s, e, pred, hand_i)
-                                   : sm_run<1, true, false>(sm, base, t0, T, s, e, pred, 0);
-            sm.start_m[tid] = pred;
-            sm.end_m[tid] = canon(sm.P, base, e, m);
+        {
+            // every lane of the wave takes part in sm_dual's wave-wide loop;
+            // the ones without a mismatch start out stopped
+            const uint32_t old = sm.start_m[tid], old_end = sm.end_m[tid];
+            const uint32_t ne = hand_i != 0xFFFFFFFFu
+                                    ? sm_dual<true>(sm, base, t0, T, s, e, old, pred, old_end, hand_i, mis)
+                                    : sm_dual<false>(sm, base, t0, T, s, e, old, pred, old_end, 0, mis);
+            if (mis) {
+                sm.end_m[tid] = ne;
+                sm.start_m[tid] = pred;
+            }
         }
         if (!__syncthreads_or(mis))
             break;

@@ -140,23 +140,25 @@ struct sdbfx_server {
             b.status.assign(n, 0);
             int rc = sdbfx_set_create(cd, &b.set);
             uint32_t m = n;   // the queries digested: [0, m) of the batch
-            bool trimmed = false;
+            bool retried = false;
             for (;;) {
                 if (rc == 0)
                     rc = sdbfx_set_digest(cd, b.set, m, bufs.data(), lens.data(), names.data(), dd.data(),
                                           b.status.data());
                 if (rc != SDBFX_EBUDGET)
                     break;
-                if (!trimmed) {
-                    // the scratch cached from earlier batches may be what is in the way
-                    trimmed = true;
-                    rc = sdbfx_ctx_trim(cd);
-                    continue;
-                }
-                if (m == 1)
+                // the scratch that earlier batches (or this failed attempt)
+                // left may be what is in the way: free it, then try the same
+                // queries once more before splitting them
+                if ((rc = sdbfx_ctx_trim(cd)) != 0)
                     break;
-                m = (m + 1) / 2;
-                rc = 0;
+                if (retried && m == 1) {
+                    rc = SDBFX_EBUDGET;
+                    break;
+                }
+                if (retried)
+                    m = (m + 1) / 2;
+                retried = true;
             }
             if (m < n) {
                 // deferred: back to the front of the queue, in order
