@@ -509,7 +509,11 @@ __device__ __forceinline__ uint32_t tile_core(TileSmem& sm, const uint8_t* __res
     // loads (chunk indices past the run clamped to its last needed chunk) are
     // issued before the tables are filled, so their latency overlaps the fill.
     const uint32_t PR = ((L + RANK_LANES - 1) / RANK_LANES + 15) & ~15u;
-    const uint32_t r0 = min(tid * PR, L);
+    // The ranks lanes are waves 0-1 of even workgroups and waves 2-3 of odd
+    // ones, so that two workgroups sharing a CU keep different SIMDs busy.
+    const uint32_t rl = ((blockIdx.x >> 8) & 1) ? (uint32_t)tid - (SDBFX_NT - RANK_LANES) : (uint32_t)tid;
+    const bool rank_lane = rl < RANK_LANES;
+    const uint32_t r0 = rank_lane ? min(rl * PR, L) : L;
     const uint32_t r1 = min(r0 + PR, L);
     const uint32_t p0 = base + r0;
     const uint32_t pv = max(p0, min(base + r1, lim));   // windows exist for [p0, pv)
@@ -523,7 +527,7 @@ __device__ __forceinline__ uint32_t tile_core(TileSmem& sm, const uint8_t* __res
     for (int k = 0; k < RT; k++)
         rt[k] = c_ranks[min(tid + k * SDBFX_NT, SDBFX_RANKS_LEN - 1)];
     const int32_t dt = (int32_t)c_entropy[min(tid, 64)] - (int32_t)c_entropy[min(max(tid, 1) - 1, 63)];
-    const bool has_run = tid < RANK_LANES && p0 < pv;
+    const bool has_run = rank_lane && p0 < pv;
     const uint4* q0 = (const uint4*)(u + (has_run ? p0 : 0));
     const uint32_t lastc = has_run ? nch + 3 : 3;
     uint4 X[RUN_CHUNKS + 4];
@@ -539,8 +543,8 @@ __device__ __forceinline__ uint32_t tile_core(TileSmem& sm, const uint8_t* __res
         ((uint4*)sm.u.hist)[q] = make_uint4(0, 0, 0, 0);
     __syncthreads();
 
-    if (tid < RANK_LANES) {
-        uint32_t* H = (uint32_t*)sm.u.hist + tid;
+    if (rank_lane) {
+        uint32_t* H = (uint32_t*)sm.u.hist + rl;
         if (p0 < pv) {
             // the first window's 64 increments (in two halves) and its
             // entropy from the counts they returned
