@@ -174,32 +174,38 @@ def serving(eng, args) -> dict:
     for k in range(nplain):
         if res[k].status != 0 or not np.array_equal(res[k].scores, plain_scores[k]):
             raise RuntimeError("served result of query %d differs from the plain one" % k)
-    # poisson arrivals at 70 % of the batched rate
-    rate = 0.7 * nq / batched_s
-    gaps = rng.exponential(1.0 / rate, nq)
-    tick = [None] * nq
+    # poisson arrivals at fixed fractions of the batched rate
+    def poisson(frac):
+        rate = frac * nq / batched_s
+        gaps = rng.exponential(1.0 / rate, nq)
+        tick = [None] * nq
 
-    def submitter():
-        tnext = time.perf_counter()
+        def submitter():
+            tnext = time.perf_counter()
+            for k in range(nq):
+                tnext += gaps[k]
+                d = tnext - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                tick[k] = srv.submit(qs[k], names[k])
+        th = threading.Thread(target=submitter)
+        nb0, _ = srv.stats()
+        t0 = time.perf_counter()
+        th.start()
+        lat = []
         for k in range(nq):
-            tnext += gaps[k]
-            d = tnext - time.perf_counter()
-            if d > 0:
-                time.sleep(d)
-            tick[k] = srv.submit(qs[k], names[k])
-    th = threading.Thread(target=submitter)
-    t0 = time.perf_counter()
-    th.start()
-    lat = []
-    for k in range(nq):
-        while tick[k] is None:
-            time.sleep(0.0002)
-        r = srv.wait(tick[k])
-        lat.append(r.latency_s)
-        if k < nplain and not np.array_equal(r.scores, plain_scores[k]):
-            raise RuntimeError("served result of query %d differs from the plain one" % k)
-    th.join()
-    poisson_s = time.perf_counter() - t0
+            while tick[k] is None:
+                time.sleep(0.0002)
+            r = srv.wait(tick[k])
+            lat.append(r.latency_s)
+            if k < nplain and not np.array_equal(r.scores, plain_scores[k]):
+                raise RuntimeError("served result of query %d differs from the plain one" % k)
+        th.join()
+        el = time.perf_counter() - t0
+        nb1, _ = srv.stats()
+        return {"offered_qps": rate, "qps": nq / el, "batches": nb1 - nb0,
+                "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99)}
+    loads = {"40pct": poisson(0.4), "70pct": poisson(0.7)}
     nb_total, _, nb_mfma = srv.stats3()
     srv.close()
     qfilters = sum(int(x.size) for x in qs) / 1e6
@@ -210,9 +216,9 @@ def serving(eng, args) -> dict:
         "plain_p50_ms": pct(plain_lat, 50), "plain_p99_ms": pct(plain_lat, 99),
         "batched_qps": nq / batched_s, "batched_batches": nb_closed,
         "batched_vs_plain": (nq / batched_s) / (nplain / plain_s),
-        "poisson_target_qps": rate, "poisson_qps": nq / poisson_s,
-        "poisson_batches": nb_total - nb_closed, "mfma_batches": nb_mfma,
-        "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+        "poisson": loads, "mfma_batches": nb_mfma,
+        "poisson_target_qps": loads["70pct"]["offered_qps"], "poisson_qps": loads["70pct"]["qps"],
+        "p50_ms": loads["70pct"]["p50_ms"], "p99_ms": loads["70pct"]["p99_ms"],
         "identical_results": True,
         "data": "device-generated segment mix (sdbfx_synth_fill); reference -b 10 over 1 GiB inputs",
         "text_form": text,
@@ -263,12 +269,49 @@ def text_form(eng, ref) -> dict:
             "parse_kernel_ms": pms, "parse_kernel_GBps": len(txt) / (pms / 1e3) / 1e9, "parse_wall_s": pwall}
 
 
+def _sample_check(eng, s, st, dbuf, offs, sizes, dd, idx, names):
+    """Digests idx of the inputs behind set s (statuses st of its
+    digest_device call) against the oracle's digests of the same bytes,
+    downloaded from dbuf, on every host thread."""
+    import oracle
+    hip = _hip()
+    th = oracle.threads_available()
+    pos = np.cumsum(np.asarray(st) == 0) - 1          # set index of each input
+    stream = [i for i in idx if dd[i] == 0]
+    bufs = [download(hip, dbuf, int(offs[i]), int(sizes[i])) for i in stream]
+    ods = dict(zip(stream, oracle.digest_stream_many(bufs, [names[i] for i in stream], threads=th)))
+    for i in idx:
+        if dd[i]:
+            ods[i] = oracle.digest_block(download(hip, dbuf, int(offs[i]), int(sizes[i])), int(dd[i]), names[i],
+                                         threads=th)
+    for i in idx:
+        od = ods[i]
+        if od is None:
+            if st[i] != -3:
+                raise RuntimeError("input %d: the oracle refuses it, the engine does not" % i)
+            continue
+        f, ec, _ = s.read(int(pos[i]))
+        inf = s.info(int(pos[i]))
+        same = st[i] == 0 and np.array_equal(f, od.filters) and inf.orig_size == od.orig_size
+        if od.elem_counts is not None:
+            same = same and np.array_equal(ec, od.elem_counts)
+        else:
+            same = same and inf.last_count == od.last_count
+        if not same:
+            raise RuntimeError("digest of input %d differs from the oracle's" % i)
+    return len(idx), int(sum(int(sizes[i]) for i in idx))
+
+
 def batched_digest(eng, args) -> dict:
-    """Batched digesting: files with log-normal sizes (median 256 KiB,
-    sigma 1.5, clamped to 4 KiB-64 MiB) of device-generated segment mix,
-    resident in HBM; >= 16 MiB hashed with -b 16, the rest stream mode (the
-    sdhash CLI policy).  batched: one call for all files; plain: one call per
-    file over the first `plain` files.  Digests compared on a sample."""
+    """Batched digesting: `batch_files` files (76,000 by default, about 60 GB)
+    with log-normal sizes (median 256 KiB, sigma 1.5, clamped to 4 KiB-64
+    MiB) of device-generated segment mix, 5 % of each file's bytes planted as
+    4-64 KiB chunks copied from a shared 1 GiB donor region (so that files
+    share content), resident in HBM; >= 16 MiB hashed with -b 16, the rest in
+    stream mode (the sdhash CLI policy).  batched: one call for all files;
+    plain: one call per file over the first `plain` files.  Checked against
+    the oracle on a sample of files of every size, and batched against
+    per-file on a sample."""
     from sdhash_amd import synth
     rng = np.random.Generator(np.random.PCG64(21))
     n = args.batch_files
@@ -279,6 +322,27 @@ def batched_digest(eng, args) -> dict:
     dd = np.where(sizes >= 16 * MIB, 16384, 0).astype(np.uint32)
     dbuf = eng.alloc(total)
     synth.device_fill(dbuf.ptr.value, total, seed=31)
+    # planted chunks: non-overlapping slots of each file, from the donor
+    DONOR = 1 << 30
+    donor = eng.alloc(DONOR)
+    synth.device_fill(donor.ptr.value, DONOR, seed=32)
+    d_off, s_off, lens = [], [], []
+    for i in range(n):
+        want = int(0.05 * sizes[i])
+        if want < 4096:
+            continue
+        c = rng.integers(4096, 65536 + 1, max(1, want // 4096)) & ~15
+        c = c[np.cumsum(c) <= want] if (np.cumsum(c) <= want).any() else c[:1]
+        slot = int(sizes[i]) // len(c)
+        for k, ck in enumerate(c):
+            ck = int(min(ck, slot)) & ~15
+            if ck < 16:
+                continue
+            d_off.append(int(offs[i]) + ((k * slot) & ~15))
+            s_off.append(int(rng.integers(0, DONOR - ck)) & ~15)
+            lens.append(ck)
+    synth.device_plant(dbuf.ptr.value, total, donor.ptr.value, DONOR, d_off, s_off, lens)
+    donor.free()
     names = [b"file%05d" % i for i in range(n)]
     s = eng.new_set()
     eng.sync()
@@ -295,29 +359,43 @@ def batched_digest(eng, args) -> dict:
         plain.append(p)
     eng.sync()
     t_plain = time.perf_counter() - t
+    pos = np.cumsum(st == 0) - 1
     for i in range(0, nplain, max(1, nplain // 200)):
-        a, b = s.read(i), plain[i].read(0)
+        if st[i] != 0:
+            continue
+        a, b = s.read(int(pos[i])), plain[i].read(0)
         if not all(np.array_equal(x, y) for x, y in zip(a, b)):
             raise RuntimeError("batched digest of file %d differs from the per-file one" % i)
     plain_bytes = int(sizes[:nplain].sum())
     for p in plain:
         p.close()
+    # oracle sample: files spread over the size range, and the largest ones
+    order = np.argsort(sizes, kind="stable")
+    idx = sorted(set(order[np.linspace(0, n - 1, 120).astype(int)].tolist() + order[-3:].tolist()))
+    nsample, sample_bytes = _sample_check(eng, s, st, dbuf, offs, sizes, dd, idx, names)
     dbuf.free()
     return {
         "files": n, "bytes": int(sizes.sum()), "block_mode_files": int((dd > 0).sum()),
+        "planted_chunks": len(lens), "planted_bytes": int(sum(lens)),
         "batched_s": t_batched, "batched_GBps": float(sizes.sum()) / t_batched / 1e9,
         "plain_files": nplain, "plain_s": t_plain, "plain_GBps": plain_bytes / t_plain / 1e9,
         "batched_vs_plain": (float(sizes.sum()) / t_batched) / (plain_bytes / t_plain),
         "too_small": int((st == -3).sum()), "identical_sample": True,
-        "data": "device-generated segment mix (sdbfx_synth_fill)",
+        "oracle_sample": {"files": nsample, "bytes": sample_bytes, "identical": True},
+        "data": "device-generated segment mix (sdbfx_synth_fill) with 5 % planted donor chunks",
     }
 
 
 def allpairs(eng, args) -> dict:
-    """All-pairs compare: digests of 1-6,000 filters (uniform) hashed with
-    -b 16 from device-generated segment mix, about `allpairs_mfilters`
-    million filters in all; compare_all with each comparison kernel, same
-    scores required.  Rates are filter pairs per second of kernel time."""
+    """All-pairs compare: about `allpairs_mfilters` million filters (2 by
+    default) in digests of 1-6,000 filters (uniform), hashed with -b 16 from
+    segments cut at 16 KiB-aligned random offsets of one device-generated
+    image half their total size, so that overlapping segments share blocks
+    and filters; compare_all with each comparison kernel, same scores
+    required.  Rates are filter pairs per second of kernel time.  Checked
+    against the oracle: a sample of digests digested on the host, and the
+    scores of a sample of digest pairs."""
+    import oracle
     import sdhash_amd
     from sdhash_amd import synth
     rng = np.random.Generator(np.random.PCG64(41))
@@ -326,19 +404,19 @@ def allpairs(eng, args) -> dict:
     while sum(counts) < target:
         counts.append(int(rng.integers(1, 6001)))
     sizes = np.array(counts, np.int64) * 16384
-    offs = np.zeros(len(sizes), np.int64)
-    offs[1:] = np.cumsum(sizes[:-1])
-    total = int(sizes.sum())
-    dbuf = eng.alloc(total)
-    synth.device_fill(dbuf.ptr.value, total, seed=43)
+    img_blocks = max(int(sizes.sum()) // 16384 // 2, max(counts))
+    offs = np.array([int(rng.integers(0, img_blocks - c + 1)) * 16384 for c in counts], np.int64)
+    dimg = eng.alloc(img_blocks * 16384)
+    synth.device_fill(dimg.ptr.value, img_blocks * 16384, seed=43)
+    names = [b"d%04d" % i for i in range(len(sizes))]
     s = eng.new_set()
-    s.digest_device(dbuf, offs, sizes, [b"d%04d" % i for i in range(len(sizes))], dd_block=16384)
+    s.digest_device(dimg, offs, sizes, names, dd_block=16384)
     eng.sync()
-    dbuf.free()
     n = len(s)
     nf = np.array([s.info(i).bf_count for i in range(n)], np.float64)
     pairs = float((nf.sum() ** 2 - (nf ** 2).sum()) / 2)
-    out = {"digests": n, "filters": s.filter_count(), "filter_pairs": pairs}
+    out = {"digests": n, "filters": s.filter_count(), "filter_pairs": pairs,
+           "image_bytes": img_blocks * 16384, "segment_bytes": int(sizes.sum())}
     ref = None
     for label, method, kname in (("mfma", sdhash_amd.CMP_MFMA, "cmp_mfma_kernel"),
                                  ("valu", sdhash_amd.CMP_VALU, "cmp_kernel")):
@@ -356,11 +434,92 @@ def allpairs(eng, args) -> dict:
             ref = sc
         elif not np.array_equal(sc, ref):
             raise RuntimeError("all-pairs scores differ between the comparison kernels")
-        out[label] = {"kernel_ms": kms, "wall_s": wall, "pairs_per_s": pairs / (kms / 1e3)}
+        out[label] = {"kernel_ms": kms, "wall_s": wall, "pairs_per_s": pairs / (kms / 1e3),
+                      "cmp_kernel_ms": eng.kernel_time(kname)[0]}
     eng.compare_method(sdhash_amd.CMP_AUTO)
     out["scores_ge_21"] = int((ref >= 21).sum())
-    out["data"] = "device-generated segment mix (sdbfx_synth_fill), -b 16"
+    # oracle: digests of a sample hashed on the host; the scores of all pairs
+    # among 12 digests of at most 1,500 filters, and of 8 pairs of digests of
+    # at most 3,000 filters that score >= 21 on the device
+    _sample_check(eng, s, np.zeros(n, np.int32), dimg, offs, sizes, [16384] * n,
+                  sorted(rng.choice(n, min(6, n), replace=False).tolist()), names)
+    dimg.free()
+    from concurrent.futures import ThreadPoolExecutor
+    cnt = np.array(counts)
+    ia, ib = np.triu_indices(n, 1)
+    hot = np.flatnonzero((ref >= 21) & (cnt[ia] <= 3000) & (cnt[ib] <= 3000))
+    hot = rng.permutation(hot)[:8]
+    small = rng.permutation(np.flatnonzero(cnt <= 1500))[:12].tolist()
+    pl = [(int(ia[k]), int(ib[k])) for k in hot] + [(a, b) for x, a in enumerate(sorted(small))
+                                                      for b in sorted(small)[x + 1:]]
+    ods = {}
+    for i in {x for p in pl for x in p}:
+        f, ec, _ = s.read(i)
+        inf = s.info(i)
+        ods[i] = oracle.Digest(names[i], f, inf.bf_count, 0, ec, inf.orig_size, 16384)
+        ods[i].cstruct()
+    with ThreadPoolExecutor(oracle.threads_available()) as ex:
+        want = list(ex.map(lambda p: oracle.score(ods[p[0]], ods[p[1]]), pl))
+    for (a, b), w in zip(pl, want):
+        if int(ref[a * n - a * (a + 1) // 2 + (b - a - 1)]) != w:
+            raise RuntimeError("score of digests %d, %d differs from the oracle's" % (a, b))
+    out["oracle_sample"] = {"digests_hashed": min(6, n), "pairs_scored": len(pl),
+                            "pairs_ge_21": int(sum(w >= 21 for w in want)), "identical": True}
+    out["data"] = ("device-generated segment mix (sdbfx_synth_fill), -b 16 segments of one image at "
+                   "16 KiB-aligned random offsets (overlapping)")
     return out
+
+
+def plain_allpairs(eng) -> dict:
+    """The plain method: 64 files of 1 MiB of segment mix with 0-50 %
+    planted cross-file chunks (host-generated), uploaded to HBM, stream
+    digests and all 2,016 pair scores (sdhash -g); the oracle on every host
+    thread does the same and must agree."""
+    import oracle
+    from sdhash_amd import synth
+    files = synth.file_set(101, 64, MIB)
+    names = [b"dir/file%02d.bin" % i for i in range(64)]
+    dbuf = eng.alloc(64 * MIB)
+    dbuf.upload(0, np.concatenate(files))
+    offs = [i * MIB for i in range(64)]
+    steps = 5
+    eng.sync()
+    eng.profiling(True)
+    eng.reset_kernel_times()
+    t = time.perf_counter()
+    for _ in range(steps):
+        s = eng.new_set()
+        s.digest_device(dbuf, offs, [MIB] * 64, names, dd_block=0)
+        sc = eng.compare_all(s)
+    eng.sync()
+    wall = (time.perf_counter() - t) / steps
+    eng.profiling(False)
+    dig_ms = sum(eng.kernel_time(k)[0] for k in ("stream_tile_kernel", "stream_fixup_plan_kernel",
+                                                 "stream_fixup_par_kernel", "stream_fixup_kernel",
+                                                 "stream_insert_kernel", "gather_filters_kernel")) / steps
+    cmp_ms = sum(eng.kernel_time(k)[0] for k in ("cmp_kernel", "cmp_mfma_kernel", "col_order_kernel",
+                                                 "cmp_prep_kernel", "chunk_max_kernel", "pair_sum_kernel")) / steps
+    nf = np.array([s.info(i).bf_count for i in range(64)], np.float64)
+    pairs = float((nf.sum() ** 2 - (nf ** 2).sum()) / 2)
+    dbuf.free()
+    th = oracle.threads_available()
+    t = time.perf_counter()
+    ods = oracle.digest_stream_many(files, names, threads=th)
+    cpu_dig = time.perf_counter() - t
+    t = time.perf_counter()
+    want = oracle.compare_all(ods, threads=th)
+    cpu_cmp = time.perf_counter() - t
+    if not np.array_equal(sc, want):
+        raise RuntimeError("plain all-pairs scores differ from the oracle's")
+    for i in range(64):
+        if not np.array_equal(s.read(i)[0], ods[i].filters):
+            raise RuntimeError("plain digest %d differs from the oracle's" % i)
+    return {"files": 64, "bytes": 64 * MIB, "filters": int(nf.sum()), "filter_pairs": pairs, "scores": int(sc.size),
+            "step_wall_ms": wall * 1e3, "digest_kernel_ms": dig_ms, "compare_kernel_ms": cmp_ms,
+            "digest_MBps": 64 * MIB / 1e6 / (dig_ms / 1e3), "compare_pairs_per_s": pairs / (cmp_ms / 1e3),
+            "cpu_baseline": {"digest_MBps": 64 * MIB / 1e6 / cpu_dig, "compare_pairs_per_s": pairs / cpu_cmp,
+                             "cores": th, "kind": "port", "sample": "the whole workload"},
+            "identical": True, "scores_ge_1": int((want >= 1).sum())}
 
 
 def copy_probe() -> int:
@@ -388,10 +547,11 @@ def main() -> int:
     ap.add_argument("--serve-queries", type=int, default=256)
     ap.add_argument("--serve-plain", type=int, default=256)
     ap.add_argument("--serve-batch-mib", type=int, default=512)
-    ap.add_argument("--batch-files", type=int, default=20000)
+    ap.add_argument("--batch-files", type=int, default=76000)
     ap.add_argument("--batch-plain", type=int, default=4000)
     ap.add_argument("--no-allpairs", action="store_true", help="skip the all-pairs measurement")
-    ap.add_argument("--allpairs-mfilters", type=float, default=1.0)
+    ap.add_argument("--allpairs-mfilters", type=float, default=2.0)
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain 64 x 1 MiB all-pairs measurement")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -529,6 +689,9 @@ def main() -> int:
                              "algorithmic_ops_per_launch": pairs * 4096},
     }
     del tset, qset
+    if not args.no_plain:
+        log("plain all-pairs ...")
+        out["plain_allpairs"] = plain_allpairs(eng)
     if not args.no_batched:
         log("batched digest ...")
         out["batched_digest"] = batched_digest(eng, args)

@@ -75,7 +75,49 @@ __global__ void synth_fill_kernel(uint8_t* __restrict__ out, uint64_t n, uint64_
     }
 }
 
+// Planted chunks: chunk c copies len[c] bytes from src + s_off[c] to
+// dst + d_off[c], one workgroup per chunk, 16 bytes per thread step (offsets
+// and lengths multiples of 16; the source is not written by any chunk).
+__global__ void synth_plant_kernel(uint8_t* __restrict__ dst, const uint8_t* __restrict__ src,
+                                   const uint64_t* __restrict__ d_off, const uint64_t* __restrict__ s_off,
+                                   const uint32_t* __restrict__ len)
+{
+    const uint64_t d = d_off[blockIdx.x], sidx = s_off[blockIdx.x];
+    const uint32_t n = len[blockIdx.x];
+    for (uint32_t i = threadIdx.x * 16; i < n; i += blockDim.x * 16)
+        *(uint4*)(dst + d + i) = *(const uint4*)(src + sidx + i);
+}
+
 extern "C" {
+
+// Copies n chunks from the device buffer src into the device buffer dst
+// (host arrays of offsets and lengths, all multiples of 16).  Returns 0 or -1.
+int sdbfx_synth_plant(void* dst, const void* src, uint32_t n, const uint64_t* d_off, const uint64_t* s_off,
+                      const uint32_t* len)
+{
+    if (!dst || !src || !n)
+        return n ? -1 : 0;
+    for (uint32_t c = 0; c < n; c++)
+        if ((d_off[c] | s_off[c] | len[c]) & 15)
+            return -1;
+    uint64_t* dd = nullptr;
+    uint64_t* ds = nullptr;
+    uint32_t* dl = nullptr;
+    int rc = -1;
+    if (hipMalloc(&dd, n * 8ull) == hipSuccess && hipMalloc(&ds, n * 8ull) == hipSuccess &&
+        hipMalloc(&dl, n * 4ull) == hipSuccess && hipMemcpy(dd, d_off, n * 8ull, hipMemcpyHostToDevice) == hipSuccess &&
+        hipMemcpy(ds, s_off, n * 8ull, hipMemcpyHostToDevice) == hipSuccess &&
+        hipMemcpy(dl, len, n * 4ull, hipMemcpyHostToDevice) == hipSuccess) {
+        hipLaunchKernelGGL(synth_plant_kernel, dim3(n), dim3(256), 0, 0, (uint8_t*)dst, (const uint8_t*)src, dd, ds,
+                           dl);
+        if (hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess)
+            rc = 0;
+    }
+    (void)hipFree(dd);
+    (void)hipFree(ds);
+    (void)hipFree(dl);
+    return rc;
+}
 
 // Fills device memory out[0..bytes) with the stream `seed` from byte
 // `offset` on; bytes and offset multiples of 16.  Returns 0 or -1.

@@ -131,18 +131,43 @@ def fragments(seed: int, image: np.ndarray, n: int, size: int = 4 * KIB) -> list
 _synth_lib = None
 
 
-def device_fill(ptr: int, nbytes: int, seed: int, offset: int = 0) -> None:
-    """Fills device memory at `ptr` with `nbytes` of the device-side segment
-    mix (libsdbfx_synth.so, same mix as segment_mix, different bytes), the
-    stream `seed` from byte `offset` on.  nbytes, offset: multiples of 16."""
+def _synth():
     import ctypes as C
     global _synth_lib
     if _synth_lib is None:
         path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsdbfx_synth.so")
         if not os.path.exists(path):
             raise RuntimeError(f"{path} is not built (run python -m sdhash_amd.build)")
-        _synth_lib = C.CDLL(path)
-        _synth_lib.sdbfx_synth_fill.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64]
-        _synth_lib.sdbfx_synth_fill.restype = C.c_int
-    if _synth_lib.sdbfx_synth_fill(C.c_void_p(ptr), nbytes, offset, seed) != 0:
+        L = C.CDLL(path)
+        L.sdbfx_synth_fill.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64]
+        L.sdbfx_synth_fill.restype = C.c_int
+        L.sdbfx_synth_plant.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.sdbfx_synth_plant.restype = C.c_int
+        _synth_lib = L
+    return _synth_lib
+
+
+def device_fill(ptr: int, nbytes: int, seed: int, offset: int = 0) -> None:
+    """Fills device memory at `ptr` with `nbytes` of the device-side segment
+    mix (libsdbfx_synth.so, same mix as segment_mix, different bytes), the
+    stream `seed` from byte `offset` on.  nbytes, offset: multiples of 16."""
+    import ctypes as C
+    if _synth().sdbfx_synth_fill(C.c_void_p(ptr), nbytes, offset, seed) != 0:
         raise RuntimeError("device fill failed")
+
+
+def device_plant(dst: int, dst_size: int, src: int, src_size: int, d_off, s_off, lens) -> None:
+    """Copies chunks (offsets and lengths multiples of 16) from device memory
+    src + s_off[i] to dst + d_off[i] in one launch (libsdbfx_synth.so); src
+    is not written.  Every chunk must lie inside both buffers."""
+    import ctypes as C
+    d = np.ascontiguousarray(d_off, np.uint64)
+    s = np.ascontiguousarray(s_off, np.uint64)
+    n = np.ascontiguousarray(lens, np.uint32)
+    if not (d.size == s.size == n.size):
+        raise ValueError("offset and length arrays differ in size")
+    if n.size and (int((d + n).max()) > dst_size or int((s + n).max()) > src_size):
+        raise ValueError("a planted chunk lies outside its buffer")
+    if n.size and _synth().sdbfx_synth_plant(C.c_void_p(dst), C.c_void_p(src), n.size, d.ctypes.data,
+                                             s.ctypes.data, n.ctypes.data) != 0:
+        raise RuntimeError("device plant failed")
