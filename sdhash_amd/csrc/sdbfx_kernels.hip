@@ -1787,7 +1787,7 @@ struct MfmaWaveSmem {
     // so that the 32 rows read by one instruction fall in different banks
     uint4 atile[SDBFX_MROWS * 16];
     uint32_t rinfo[SDBFX_MROWS];                    // size_code(s1) | e1 << 16
-    const uint32_t* rptr[SDBFX_MROWS];              // reference filter of each row
+    uint64_t rfil[SDBFX_MROWS];                     // reference filter of each row: index | set << 63
     unsigned long long best[SDBFX_MROWS];
     uint32_t clist[MF_CAND_CAP];    // row | (target filter index in its digest) << 7
     uint32_t bfix[SDBFX_MROWS];     // per row: floor(best fraction * 2^16), or its initial accumulator
@@ -2014,7 +2014,7 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
     const int h = lane >> 5, r32 = lane & 31;
     uint4* const atile = wsm[wv].atile;
     uint32_t* const rinfo = wsm[wv].rinfo;
-    const uint32_t** const rptr = wsm[wv].rptr;
+    uint64_t* const rfil = wsm[wv].rfil;
     unsigned long long* const best = wsm[wv].best;
     uint32_t* const clist = wsm[wv].clist;
     uint32_t* const bfix = wsm[wv].bfix;
@@ -2049,22 +2049,24 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
                 const uint32_t li = b * SDBFX_MROWS + row;
                 const bool valid = li < g.inst_count;
                 const uint64_t inst = g.inst_begin + (valid ? li : g.inst_count - 1);
-                const uint64_t rm = rmap[inst];
-                const SetView R = (rm >> 63) ? B : A;
-                rptr[row] = R.filt + (rm & ~(1ull << 63)) * SDBFX_BFW;
+                rfil[row] = rmap[inst];
                 rinfo[row] = valid ? rcode[inst] : size_code(0);
                 best[row] = valid ? gbest[inst] : 0ull;
             }
         }
         wave_lds_sync();
-        // 16 loads in flight per lane, then their LDS stores
+        // 16 loads in flight per lane, then their LDS stores.  The filter
+        // address is formed from the kernel's set arguments (not kept in LDS),
+        // so that the loads are global ones.
 #pragma unroll
         for (uint32_t k0 = 0; k0 < SDBFX_MROWS * 16; k0 += 64 * 16) {
             uint4 v[16];
 #pragma unroll
             for (int i = 0; i < 16; i++) {
                 const uint32_t k = k0 + lane + 64 * i;
-                v[i] = ((const uint4*)rptr[k >> 4])[k & 15];
+                const uint64_t rm = rfil[k >> 4];
+                const uint32_t* __restrict__ rf = ((rm >> 63) ? B.filt : A.filt) + (rm & ~(1ull << 63)) * SDBFX_BFW;
+                v[i] = ((const uint4*)rf)[k & 15];
             }
 #pragma unroll
             for (int i = 0; i < 16; i++) {
@@ -2101,6 +2103,11 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
         };
         uint32_t cl[MF_B], ky[MF_B];
         col_load(0, cl, ky);
+        // words 8q + 4h .. 8q + 4h + 3 of a row or column are its chunk 2q + h;
+        // bw: the columns' first chunk, loaded ahead by the tile before when
+        // bw_ready
+        uint4 bw[MF_B];
+        bool bw_ready = false;
 
         for (uint32_t t0 = 0; t0 < nj; t0 += 32 * MF_B) {
             const uint32_t tn0 = t0 + 32 * MF_B < nj ? t0 + 32 * MF_B : t0;   // the next tile (or this one)
@@ -2125,19 +2132,21 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
             // the next tile's column data, consumed after this tile
             col_load(tn0, cl, ky);
             // no qualifying target filter: no pair of the tile can pass
-            if (__ballot(sx != 0) == 0)
+            if (__ballot(sx != 0) == 0) {
+                bw_ready = false;
                 continue;
+            }
             // all columns of one size code: the cutoff depends on the row only
             const uint32_t umin = (uint32_t)__builtin_amdgcn_readlane((int)wave_prefix_min(cmn, lane), 63);
             const uint32_t umax =
                 0xFFFFu - (uint32_t)__builtin_amdgcn_readlane((int)wave_prefix_min(0xFFFFu - cmx, lane), 63);
             const bool uniform = umin == umax;
-            // words 8q + 4h .. 8q + 4h + 3 of a row or column are its chunk 2q + h
-            uint4 aw[MF_B], bw[MF_B];
+            uint4 aw[MF_B];
 #pragma unroll
             for (int k = 0; k < MF_B; k++) {
                 aw[k] = atile[(32 * k + r32) * 16 + (h ^ (r32 & 15))];
-                bw[k] = tbase[cbt[k]];
+                if (!bw_ready)
+                    bw[k] = tbase[cbt[k]];
             }
 
             // the smallest qualifying Hamming weight (+1) of the tile
@@ -2270,6 +2279,12 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
                     bw[k] = bnx[k];
                 }
             }
+            // the next tile's first column chunks (cl holds its columns), in
+            // flight during the candidate pass
+#pragma unroll
+            for (int k = 0; k < MF_B; k++)
+                bw[k] = tbase[cl[k] * 16 + h];
+            bw_ready = true;
 
             // candidates: accumulators >= 0, one bit per accumulator, from
             // the sign bits (shifted in by v_alignbit).  Word w holds blocks
@@ -2285,6 +2300,9 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
 #pragma unroll
                 for (int k = 0; k < 2; k++) {
                     const int bm = (2 * wi + k) / MF_B, bn = (2 * wi + k) % MF_B;
+                    // a new value of the block, defined here in AGPRs, so that
+                    // its copies to VGPRs are made here and not all at once
+                    in_agprs(acc[bm][bn]);
 #pragma unroll
                     for (int i = 0; i < 16; i++)
                         x = __builtin_amdgcn_alignbit(x, (uint32_t)acc[bm][bn][i], 31);
