@@ -70,6 +70,20 @@ def lib():
         L.orc_block_digest_scored.argtypes = [P, P, C.c_uint64, C.c_uint32, P, P]
         L.orc_block_hash_scored.argtypes = [P, P, C.c_uint32, C.c_uint32, C.c_int32, P]
         L.orc_block_hash_scored.restype = C.c_uint32
+        U64P = C.POINTER(C.c_uint64)
+        L.orc_index_mask.argtypes = [C.c_uint64]
+        L.orc_index_mask.restype = C.c_uint32
+        L.orc_index_insert.argtypes = [P, C.c_uint64, U64P, P]
+        L.orc_index_query.argtypes = [P, C.c_uint64, P]
+        L.orc_digest_stream_indexed.argtypes = [P, C.c_uint64, P, C.c_uint64, U64P, C.POINTER(P),
+                                                C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.orc_stream_hash_scored_indexed.argtypes = [P, P, C.c_uint64, C.c_uint64, P, C.c_uint64, U64P,
+                                                     C.POINTER(P), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.orc_digest_block_indexed.argtypes = [P, C.c_uint64, C.c_uint32, P, C.c_uint64, U64P, P, P, C.c_uint32,
+                                               P, P, P]
+        L.orc_block_hash_scored_indexed.argtypes = [P, P, C.c_uint32, C.c_uint32, C.c_int32, P, P, C.c_uint64,
+                                                    U64P, P, P, C.c_uint32, P]
+        L.orc_block_hash_scored_indexed.restype = C.c_uint32
         _lib = L
     return _lib
 
@@ -319,3 +333,112 @@ def compare_to(q: list[Digest], t: list[Digest], threads: int = 1, sample: int =
 
 def threads_available() -> int:
     return int(lib().orc_threads_available())
+
+
+# ------------------------------------------------------------------ index
+
+
+class Index:
+    """An index as the oracle holds it: bloom_filter(size, 5, 0, 0.01)
+    (sdbf/bloom_filter.cc:52-71) -- `size` bytes and an element count."""
+
+    def __init__(self, size: int = 64 << 20, bits=None, elem_count: int = 0):
+        self.size = int(size)
+        self.bf = np.zeros(self.size, np.uint8) if bits is None else np.array(bits, np.uint8, copy=True)
+        assert self.bf.size == self.size
+        self.elem_count = int(elem_count)
+
+    @property
+    def mask(self) -> int:
+        return int(lib().orc_index_mask(self.size))
+
+    def copy(self) -> "Index":
+        return Index(self.size, self.bf, self.elem_count)
+
+    def insert(self, h) -> bool:
+        w = np.ascontiguousarray(h, np.uint32)
+        e = C.c_uint64(self.elem_count)
+        r = lib().orc_index_insert(_ptr(self.bf), self.size, C.byref(e), _ptr(w))
+        self.elem_count = e.value
+        return bool(r)
+
+    def query(self, h) -> bool:
+        w = np.ascontiguousarray(h, np.uint32)
+        return bool(lib().orc_index_query(_ptr(self.bf), self.size, _ptr(w)))
+
+
+def _search_arrays(search):
+    n = len(search)
+    bfs = (C.c_void_p * max(n, 1))(*[x.bf.ctypes.data for x in search])
+    sizes = np.array([x.size for x in search] or [0], np.uint64)
+    return n, bfs, sizes
+
+
+def digest_stream_indexed(buf, index: Index, name: bytes = b"x") -> Digest:
+    """Stream digest with info->index = index (updated in place)."""
+    b = _u8(buf)
+    p = C.c_void_p()
+    n = C.c_uint32()
+    last = C.c_uint32()
+    e = C.c_uint64(index.elem_count)
+    rc = lib().orc_digest_stream_indexed(_ptr(b), b.size, _ptr(index.bf), index.size, C.byref(e), C.byref(p),
+                                         C.byref(n), C.byref(last))
+    if rc != 0:
+        raise ValueError(rc)
+    index.elem_count = e.value
+    f = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint8)), shape=(n.value * 256,)).copy()
+    lib().orc_free(p)
+    return Digest(name, f, n.value, last.value, None, b.size, 0)
+
+
+def digest_block_indexed(buf, dd: int = 16384, name: bytes = b"x", index: Index | None = None, search=()):
+    """Block digest with an index_info: index (info->index, updated) and/or
+    the searched indexes; returns (Digest, matches uint32[blocks, len(search)])."""
+    b = _u8(buf)
+    if b.size < 512:
+        raise ValueError(-3)
+    cnt = int(lib().orc_block_count(b.size, dd))
+    f = np.zeros(cnt * 256, dtype=np.uint8)
+    ec = np.zeros(max(cnt, 1), dtype=np.uint16)
+    ns, bfs, sizes = _search_arrays(list(search))
+    m = np.zeros(max(cnt * ns, 1), np.uint32)
+    e = C.c_uint64(index.elem_count if index else 0)
+    rc = lib().orc_digest_block_indexed(_ptr(b), b.size, dd, _ptr(index.bf) if index else None,
+                                        index.size if index else 0, C.byref(e), bfs, _ptr(sizes), ns, _ptr(f),
+                                        _ptr(ec), _ptr(m))
+    if rc != 0:
+        raise ValueError(rc)
+    if index:
+        index.elem_count = e.value
+    return Digest(name, f, cnt, 0, ec[:cnt], b.size, dd), m[:cnt * ns].reshape(cnt, ns)
+
+
+def stream_hash_scored_indexed(scores, words, index: Index, unit: int = 32 << 20):
+    """stream_hash_scored with the index gate; index updated in place."""
+    sc, w = _scored_inputs(scores, words)
+    p = C.c_void_p()
+    n = C.c_uint32()
+    last = C.c_uint32()
+    e = C.c_uint64(index.elem_count)
+    lib().orc_stream_hash_scored_indexed(_ptr(sc), _ptr(w), sc.size, unit, _ptr(index.bf), index.size, C.byref(e),
+                                         C.byref(p), C.byref(n), C.byref(last))
+    index.elem_count = e.value
+    f = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint8)), shape=(n.value * 256,)).copy()
+    lib().orc_free(p)
+    return f, n.value, last.value
+
+
+def block_hash_scored_indexed(scores, words, threshold: int, allowed: int, index: Index | None = None, search=()):
+    """block_hash_scored with an index_info; returns (filter, hash_cnt,
+    matches uint32[len(search)])."""
+    sc, w = _scored_inputs(scores, words)
+    bf = np.zeros(256, dtype=np.uint8)
+    ns, bfs, sizes = _search_arrays(list(search))
+    m = np.zeros(max(ns, 1), np.uint32)
+    e = C.c_uint64(index.elem_count if index else 0)
+    r = lib().orc_block_hash_scored_indexed(_ptr(sc), _ptr(w), sc.size, threshold, allowed, _ptr(bf),
+                                            _ptr(index.bf) if index else None, index.size if index else 0,
+                                            C.byref(e), bfs, _ptr(sizes), ns, _ptr(m))
+    if index:
+        index.elem_count = e.value
+    return bf, int(r), m[:ns]

@@ -27,6 +27,13 @@
  *                        orc_block_hash_scored): allowance spent only on
  *                        counted features, the stop at 192, the threshold
  *                        walk with its k = 15 exit, the tail rule
+ *   index gate / insert  hand-derived known answers (the *_indexed entries):
+ *   / search             a feature already in the index is skipped by the
+ *                        stream path after setting its filter bits; one with
+ *                        no new filter bit never reaches the index; block mode
+ *                        inserts every counted feature and queries every
+ *                        fourth one; bloom_filter mask and insert/query by
+ *                        brute force
  *   bitcount_cut         brute-force popcount
  *   fast mode            fold = OR of the four 64-byte quarters (identity of
  *                        the two halving folds); 64-byte counts by brute
@@ -200,6 +207,69 @@ static int big_insert(uint8_t *big, const uint32_t *h)
     return set < 5;
 }
 
+/* An index: bloom_filter(size, 5, 0, 0.01) (sdbf/bloom_filter.cc:52-71) --
+ * `size` bytes, a power of two >= 64, bit_mask = BIT_MASKS_32[log_size + 1]
+ * with log_size the bit length of size (so bit_mask = 8 * size - 1), and
+ * its element count. */
+typedef struct {
+    uint8_t *bf;
+    uint64_t size;
+    uint32_t mask;
+    uint64_t elem_count;
+} orc_index;
+
+uint32_t orc_index_mask(uint64_t size)
+{
+    if (!size || (size & (size - 1)) || size < 64)
+        return 0;
+    uint32_t log_size = 0;
+    for (uint64_t t = size; t; t >>= 1)
+        log_size++;
+    uint32_t bits = log_size + 2;                 /* BIT_MASKS_32[log_size + 1] has log_size + 2 ones */
+    return bits >= 32 ? 0xFFFFFFFFu : ((1u << bits) - 1);
+}
+
+/* sdbf/bloom_filter.cc:334-359 (query_and_set): insert (mode_set) returns
+ * whether some bit was clear, and counts the element; query returns whether
+ * all five bits are set */
+static int index_query_and_set(orc_index *ix, const uint32_t *h, int mode_set)
+{
+    uint32_t bit_cnt = 0;
+    for (int i = 0; i < 5; i++) {
+        uint32_t pos = h[i] & ix->mask;
+        uint64_t k = pos >> 3;
+        uint8_t bit = (uint8_t)(1u << (pos & 7));
+        if (ix->bf[k] & bit) {
+            bit_cnt++;
+        } else {
+            if (mode_set)
+                ix->bf[k] |= bit;
+            else
+                return 0;
+        }
+    }
+    if (mode_set) {
+        if (bit_cnt < 5) {
+            ix->elem_count++;
+            return 1;
+        }
+        return 0;
+    }
+    return bit_cnt == 5;
+}
+
+/* What an sdbf's index_info supplies to the hashing (sdbf/index_info.h):
+ * `index` (info->index: the stream-mode gate, sdbf_core.cc:110-115, and the
+ * block-mode insert, :163-166) and the indexes of info->setlist
+ * (`search`, queried by check_indexes, :167-177, 482-493); `matches`
+ * receives per block the nsearch counts of check_indexes. */
+typedef struct {
+    orc_index *index;
+    orc_index *const *search;
+    uint32_t nsearch;
+    uint32_t *matches;
+} orc_info;
+
 /* the 20 digest bytes reinterpreted as five host (little-endian) words,
  * as done by the (uint32_t*) casts at sdbf_core.cc:105-106 */
 void orc_sha1_words(const uint8_t *p, uint32_t *h)
@@ -235,11 +305,12 @@ typedef struct {
     uint8_t *big;        /* current big filter */
 } stream_state;
 
-/* sdbf/sdbf_core.cc:94-136 (gen_chunk_hash, info == NULL) over one unit:
- * positions [0, chunk_size - 64) of the unit that starts at `pos` of the
- * feature source */
+/* sdbf/sdbf_core.cc:94-136 (gen_chunk_hash) over one unit: positions
+ * [0, chunk_size - 64) of the unit that starts at `pos` of the feature
+ * source; `index` (info->index, or NULL) gates the features that set a new
+ * filter bit (:110-115) */
 static void chunk_hash(stream_state *st, const feat_src *fs, uint64_t pos,
-                       const uint16_t *scores, uint64_t chunk_size)
+                       const uint16_t *scores, uint64_t chunk_size, orc_index *index)
 {
     uint32_t h[5];
     uint32_t bf_count = st->bf_count, last_count = st->last_count;
@@ -251,6 +322,8 @@ static void chunk_hash(stream_state *st, const feat_src *fs, uint64_t pos,
         if (scores[i] > ORC_THRESHOLD) {
             feat_words(fs, pos + i, h);
             if (!orc_bf_sha1_insert(cur, h))
+                continue;
+            if (index && !index_query_and_set(index, h, 1))
                 continue;
             if (!big_insert(st->big, h))
                 continue;
@@ -279,7 +352,7 @@ static void chunk_hash(stream_state *st, const feat_src *fs, uint64_t pos,
  * reference's estimate (:202-204), plus one.
  */
 static void stream_digest_core(const feat_src *fs, const uint16_t *given, uint64_t len, uint64_t chunk,
-                               uint8_t **filters_out, uint32_t *bf_count, uint32_t *last_count)
+                               orc_index *index, uint8_t **filters_out, uint32_t *bf_count, uint32_t *last_count)
 {
     uint64_t buff_size = ((len >> 11) + 1) << 8;
     if (buff_size < 256)
@@ -307,14 +380,14 @@ static void stream_digest_core(const feat_src *fs, const uint16_t *given, uint64
             orc_gen_scores(ranks, chunk, scores, histo);
         }
         /* the threshold loop at sdbf_core.cc:220-224 has no effect in stream mode */
-        chunk_hash(&st, fs, pos, given ? given + pos : scores, chunk);
+        chunk_hash(&st, fs, pos, given ? given + pos : scores, chunk, index);
     }
     if (rem > 0) {
         if (!given) {
             orc_gen_ranks(fs->bytes + qt * chunk, rem, ranks);
             orc_gen_scores(ranks, rem, scores, NULL);
         }
-        chunk_hash(&st, fs, pos, given ? given + pos : scores, rem);
+        chunk_hash(&st, fs, pos, given ? given + pos : scores, rem, index);
     }
     /* sdbf_core.cc:235-238 */
     if (st.bf_count > 1 && st.last_count < ORC_MAX_ELEM / 8) {
@@ -341,7 +414,26 @@ int orc_digest_stream(const uint8_t *buf, uint64_t len, uint8_t **filters_out,
     if (len < ORC_MIN_FILE)
         return -3;
     feat_src fs = {buf, NULL};
-    stream_digest_core(&fs, NULL, len, ORC_CHUNK, filters_out, bf_count, last_count);
+    stream_digest_core(&fs, NULL, len, ORC_CHUNK, NULL, filters_out, bf_count, last_count);
+    return 0;
+}
+
+/*
+ * Stream-mode digest with info->index set (the --index hashing of small
+ * files, sdhash_threads.cc:376-431 -> sdbf_hash_files): index_bf holds the
+ * index's index_size bytes and *index_elem its element count, both updated.
+ * Inputs hashed one after another into the same index see each other's
+ * features, so the caller runs them in order.
+ */
+int orc_digest_stream_indexed(const uint8_t *buf, uint64_t len, uint8_t *index_bf, uint64_t index_size,
+                              uint64_t *index_elem, uint8_t **filters_out, uint32_t *bf_count, uint32_t *last_count)
+{
+    if (len < ORC_MIN_FILE)
+        return -3;
+    orc_index ix = {index_bf, index_size, orc_index_mask(index_size), *index_elem};
+    feat_src fs = {buf, NULL};
+    stream_digest_core(&fs, NULL, len, ORC_CHUNK, &ix, filters_out, bf_count, last_count);
+    *index_elem = ix.elem_count;
     return 0;
 }
 
@@ -355,22 +447,51 @@ void orc_stream_hash_scored(const uint16_t *scores, const uint32_t *words, uint6
                             uint8_t **filters_out, uint32_t *bf_count, uint32_t *last_count)
 {
     feat_src fs = {NULL, words};
-    stream_digest_core(&fs, scores, len, unit, filters_out, bf_count, last_count);
+    stream_digest_core(&fs, scores, len, unit, NULL, filters_out, bf_count, last_count);
+}
+
+/* orc_stream_hash_scored with an index gate (index_bf / index_size /
+ * *index_elem as orc_digest_stream_indexed) */
+void orc_stream_hash_scored_indexed(const uint16_t *scores, const uint32_t *words, uint64_t len, uint64_t unit,
+                                    uint8_t *index_bf, uint64_t index_size, uint64_t *index_elem,
+                                    uint8_t **filters_out, uint32_t *bf_count, uint32_t *last_count)
+{
+    orc_index ix = {index_bf, index_size, orc_index_mask(index_size), *index_elem};
+    feat_src fs = {NULL, words};
+    stream_digest_core(&fs, scores, len, unit, &ix, filters_out, bf_count, last_count);
+    *index_elem = ix.elem_count;
 }
 
 /* ---------------------------------------------------------------- block */
 
-/* sdbf/sdbf_core.cc:142-191 (gen_block_hash, no index): the positions of
- * one block start at `pos` of the feature source */
+/* sdbf/sdbf_core.cc:142-191 (gen_block_hash): the positions of one block
+ * start at `pos` of the feature source.  With info: without searched
+ * indexes every counted feature goes into info->index (:163-166); with
+ * them, every fourth counted feature (hash_cnt % 4 == 0 before the count)
+ * is looked up in each (check_indexes, :167-177) and the block's hits per
+ * index go to matches[0 .. nsearch).  (The hashes[] copies and their cap at
+ * :169-175 only stage the lookups.) */
 static uint32_t block_hash(const feat_src *fs, uint64_t pos, const uint16_t *scores, uint32_t max_offset,
-                           uint32_t threshold, int32_t allowed, uint8_t *bf)
+                           uint32_t threshold, int32_t allowed, uint8_t *bf, const orc_info *info,
+                           uint32_t *matches)
 {
     uint32_t h[5], hash_cnt = 0;
+    const uint32_t nsearch = info ? info->nsearch : 0;
+    for (uint32_t m = 0; m < nsearch; m++)
+        matches[m] = 0;
     for (uint32_t i = 0; i < max_offset - ORC_WIN && hash_cnt < ORC_MAX_ELEM_DD; i++) {
         if (scores[i] > threshold || (scores[i] == threshold && allowed > 0)) {
             feat_words(fs, pos + i, h);
             if (!orc_bf_sha1_insert(bf, h))
                 continue;
+            if (nsearch == 0) {
+                if (info && info->index)
+                    (void)index_query_and_set(info->index, h, 1);
+            } else if (hash_cnt % 4 == 0) {
+                for (uint32_t m = 0; m < nsearch; m++)
+                    if (index_query_and_set(info->search[m], h, 0))
+                        matches[m]++;
+            }
             hash_cnt++;
             if (scores[i] == threshold)
                 allowed--;
@@ -409,9 +530,13 @@ uint64_t orc_block_count(uint64_t len, uint32_t dd)
  * `threads` > 1 spreads them over OpenMP threads.
  */
 static void block_digest_core(const feat_src *fs, const uint16_t *given, uint64_t len, uint32_t dd,
-                              uint8_t *filters, uint16_t *elem_counts, int threads)
+                              uint8_t *filters, uint16_t *elem_counts, int threads, const orc_info *info)
 {
     uint64_t qt = len / dd, rem = len % dd;
+    /* with an index the blocks run in order (an index insert is not
+     * thread-safe, and its element count depends on the order) */
+    if (info)
+        threads = 1;
 #ifdef _OPENMP
     if (threads < 1)
         threads = 1;
@@ -439,7 +564,8 @@ static void block_digest_core(const feat_src *fs, const uint16_t *given, uint64_
                 orc_gen_scores(ranks, dd, scores, histo);
             }
             orc_block_threshold(histo, &k, &allowed);
-            elem_counts[i] = (uint16_t)block_hash(fs, pos, sc, dd, k, allowed, filters + (uint64_t)i * ORC_BF);
+            elem_counts[i] = (uint16_t)block_hash(fs, pos, sc, dd, k, allowed, filters + (uint64_t)i * ORC_BF, info,
+                                                  info ? info->matches + (uint64_t)i * info->nsearch : NULL);
         }
         free(ranks);
         free(scores);
@@ -456,7 +582,8 @@ static void block_digest_core(const feat_src *fs, const uint16_t *given, uint64_
             orc_gen_scores(ranks, rem, scores, NULL);
         }
         elem_counts[qt] = (uint16_t)block_hash(fs, qt * dd, sc, (uint32_t)rem, ORC_THRESHOLD, ORC_MAX_ELEM_DD,
-                                               filters + qt * ORC_BF);
+                                               filters + qt * ORC_BF, info,
+                                               info ? info->matches + qt * info->nsearch : NULL);
         free(ranks);
         free(scores);
     }
@@ -468,7 +595,40 @@ int orc_digest_block(const uint8_t *buf, uint64_t len, uint32_t dd, uint8_t *fil
     if (len < ORC_MIN_FILE)
         return -3;
     feat_src fs = {buf, NULL};
-    block_digest_core(&fs, NULL, len, dd, filters, elem_counts, threads);
+    block_digest_core(&fs, NULL, len, dd, filters, elem_counts, threads, NULL);
+    return 0;
+}
+
+/*
+ * Block-mode digest with an index_info: index_bf (index_size bytes,
+ * *index_elem elements; or NULL) is info->index, filled by every counted
+ * feature (the --index hashing of large files, sdhash_threads.cc:432-489);
+ * search_bfs / search_sizes (nsearch of them) are the indexes of
+ * info->setlist (--index-search, sdhash.cc:241-264), and matches receives
+ * orc_block_count() * nsearch counts, block-major.
+ */
+int orc_digest_block_indexed(const uint8_t *buf, uint64_t len, uint32_t dd, uint8_t *index_bf, uint64_t index_size,
+                             uint64_t *index_elem, uint8_t *const *search_bfs, const uint64_t *search_sizes,
+                             uint32_t nsearch, uint8_t *filters, uint16_t *elem_counts, uint32_t *matches)
+{
+    if (len < ORC_MIN_FILE)
+        return -3;
+    orc_index ix = {index_bf, index_size, index_bf ? orc_index_mask(index_size) : 0, index_elem ? *index_elem : 0};
+    orc_index *sx = nsearch ? (orc_index *)calloc(nsearch, sizeof(orc_index)) : NULL;
+    orc_index **sp = nsearch ? (orc_index **)calloc(nsearch, sizeof(orc_index *)) : NULL;
+    for (uint32_t m = 0; m < nsearch; m++) {
+        sx[m].bf = search_bfs[m];
+        sx[m].size = search_sizes[m];
+        sx[m].mask = orc_index_mask(search_sizes[m]);
+        sp[m] = &sx[m];
+    }
+    orc_info info = {index_bf ? &ix : NULL, sp, nsearch, matches};
+    feat_src fs = {buf, NULL};
+    block_digest_core(&fs, NULL, len, dd, filters, elem_counts, 1, &info);
+    if (index_elem)
+        *index_elem = ix.elem_count;
+    free(sx);
+    free(sp);
     return 0;
 }
 
@@ -482,14 +642,55 @@ void orc_block_digest_scored(const uint16_t *scores, const uint32_t *words, uint
                              uint8_t *filters, uint16_t *elem_counts)
 {
     feat_src fs = {NULL, words};
-    block_digest_core(&fs, scores, len, dd, filters, elem_counts, 1);
+    block_digest_core(&fs, scores, len, dd, filters, elem_counts, 1, NULL);
 }
 
 uint32_t orc_block_hash_scored(const uint16_t *scores, const uint32_t *words, uint32_t max_offset,
                                uint32_t threshold, int32_t allowed, uint8_t *bf)
 {
     feat_src fs = {NULL, words};
-    return block_hash(&fs, 0, scores, max_offset, threshold, allowed, bf);
+    return block_hash(&fs, 0, scores, max_offset, threshold, allowed, bf, NULL, NULL);
+}
+
+/* orc_block_hash_scored with an index_info: index_bf (or NULL) filled, or
+ * the nsearch indexes queried with the counts in matches[0 .. nsearch) */
+uint32_t orc_block_hash_scored_indexed(const uint16_t *scores, const uint32_t *words, uint32_t max_offset,
+                                       uint32_t threshold, int32_t allowed, uint8_t *bf, uint8_t *index_bf,
+                                       uint64_t index_size, uint64_t *index_elem, uint8_t *const *search_bfs,
+                                       const uint64_t *search_sizes, uint32_t nsearch, uint32_t *matches)
+{
+    orc_index ix = {index_bf, index_size, index_bf ? orc_index_mask(index_size) : 0, index_elem ? *index_elem : 0};
+    orc_index sx[64], *sp[64];
+    if (nsearch > 64)
+        return 0xFFFFFFFFu;
+    for (uint32_t m = 0; m < nsearch; m++) {
+        sx[m].bf = search_bfs[m];
+        sx[m].size = search_sizes[m];
+        sx[m].mask = orc_index_mask(search_sizes[m]);
+        sx[m].elem_count = 0;
+        sp[m] = &sx[m];
+    }
+    orc_info info = {index_bf ? &ix : NULL, sp, nsearch, matches};
+    feat_src fs = {NULL, words};
+    uint32_t r = block_hash(&fs, 0, scores, max_offset, threshold, allowed, bf, &info, matches);
+    if (index_elem)
+        *index_elem = ix.elem_count;
+    return r;
+}
+
+/* bloom_filter::insert_sha1 / query_sha1 of one hash (five words) */
+int orc_index_insert(uint8_t *index_bf, uint64_t index_size, uint64_t *index_elem, const uint32_t *h)
+{
+    orc_index ix = {index_bf, index_size, orc_index_mask(index_size), *index_elem};
+    int r = index_query_and_set(&ix, h, 1);
+    *index_elem = ix.elem_count;
+    return r;
+}
+
+int orc_index_query(uint8_t *index_bf, uint64_t index_size, const uint32_t *h)
+{
+    orc_index ix = {index_bf, index_size, orc_index_mask(index_size), 0};
+    return index_query_and_set(&ix, h, 0);
 }
 
 /* ------------------------------------------------------------- compare */
