@@ -29,6 +29,7 @@ extern "C" {
 
 typedef struct sdbfx_ctx sdbfx_ctx;   /* device + stream + scratch buffers */
 typedef struct sdbfx_set sdbfx_set;   /* device-resident collection of digests */
+typedef struct sdbfx_index sdbfx_index; /* device-resident index (sdhash --index) */
 
 /* Per-digest metadata, as held by class sdbf (sdbf/sdbf_class.h:95-117). */
 typedef struct {
@@ -222,6 +223,60 @@ int sdbfx_compare_to_sampled(sdbfx_ctx *ctx, const sdbfx_set *q, const sdbfx_set
 int sdbfx_set_compare_method(sdbfx_ctx *ctx, int method);
 
 /* ---- serving -------------------------------------------------------- */
+
+/* ---- indexes (sdhash --index / --index-search) ------------------------- */
+
+/*
+ * An index is bloom_filter(size, 5, 0, 0.01) (sdbf/bloom_filter.cc:52-71) in
+ * device memory: `size` bytes (a power of two from 64 to 256 MiB; sdhash uses
+ * 64 MiB, sdhash-src/sdhash_threads.cc:409,458), bit mask 8 * size - 1, the
+ * five SHA-1 words of a feature as its positions, and an element count (the
+ * inserts that set a clear bit, bloom_filter.cc:334-359).  Created empty.
+ * Block-mode insertion also holds 32 bytes per index byte of scratch (2 GiB
+ * for 64 MiB), allocated on its first use and freed with the index.
+ */
+int sdbfx_index_create(sdbfx_ctx *ctx, uint64_t size, sdbfx_index **out);
+void sdbfx_index_destroy(sdbfx_index *ix);
+uint64_t sdbfx_index_size(const sdbfx_index *ix);
+/* Copies the index bits (size bytes; bits may be NULL) and the element count
+ * to the host: the content of bloom_filter::write_out's .idx file
+ * (bloom_filter.cc:208-228) before its LZ4 compression. */
+int sdbfx_index_read(sdbfx_ctx *ctx, const sdbfx_index *ix, uint8_t *bits, uint64_t *elem_count);
+/* Replaces the index content, e.g. with a .idx file's decompressed bits
+ * (bloom_filter(string), bloom_filter.cc:77-114). */
+int sdbfx_index_write(sdbfx_ctx *ctx, sdbfx_index *ix, const uint8_t *bits, uint64_t elem_count);
+
+/*
+ * sdbfx_set_digest with an index_info (sdbf/index_info.h), as the sdbf
+ * constructors take it (sdbf_class.cc:71-136):
+ *   - index (info->index, may be NULL): a stream-mode input skips every
+ *     feature whose five index bits were all set, after setting its filter
+ *     bits (gen_chunk_hash, sdbf_core.cc:110-115); a block-mode input inserts
+ *     every counted feature (gen_block_hash, :163-166).  Either way the index
+ *     takes the inserts.  Inputs are taken in the order given, as sdhash does
+ *     with -p 1 (with more threads the reference's result depends on their
+ *     timing), so the digests of stream inputs depend on everything inserted
+ *     before them.
+ *   - search[0 .. nsearch) (info->setlist's indexes, --index-search): in
+ *     block mode, every fourth counted feature (k % 4 == 0 in counted order)
+ *     is looked up in each (check_indexes, :167-177, 482-493) and nothing is
+ *     inserted.  matches receives nsearch counts per dd-block of the call's
+ *     block-mode inputs that are digested (inputs in order, blocks in order,
+ *     block-major); matches_cap (in blocks) must hold them all.
+ *     print_indexes (:458-472) reports a block when a count exceeds 4.
+ * The digests are those of sdbfx_set_digest except for the stream gate.  A
+ * call that fails may leave the inserts of its completed passes in the index.
+ */
+int sdbfx_set_digest_indexed(sdbfx_ctx *ctx, sdbfx_set *set, uint32_t n, const uint8_t *const *bufs,
+                             const uint64_t *lens, const char *const *names, const uint32_t *dd_block,
+                             sdbfx_index *index, const sdbfx_index *const *search, uint32_t nsearch,
+                             uint32_t *matches, uint64_t matches_cap, int32_t *status);
+/* As sdbfx_set_digest_indexed for inputs in device memory (as
+ * sdbfx_set_digest_device). */
+int sdbfx_set_digest_device_indexed(sdbfx_ctx *ctx, sdbfx_set *set, uint32_t n, const void *d_base,
+                                    const uint64_t *offsets, const uint64_t *lens, const char *const *names,
+                                    const uint32_t *dd_block, sdbfx_index *index, const sdbfx_index *const *search,
+                                    uint32_t nsearch, uint32_t *matches, uint64_t matches_cap, int32_t *status);
 
 /*
  * A query server over a resident reference set (continuous batching).

@@ -27,6 +27,8 @@ SYMBOLS = (
     "sdbfx_reset_kernel_times", "sdbfx_set_compare_method", "sdbfx_digest_blocks_noalloc",
     "sdbfx_serve_create", "sdbfx_serve_destroy", "sdbfx_serve_error", "sdbfx_serve_ref_size",
     "sdbfx_serve_submit", "sdbfx_serve_poll", "sdbfx_serve_wait", "sdbfx_serve_stats", "sdbfx_serve_admission",
+    "sdbfx_index_create", "sdbfx_index_destroy", "sdbfx_index_size", "sdbfx_index_read", "sdbfx_index_write",
+    "sdbfx_set_digest_indexed", "sdbfx_set_digest_device_indexed",
 )
 
 # comparison kernels (sdbfx_set_compare_method)
@@ -109,6 +111,15 @@ def load():
     L.sdbfx_serve_wait.argtypes = [P, U64, P, C.POINTER(I32), C.POINTER(U64)]
     L.sdbfx_serve_stats.argtypes = [P, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64)]
     L.sdbfx_serve_admission.argtypes = [P, C.POINTER(U64), C.POINTER(U64)]
+    L.sdbfx_index_create.argtypes = [P, U64, C.POINTER(P)]
+    L.sdbfx_index_destroy.argtypes = [P]
+    L.sdbfx_index_destroy.restype = None
+    L.sdbfx_index_size.argtypes = [P]
+    L.sdbfx_index_size.restype = U64
+    L.sdbfx_index_read.argtypes = [P, P, P, C.POINTER(U64)]
+    L.sdbfx_index_write.argtypes = [P, P, P, U64]
+    L.sdbfx_set_digest_indexed.argtypes = [P, P, U32, P, P, P, P, P, P, U32, P, U64, P]
+    L.sdbfx_set_digest_device_indexed.argtypes = [P, P, U32, P, P, P, P, P, P, P, U32, P, U64, P]
     for name in SYMBOLS:
         getattr(L, name).restype = getattr(L, name).restype or I32
     for name in ("sdbfx_ctx_create", "sdbfx_sync", "sdbfx_ctx_trim", "sdbfx_set_create", "sdbfx_set_digest",
@@ -118,7 +129,8 @@ def load():
                  "sdbfx_upload", "sdbfx_set_profiling", "sdbfx_kernel_time",
                  "sdbfx_reset_kernel_times", "sdbfx_set_compare_method", "sdbfx_digest_blocks_noalloc", "sdbfx_serve_create",
                  "sdbfx_serve_submit", "sdbfx_serve_poll", "sdbfx_serve_wait", "sdbfx_serve_stats",
-                 "sdbfx_serve_admission"):
+                 "sdbfx_serve_admission", "sdbfx_index_create", "sdbfx_index_read", "sdbfx_index_write",
+                 "sdbfx_set_digest_indexed", "sdbfx_set_digest_device_indexed"):
         getattr(L, name).restype = I32
     _lib = L
     return L
@@ -247,6 +259,41 @@ class DigestSet:
             self.engine.ctx, self.handle, n, ptrs, _ptr(lens), nm, _ptr(dd), _ptr(st)))
         return st[:n]
 
+    def digest_indexed(self, bufs, names=None, dd_block=0, index=None, search=()):
+        """digest() with an index_info: `index` (an Index, or None) gates
+        stream-mode inputs and takes block-mode inserts; `search` (Indexes)
+        are looked up by block-mode inputs instead.  Returns (status,
+        matches uint32[blocks of the block-mode inputs, len(search)])."""
+        n = len(bufs)
+        arrs, ptrs = _bytes_array(bufs)
+        lens = np.array([a.size for a in arrs], np.uint64)
+        return self._indexed(n, lens, names, dd_block, index, search,
+                             lambda *a: self.engine.lib.sdbfx_set_digest_indexed(
+                                 self.engine.ctx, self.handle, n, ptrs, _ptr(lens), *a))
+
+    def digest_device_indexed(self, dbuf: DeviceBuffer, offsets, lens, names=None, dd_block=0, index=None,
+                              search=()):
+        """digest_device() with an index_info, as digest_indexed."""
+        n = len(lens)
+        off = np.ascontiguousarray(offsets, np.uint64)
+        ln = np.ascontiguousarray(lens, np.uint64)
+        return self._indexed(n, ln, names, dd_block, index, search,
+                             lambda *a: self.engine.lib.sdbfx_set_digest_device_indexed(
+                                 self.engine.ctx, self.handle, n, dbuf.ptr, _ptr(off), _ptr(ln), *a))
+
+    def _indexed(self, n, lens, names, dd_block, index, search, call):
+        dd = self._dd(dd_block, n)
+        nm = self._names(names, n)
+        st = np.zeros(max(n, 1), np.int32)
+        search = list(search)
+        nblk = sum(int(l) // int(d) + (1 if int(l) % int(d) >= 512 else 0)
+                   for l, d in zip(lens, dd[:n]) if d and l >= 512)
+        m = np.zeros(max(nblk * len(search), 1), np.uint32)
+        sarr = (C.c_void_p * max(len(search), 1))(*[x.handle for x in search])
+        self.engine._check(call(nm, _ptr(dd), index.handle if index else None, sarr, len(search), _ptr(m), nblk,
+                                _ptr(st)))
+        return st[:n], m[:nblk * len(search)].reshape(nblk, len(search))
+
     def digest_device(self, dbuf: DeviceBuffer, offsets, lens, names=None, dd_block=0) -> np.ndarray:
         n = len(lens)
         off = np.ascontiguousarray(offsets, np.uint64)
@@ -290,6 +337,41 @@ class DigestSet:
             pass
 
 
+class Index:
+    """A device-resident index (sdhash --index): bloom_filter(size, 5, 0,
+    0.01) bits and element count."""
+
+    def __init__(self, engine: "Engine", size: int = 64 << 20):
+        self.engine = engine
+        p = C.c_void_p()
+        engine._check(engine.lib.sdbfx_index_create(engine.ctx, size, C.byref(p)))
+        self.handle = p
+        self.size = size
+
+    def read(self) -> tuple[np.ndarray, int]:
+        """(bits as uint8[size], element count)"""
+        bits = np.zeros(self.size, np.uint8)
+        e = C.c_uint64()
+        self.engine._check(self.engine.lib.sdbfx_index_read(self.engine.ctx, self.handle, _ptr(bits), C.byref(e)))
+        return bits, e.value
+
+    def write(self, bits, elem_count: int) -> None:
+        b = np.ascontiguousarray(bits, np.uint8)
+        assert b.size == self.size
+        self.engine._check(self.engine.lib.sdbfx_index_write(self.engine.ctx, self.handle, _ptr(b), elem_count))
+
+    def close(self) -> None:
+        if self.handle:
+            self.engine.lib.sdbfx_index_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Engine:
     """One device context (HIP stream) of the engine."""
 
@@ -313,6 +395,9 @@ class Engine:
 
     def new_set(self) -> DigestSet:
         return DigestSet(self)
+
+    def new_index(self, size: int = 64 << 20) -> "Index":
+        return Index(self, size)
 
     def alloc(self, nbytes: int) -> DeviceBuffer:
         return DeviceBuffer(self, nbytes)

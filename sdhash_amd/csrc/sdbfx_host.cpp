@@ -51,7 +51,8 @@ struct sdbfx_ctx {
     std::map<std::string, std::pair<double, uint64_t>> times;
     std::vector<hipEvent_t> evpool;
     DevBuf stage, blocks, tiles, tstate, units, files, cand, sfilt, sham, counts, gsrc, gdst, gcnt,
-        glast, groups, pairs, partial, imax, scores, gbest, rmap, rcode, text, tdesc, thdr, cofirst, cocount;
+        glast, groups, pairs, partial, imax, scores, gbest, rmap, rcode, text, tdesc, thdr, cofirst, cocount, feat,
+        imatch, irefs;
     // two pinned host buffers for staging caller memory to the device
     void* pin[2] = {nullptr, nullptr};
     hipEvent_t pin_ev[2] = {nullptr, nullptr};
@@ -75,6 +76,19 @@ struct sdbfx_set {
     mutable std::vector<uint8_t> cbuilt;
     mutable hipEvent_t cev = nullptr;
     mutable std::mutex cmu;
+};
+
+// An index: bloom_filter(size, 5, 0, 0.01) (sdbf/bloom_filter.cc:52-71) in
+// device memory -- `size` bytes of bits, bit mask 8 * size - 1, and its
+// element count.  `first` (one word per bit, all ~0 between calls) is the
+// per-bit scratch of block-mode insertion, allocated on first use.
+struct sdbfx_index {
+    int device = 0;
+    uint64_t size = 0;
+    uint32_t mask = 0;
+    uint32_t* bits = nullptr;
+    unsigned long long* elem = nullptr;
+    uint32_t* first = nullptr;
 };
 
 // The no-allocation block digest (sdbfx_digest_blocks_noalloc) keeps its
@@ -126,7 +140,8 @@ std::vector<DevBuf*> scratch_bufs(sdbfx_ctx* ctx)
     return {&ctx->stage, &ctx->blocks, &ctx->tiles, &ctx->tstate, &ctx->units, &ctx->files, &ctx->cand,
             &ctx->sfilt, &ctx->sham, &ctx->counts, &ctx->gsrc, &ctx->gdst, &ctx->gcnt, &ctx->glast, &ctx->groups,
             &ctx->pairs, &ctx->partial, &ctx->imax, &ctx->scores, &ctx->gbest, &ctx->rmap, &ctx->rcode,
-            &ctx->text, &ctx->tdesc, &ctx->thdr, &ctx->cofirst, &ctx->cocount};
+            &ctx->text, &ctx->tdesc, &ctx->thdr, &ctx->cofirst, &ctx->cocount, &ctx->feat, &ctx->imatch,
+            &ctx->irefs};
 }
 
 int fail(sdbfx_ctx* ctx, const std::string& msg)
@@ -269,10 +284,60 @@ uint64_t block_count(uint64_t len, uint32_t dd)
     return len / dd + ((len % dd) >= SDBFX_MIN_FILE ? 1 : 0);
 }
 
+// The index_info of a digest call (sdbf/index_info.h): `ix` is info->index
+// (stream-mode gate, block-mode insertion), `search` the indexes of
+// info->setlist (block-mode lookups, which take the place of the insertion);
+// matches receives nsearch counts per block of the call's block-mode
+// inputs, `mpos` blocks written so far.
+struct IndexCall {
+    sdbfx_index* ix = nullptr;
+    std::vector<IndexRef> search;
+    uint32_t* matches = nullptr;
+    uint64_t mpos = 0;
+};
+
+// Block-mode index work of one pass, after block_digest_kernel left the
+// counted features in ctx->feat.
+int block_index_pass(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t nblk, IndexCall* ic)
+{
+    const uint32_t nix = (uint32_t)ic->search.size();
+    const BlockDesc* bl = (const BlockDesc*)ctx->blocks.p;
+    const uint16_t* ecf = (const uint16_t*)set->ecf.p;
+    const uint32_t* feat = (const uint32_t*)ctx->feat.p;
+    if (nix) {
+        int rc;
+        if ((rc = ensure(ctx, ctx->imatch, (uint64_t)nblk * nix * 4 + 256)) || (rc = upload(ctx, ctx->irefs, ic->search)))
+            return rc;
+        HIPC(ctx, hipMemsetAsync(ctx->imatch.p, 0, (uint64_t)nblk * nix * 4, ctx->stream));
+        const uint64_t nt = (uint64_t)nblk * (SDBFX_MAX_ELEM_DD / 4);
+        LAUNCH(ctx, index_search_kernel, (uint32_t)((nt + 255) / 256), 256, bl, nblk, ecf, feat,
+               (const IndexRef*)ctx->irefs.p, nix, (uint32_t*)ctx->imatch.p);
+        HIPC(ctx, hipMemcpyAsync(ic->matches + ic->mpos * nix, ctx->imatch.p, (uint64_t)nblk * nix * 4,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+        HIPC(ctx, hipStreamSynchronize(ctx->stream));
+        ic->mpos += nblk;
+        return 0;
+    }
+    sdbfx_index* ix = ic->ix;
+    const uint64_t nbits = 8ull * ix->size;
+    if (!ix->first) {
+        HIPC(ctx, hipMalloc(&ix->first, nbits * 4));
+        LAUNCH(ctx, fill_u32_kernel, (uint32_t)std::min<uint64_t>(2048, (nbits / 64 + 255) / 256), 256, ix->first,
+               nbits, 0xFFFFFFFFu);
+    }
+    const uint64_t nt = (uint64_t)nblk * SDBFX_MAX_ELEM_DD;
+    const uint32_t grid = (uint32_t)((nt + 255) / 256);
+    LAUNCH(ctx, index_first_kernel, grid, 256, bl, nblk, ecf, feat, (const uint32_t*)ix->bits, ix->mask, ix->first);
+    LAUNCH(ctx, index_commit_kernel, grid, 256, bl, nblk, ecf, feat, ix->bits, ix->mask,
+           (const uint32_t*)ix->first, ix->elem);
+    LAUNCH(ctx, index_reset_kernel, grid, 256, bl, nblk, ecf, feat, ix->mask, ix->first);
+    return 0;
+}
+
 // One device pass over inputs [i0, i1) that are resident at d_base + off[i].
 int digest_pass(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t i0, uint32_t i1, const uint8_t* d_base,
                 const uint64_t* off, const uint64_t* lens, const char* const* names, const uint32_t* dd,
-                const int32_t* status)
+                const int32_t* status, IndexCall* ic)
 {
     // ---- stream-mode work list
     std::vector<TileDesc> tiles;
@@ -335,10 +400,16 @@ int digest_pass(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t i0, uint32_t i1, const 
                    (const TileDesc*)ctx->tiles.p, (const UnitDesc*)ctx->units.p, (uint32_t*)ctx->cand.p,
                    (TileState*)ctx->tstate.p);
         }
-        LAUNCH(ctx, stream_insert_kernel, (uint32_t)sfiles.size(), 64, (const StreamFile*)ctx->files.p,
-               (const TileDesc*)ctx->tiles.p, (const TileState*)ctx->tstate.p,
-               (const uint32_t*)ctx->cand.p, (uint8_t*)ctx->sfilt.p, (uint16_t*)ctx->sham.p,
-               (uint32_t*)ctx->counts.p);
+        if (ic && ic->ix)
+            LAUNCH(ctx, stream_insert_indexed_kernel, 1, 64, (const StreamFile*)ctx->files.p, (uint32_t)sfiles.size(),
+                   (const TileDesc*)ctx->tiles.p, (const TileState*)ctx->tstate.p, (const uint32_t*)ctx->cand.p,
+                   (uint8_t*)ctx->sfilt.p, (uint16_t*)ctx->sham.p, (uint32_t*)ctx->counts.p, ic->ix->bits,
+                   ic->ix->mask, ic->ix->elem);
+        else
+            LAUNCH(ctx, stream_insert_kernel, (uint32_t)sfiles.size(), 64, (const StreamFile*)ctx->files.p,
+                   (const TileDesc*)ctx->tiles.p, (const TileState*)ctx->tstate.p,
+                   (const uint32_t*)ctx->cand.p, (uint8_t*)ctx->sfilt.p, (uint16_t*)ctx->sham.p,
+                   (uint32_t*)ctx->counts.p);
         HIPC(ctx, hipMemcpyAsync(counts.data(), ctx->counts.p, counts.size() * 4, hipMemcpyDeviceToHost,
                                  ctx->stream));
         HIPC(ctx, hipStreamSynchronize(ctx->stream));
@@ -386,9 +457,16 @@ int digest_pass(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t i0, uint32_t i1, const 
     if (!blocks.empty()) {
         if (int rc = upload(ctx, ctx->blocks, blocks))
             return rc;
+        const bool idx = ic && (ic->ix || !ic->search.empty());
+        if (idx)
+            if (int rc = ensure(ctx, ctx->feat, blocks.size() * (SDBFX_MAX_ELEM_DD * 5 * 4ull) + 256))
+                return rc;
         LAUNCH(ctx, block_digest_kernel, (uint32_t)blocks.size(), SDBFX_NT, d_base,
                (const BlockDesc*)ctx->blocks.p, (uint8_t*)set->filt.p, (uint16_t*)set->ecf.p,
-               (uint16_t*)set->hamf.p);
+               (uint16_t*)set->hamf.p, idx ? (uint32_t*)ctx->feat.p : (uint32_t*)nullptr);
+        if (idx)
+            if (int rc = block_index_pass(ctx, set, (uint32_t)blocks.size(), ic))
+                return rc;
     }
 
     // ---- stream filters into the set
@@ -475,7 +553,7 @@ int upload_host(sdbfx_ctx* ctx, uint8_t* dst, const uint8_t* src, uint64_t len)
 
 int digest_common(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t n, const uint8_t* d_base, const uint64_t* off,
                   const uint64_t* lens, const char* const* names, const uint32_t* dd, int32_t* status,
-                  const uint8_t* const* host_bufs)
+                  const uint8_t* const* host_bufs, IndexCall* ic = nullptr)
 {
     if (!ctx || !set || (n && (!lens || !status)))
         return fail(ctx, "null argument");
@@ -510,7 +588,11 @@ int digest_common(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t n, const uint8_t* d_b
     while (i0 < n) {
         uint32_t i1 = i0;
         uint64_t bytes = 0;
-        while (i1 < n && (i1 == i0 || bytes + lens[i1] <= kPassBytes)) {
+        // with an index, a pass holds inputs of one mode only, so that the
+        // index sees stream and block inputs in the order given
+        auto blk = [&](uint32_t i) { return dd && dd[i]; };
+        while (i1 < n && (i1 == i0 || bytes + lens[i1] <= kPassBytes) &&
+               !(ic && ic->ix && i1 > i0 && blk(i1) != blk(i0))) {
             bytes += (lens[i1] + SDBFX_PAD - 1) / SDBFX_PAD * SDBFX_PAD;
             i1++;
         }
@@ -535,7 +617,7 @@ int digest_common(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t n, const uint8_t* d_b
             base = (const uint8_t*)ctx->stage.p;
             o = hoff.data();
         }
-        if (int rc = digest_pass(ctx, set, i0, i1, base, o, lens, names, dd, status))
+        if (int rc = digest_pass(ctx, set, i0, i1, base, o, lens, names, dd, status, ic))
             return rollback(rc);
         i0 = i1;
     }
@@ -1011,6 +1093,139 @@ int sdbfx_set_digest_device(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t n, const vo
     HIPC(ctx, hipSetDevice(ctx->device));
     return digest_common(ctx, set, n, (const uint8_t*)d_base, offsets, lens, names, dd_block, status,
                          nullptr);
+}
+
+// ---- indexes
+
+int sdbfx_index_create(sdbfx_ctx* ctx, uint64_t size, sdbfx_index** out)
+{
+    if (!ctx || !out)
+        return -1;
+    *out = nullptr;
+    if (size < 64 || (size & (size - 1)) || size > (256ull << 20))
+        return fail(ctx, "index size must be a power of two in [64, 256 MiB]");
+    HIPC(ctx, hipSetDevice(ctx->device));
+    sdbfx_index* ix = new sdbfx_index;
+    ix->device = ctx->device;
+    ix->size = size;
+    ix->mask = (uint32_t)(8 * size - 1);
+    if (hipMalloc(&ix->bits, size) != hipSuccess || hipMalloc(&ix->elem, sizeof(unsigned long long)) != hipSuccess ||
+        hipMemsetAsync(ix->bits, 0, size, ctx->stream) != hipSuccess ||
+        hipMemsetAsync(ix->elem, 0, sizeof(unsigned long long), ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        sdbfx_index_destroy(ix);
+        return fail(ctx, "index allocation failed");
+    }
+    *out = ix;
+    return 0;
+}
+
+void sdbfx_index_destroy(sdbfx_index* ix)
+{
+    if (!ix)
+        return;
+    (void)hipSetDevice(ix->device);
+    (void)hipDeviceSynchronize();
+    if (ix->bits)
+        (void)hipFree(ix->bits);
+    if (ix->elem)
+        (void)hipFree(ix->elem);
+    if (ix->first)
+        (void)hipFree(ix->first);
+    delete ix;
+}
+
+uint64_t sdbfx_index_size(const sdbfx_index* ix) { return ix ? ix->size : 0; }
+
+int sdbfx_index_read(sdbfx_ctx* ctx, const sdbfx_index* ix, uint8_t* bits, uint64_t* elem_count)
+{
+    if (!ctx || !ix)
+        return -1;
+    HIPC(ctx, hipSetDevice(ctx->device));
+    if (bits)
+        HIPC(ctx, hipMemcpyAsync(bits, ix->bits, ix->size, hipMemcpyDeviceToHost, ctx->stream));
+    unsigned long long e = 0;
+    HIPC(ctx, hipMemcpyAsync(&e, ix->elem, sizeof(e), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    if (elem_count)
+        *elem_count = e;
+    return 0;
+}
+
+int sdbfx_index_write(sdbfx_ctx* ctx, sdbfx_index* ix, const uint8_t* bits, uint64_t elem_count)
+{
+    if (!ctx || !ix || !bits)
+        return -1;
+    HIPC(ctx, hipSetDevice(ctx->device));
+    const unsigned long long e = elem_count;
+    HIPC(ctx, hipMemcpyAsync(ix->bits, bits, ix->size, hipMemcpyHostToDevice, ctx->stream));
+    HIPC(ctx, hipMemcpyAsync(ix->elem, &e, sizeof(e), hipMemcpyHostToDevice, ctx->stream));
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+namespace {
+
+int digest_indexed(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t n, const uint8_t* d_base, const uint64_t* offsets,
+                   const uint64_t* lens, const char* const* names, const uint32_t* dd_block, sdbfx_index* index,
+                   const sdbfx_index* const* search, uint32_t nsearch, uint32_t* matches, uint64_t matches_cap,
+                   int32_t* status, const uint8_t* const* bufs)
+{
+    if (nsearch && (!search || !matches))
+        return fail(ctx, "null search indexes or matches");
+    IndexCall ic;
+    ic.ix = index;
+    for (uint32_t m = 0; m < nsearch; m++) {
+        if (!search[m] || search[m]->device != ctx->device)
+            return fail(ctx, "searched index missing or on another device");
+        ic.search.push_back(IndexRef{search[m]->bits, search[m]->mask, 0});
+    }
+    if (index && index->device != ctx->device)
+        return fail(ctx, "index on another device");
+    if (nsearch) {
+        uint64_t need = 0;
+        for (uint32_t i = 0; i < n; i++)
+            if (dd_block && dd_block[i] && lens && lens[i] >= SDBFX_MIN_FILE)
+                need += block_count(lens[i], dd_block[i]);
+        if (need > matches_cap)
+            return fail(ctx, "matches holds " + std::to_string(matches_cap) + " blocks, the call has " +
+                                 std::to_string(need));
+        ic.matches = matches;
+    }
+    HIPC(ctx, hipSetDevice(ctx->device));
+    return digest_common(ctx, set, n, d_base, offsets, lens, names, dd_block, status, bufs,
+                         (index || nsearch) ? &ic : nullptr);
+}
+
+}  // namespace
+
+int sdbfx_set_digest_indexed(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t n, const uint8_t* const* bufs,
+                             const uint64_t* lens, const char* const* names, const uint32_t* dd_block,
+                             sdbfx_index* index, const sdbfx_index* const* search, uint32_t nsearch,
+                             uint32_t* matches, uint64_t matches_cap, int32_t* status)
+{
+    if (!ctx)
+        return -1;
+    if (n && !bufs)
+        return fail(ctx, "null buffers");
+    return digest_indexed(ctx, set, n, nullptr, nullptr, lens, names, dd_block, index, search, nsearch, matches,
+                          matches_cap, status, bufs);
+}
+
+int sdbfx_set_digest_device_indexed(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t n, const void* d_base,
+                                    const uint64_t* offsets, const uint64_t* lens, const char* const* names,
+                                    const uint32_t* dd_block, sdbfx_index* index, const sdbfx_index* const* search,
+                                    uint32_t nsearch, uint32_t* matches, uint64_t matches_cap, int32_t* status)
+{
+    if (!ctx)
+        return -1;
+    if (n && (!d_base || !offsets))
+        return fail(ctx, "null device input");
+    for (uint32_t i = 0; i < n; i++)
+        if (offsets[i] % SDBFX_PAD)
+            return fail(ctx, "device input offsets must be multiples of 256");
+    return digest_indexed(ctx, set, n, (const uint8_t*)d_base, offsets, lens, names, dd_block, index, search,
+                          nsearch, matches, matches_cap, status, nullptr);
 }
 
 int sdbfx_set_add(sdbfx_ctx* ctx, sdbfx_set* set, const char* name, uint64_t orig_size, uint32_t bf_count,
@@ -1495,7 +1710,8 @@ int sdbfx_digest_blocks_noalloc(sdbfx_ctx* ctx, const void* d_input, uint64_t le
                        len, dd_block, (uint32_t)nblk);
     HIPC(ctx, hipGetLastError());
     hipLaunchKernelGGL(block_digest_kernel, dim3((uint32_t)nblk), dim3(SDBFX_NT), 0, ctx->stream,
-                       (const uint8_t*)d_input, (const BlockDesc*)bd, d_filters, d_elem_counts, d_hamming);
+                       (const uint8_t*)d_input, (const BlockDesc*)bd, d_filters, d_elem_counts, d_hamming,
+                       (uint32_t*)nullptr);
     HIPC(ctx, hipGetLastError());
     *nfilters = (uint32_t)nblk;
     return 0;

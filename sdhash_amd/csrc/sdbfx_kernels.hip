@@ -41,6 +41,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "sdbf_tables.h"
 #include "sdbfx_internal.h"
 
@@ -865,9 +867,13 @@ __device__ __forceinline__ uint32_t compact(TileSmem& sm, uint32_t cnt, Pred pre
 // ------------------------------------------------------------------ block mode
 
 // One workgroup per dd-block (sdbf_core.cc:252-282 per block, :305-319 tail).
+// With feat (not null), the SHA-1 words of the block's counted features, the
+// ones gen_block_hash hands to an index (sdbf_core.cc:163-177), go to
+// feat[((blockIdx.x * 192) + k) * 5 + j] for the k-th counted feature, in
+// order.
 __global__ __launch_bounds__(SDBFX_NT, 2) void block_digest_kernel(
     const uint8_t* __restrict__ data, const BlockDesc* __restrict__ blocks, uint8_t* __restrict__ filters,
-    uint16_t* __restrict__ elem_counts, uint16_t* __restrict__ hamming)
+    uint16_t* __restrict__ elem_counts, uint16_t* __restrict__ hamming, uint32_t* __restrict__ feat)
 {
     __shared__ TileSmem sm;
     const BlockDesc bd = blocks[blockIdx.x];
@@ -1006,6 +1012,12 @@ __global__ __launch_bounds__(SDBFX_NT, 2) void block_digest_kernel(
 #pragma unroll
                     for (int j = 0; j < 5; j++)
                         atomicOr(&sm.filt[b[j] >> 5], 1u << (b[j] & 31));
+                }
+                if (feat && nw && (uint32_t)lane < r) {
+                    uint32_t* fo = feat + ((uint64_t)blockIdx.x * SDBFX_MAX_ELEM_DD + hash_cnt + before) * 5;
+#pragma unroll
+                    for (int j = 0; j < 5; j++)
+                        fo[j] = sm.u.s.hsha[j][li];
                 }
                 __builtin_amdgcn_wave_barrier();
                 const uint64_t mr = r >= 64 ? ~0ull : ((1ull << r) - 1);
@@ -1192,6 +1204,20 @@ __global__ __launch_bounds__(SDBFX_NT) void stream_fixup_kernel(
 // batch is committed up to the one that completes a filter (160 elements)
 // or a de-duplication filter (8738), and the rest is taken again with the
 // new state.
+//
+// INDEXED (info->index set, the --index hashing of sdhash_threads.cc:376-431):
+// between the filter and the de-duplication filter, every candidate with new
+// filter bits is inserted into the index (bloom_filter::insert_sha1,
+// bloom_filter.cc:312-359, in global memory at ixw with bit mask imask) and
+// skipped when all its index bits were already set (sdbf_core.cc:110-115).
+// Inputs share the index, so one wave takes all `nfiles` inputs in order.  A
+// lane's index positions are read once per batch (loads that bypass the
+// wave's L1, which the index atomics do not update); a position clear before
+// the batch that another lane of the batch also sets is found with an LDS
+// hash table of the batch's clear positions, and a lane whose clear positions
+// are all shared sends the batch to the one-at-a-time replay, which inserts
+// with atomics.  The index's element count (counted inserts) is carried in a
+// register and written back to *ielem at the end.
 struct InsertSmem {
     uint32_t big[SDBFX_BIG_BYTES / 4];      // de-duplication filter
     uint32_t seen[SDBFX_BIG_BYTES / 4];     // positions of the batch
@@ -1199,17 +1225,39 @@ struct InsertSmem {
     uint32_t own[SDBFX_BF * 8];             // lowest lane setting each filter bit
     uint32_t fw[SDBFX_BFW];                 // current filter
 };
+#define IX_SLOTS 1024                        // index positions of one batch: at most 64 * 5
+struct InsertSmemIx : InsertSmem {
+    uint32_t ikey[IX_SLOTS];                // open addressing: an index position, or ~0
+    uint32_t itw[IX_SLOTS];                 // 1: the position is set by two lanes
+};
 
-__global__ __launch_bounds__(64) void stream_insert_kernel(
-    const StreamFile* __restrict__ files, const TileDesc* __restrict__ tiles,
-    const TileState* __restrict__ tstate, const uint32_t* __restrict__ cand, uint8_t* __restrict__ stage,
-    uint16_t* __restrict__ stage_ham, uint32_t* __restrict__ out_counts)
+// index word w, read at the device's coherence point (not from the L1)
+__device__ __forceinline__ uint32_t index_word(const uint32_t* ixw, uint32_t w)
 {
-    __shared__ InsertSmem ism;
+    return __hip_atomic_load(ixw + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <bool INDEXED>
+__device__ __forceinline__ void stream_insert_body(
+    const StreamFile* __restrict__ files, uint32_t fbegin, uint32_t fend, const TileDesc* __restrict__ tiles,
+    const TileState* __restrict__ tstate, const uint32_t* __restrict__ cand, uint8_t* __restrict__ stage,
+    uint16_t* __restrict__ stage_ham, uint32_t* __restrict__ out_counts, uint32_t* __restrict__ ixw,
+    uint32_t imask, unsigned long long* __restrict__ ielem)
+{
+    __shared__ typename std::conditional<INDEXED, InsertSmemIx, InsertSmem>::type ism;
     uint32_t* const big = ism.big;
     uint32_t* const fw = ism.fw;
-    const StreamFile sf = files[blockIdx.x];
     const int lane = threadIdx.x;
+    unsigned long long ecount = 0;   // index elements (INDEXED)
+    if constexpr (INDEXED) {
+        ecount = *ielem;
+        for (int q = lane; q < IX_SLOTS; q += 64) {
+            ism.ikey[q] = 0xFFFFFFFFu;
+            ism.itw[q] = 0;
+        }
+    }
+    for (uint32_t fi = fbegin; fi < fend; fi++) {
+    const StreamFile sf = files[fi];
     for (int q = lane; q < SDBFX_BIG_BYTES / 4; q += 64) {
         big[q] = 0;
         ism.seen[q] = 0;
@@ -1267,15 +1315,17 @@ __global__ __launch_bounds__(64) void stream_insert_kernel(
             for (int j = 0; j < 5; j++)
                 myh[j] = (c0 + lane < C) ? cw[j * SDBFX_CAND_CAP + c0 + lane] : 0u;
             // positions and which of them repeat an earlier one of the lane
-            uint32_t p[5], q[5], pdup = 0, qdup = 0;
+            uint32_t p[5], q[5], ip[5], pdup = 0, qdup = 0, idup = 0;
 #pragma unroll
             for (int j = 0; j < 5; j++) {
                 p[j] = myh[j] & 0x7FFu;
                 q[j] = myh[j] & 0x1FFFFu;
+                ip[j] = myh[j] & imask;
 #pragma unroll
                 for (int k = 0; k < j; k++) {
                     pdup |= (p[k] == p[j] ? 1u : 0u) << j;
                     qdup |= (q[k] == q[j] ? 1u : 0u) << j;
+                    idup |= (ip[k] == ip[j] ? 1u : 0u) << j;
                 }
             }
             uint32_t s = 0;   // first candidate of the chunk not yet committed
@@ -1297,10 +1347,63 @@ __global__ __launch_bounds__(64) void stream_insert_kernel(
                             ism.own[p[j]] == (uint32_t)lane)
                             nnew++;
                 }
-                // 2. the de-duplication filter for those with new bits
                 const bool reach = act && nnew > 0;
+                // 1b. the index gate for those with new bits: positions clear
+                // before the batch (iclr), and whether another lane of the
+                // batch sets them too
+                uint32_t iclr = 0;
+                bool iamb = false;
+                if constexpr (INDEXED) {
+                    uint32_t slot[5];
+                    if (reach) {
+                        uint32_t iw[5];
+#pragma unroll
+                        for (int j = 0; j < 5; j++)
+                            iw[j] = (idup >> j) & 1 ? 0xFFFFFFFFu : index_word(ixw, ip[j] >> 5);
+#pragma unroll
+                        for (int j = 0; j < 5; j++) {
+                            slot[j] = 0;
+                            if (!((iw[j] >> (ip[j] & 31)) & 1u)) {
+                                iclr |= 1u << j;
+                                uint32_t h = (ip[j] * 2654435761u) >> 22;   // 10 bits
+                                for (;;) {
+                                    const uint32_t o = atomicCAS(&ism.ikey[h], 0xFFFFFFFFu, ip[j]);
+                                    if (o == 0xFFFFFFFFu)
+                                        break;
+                                    if (o == ip[j]) {
+                                        atomicOr(&ism.itw[h], 1u);
+                                        break;
+                                    }
+                                    h = (h + 1) & (IX_SLOTS - 1);
+                                }
+                                slot[j] = h;
+                            }
+                        }
+                    }
+                    wave_lds_sync();
+                    uint32_t ishared = 0;
+                    if (reach) {
+#pragma unroll
+                        for (int j = 0; j < 5; j++)
+                            if (((iclr >> j) & 1) && ism.itw[slot[j]])
+                                ishared |= 1u << j;
+                    }
+                    iamb = reach && iclr != 0 && ishared == iclr;
+                    wave_lds_sync();
+                    if (reach) {
+#pragma unroll
+                        for (int j = 0; j < 5; j++)
+                            if ((iclr >> j) & 1) {
+                                ism.ikey[slot[j]] = 0xFFFFFFFFu;
+                                ism.itw[slot[j]] = 0;
+                            }
+                    }
+                }
+                // lanes that go on to the de-duplication filter
+                const bool reach2 = INDEXED ? (reach && iclr != 0) : reach;
+                // 2. the de-duplication filter for those
                 uint32_t clr = 0;   // positions clear at the start of the batch
-                if (reach) {
+                if (reach2) {
 #pragma unroll
                     for (int j = 0; j < 5; j++)
                         if (!((qdup >> j) & 1) && !((big[q[j] >> 5] >> (q[j] & 31)) & 1u)) {
@@ -1312,17 +1415,18 @@ __global__ __launch_bounds__(64) void stream_insert_kernel(
                 }
                 wave_lds_sync();
                 uint32_t shared = 0;
-                if (reach) {
+                if (reach2) {
 #pragma unroll
                     for (int j = 0; j < 5; j++)
                         if (((clr >> j) & 1) && ((ism.twice[q[j] >> 5] >> (q[j] & 31)) & 1u))
                             shared |= 1u << j;
                 }
                 // a rejection decided inside the batch: replay it in order
-                const bool replay = SDBFX_INSERT_REPLAY || __ballot(reach && clr != 0 && shared == clr) != 0;
-                const bool acc = reach && clr != 0;
+                const bool replay =
+                    SDBFX_INSERT_REPLAY || __ballot((reach2 && clr != 0 && shared == clr) || iamb) != 0;
+                const bool acc = reach2 && clr != 0;
                 wave_lds_sync();
-                if (reach) {
+                if (reach2) {
 #pragma unroll
                     for (int j = 0; j < 5; j++)
                         if ((clr >> j) & 1) {
@@ -1355,6 +1459,21 @@ __global__ __launch_bounds__(64) void stream_insert_kernel(
                         }
                         if (nb == 0)
                             continue;
+                        if constexpr (INDEXED) {
+                            // bloom_filter::insert_sha1 on the index: lane j
+                            // sets position j; the insert counts when one of
+                            // them was clear
+                            bool was = false;
+                            if (lane < 5) {
+                                const uint32_t hp = (lane == 0 ? h[0] : lane == 1 ? h[1] : lane == 2 ? h[2]
+                                                     : lane == 3 ? h[3] : h[4]) & imask;
+                                const uint32_t bit = 1u << (hp & 31);
+                                was = (atomicOr(&ixw[hp >> 5], bit) & bit) != 0;
+                            }
+                            if (__popcll(__ballot(was)) == 5)
+                                continue;
+                            ecount++;
+                        }
                         uint32_t setc = 0;
 #pragma unroll
                         for (int j = 0; j < 5; j++) {
@@ -1389,6 +1508,17 @@ __global__ __launch_bounds__(64) void stream_insert_kernel(
                             atomicOr(&big[q[j] >> 5], 1u << (q[j] & 31));
                     }
                 }
+                if constexpr (INDEXED) {
+                    // every committed candidate with new filter bits sets its
+                    // index bits; those with a clear one count as elements
+                    if (com && reach) {
+#pragma unroll
+                        for (int j = 0; j < 5; j++)
+                            if ((iclr >> j) & 1)
+                                atomicOr(&ixw[ip[j] >> 5], 1u << (ip[j] & 31));
+                    }
+                    ecount += (unsigned long long)__popcll(__ballot(com && reach2));
+                }
                 wave_lds_sync();
                 const unsigned long long upto = cut >= 63 ? ~0ull : ((2ull << cut) - 1);
                 const uint32_t k = (uint32_t)__popcll(am & upto);
@@ -1404,9 +1534,161 @@ __global__ __launch_bounds__(64) void stream_insert_kernel(
         last = SDBFX_MAX_ELEM;
     }
     if (lane == 0) {
-        out_counts[3 * blockIdx.x] = bfc;
-        out_counts[3 * blockIdx.x + 1] = last;
-        out_counts[3 * blockIdx.x + 2] = overflow;
+        out_counts[3 * fi] = bfc;
+        out_counts[3 * fi + 1] = last;
+        out_counts[3 * fi + 2] = overflow;
+    }
+    __syncthreads();
+    }
+    if constexpr (INDEXED) {
+        if (lane == 0)
+            *ielem = ecount;
+    }
+}
+
+__global__ __launch_bounds__(64) void stream_insert_kernel(
+    const StreamFile* __restrict__ files, const TileDesc* __restrict__ tiles,
+    const TileState* __restrict__ tstate, const uint32_t* __restrict__ cand, uint8_t* __restrict__ stage,
+    uint16_t* __restrict__ stage_ham, uint32_t* __restrict__ out_counts)
+{
+    stream_insert_body<false>(files, blockIdx.x, blockIdx.x + 1, tiles, tstate, cand, stage, stage_ham, out_counts,
+                              nullptr, 0u, nullptr);
+}
+
+// All `nfiles` inputs, in order, on one wave (launch one workgroup).
+__global__ __launch_bounds__(64) void stream_insert_indexed_kernel(
+    const StreamFile* __restrict__ files, uint32_t nfiles, const TileDesc* __restrict__ tiles,
+    const TileState* __restrict__ tstate, const uint32_t* __restrict__ cand, uint8_t* __restrict__ stage,
+    uint16_t* __restrict__ stage_ham, uint32_t* __restrict__ out_counts, uint32_t* __restrict__ ixw, uint32_t imask,
+    unsigned long long* __restrict__ ielem)
+{
+    stream_insert_body<true>(files, 0, nfiles, tiles, tstate, cand, stage, stage_ham, out_counts, ixw, imask, ielem);
+}
+
+// ------------------------------------------------------------------ index
+//
+// Block mode with an index (gen_block_hash, sdbf_core.cc:157-177): every
+// counted feature of a block is inserted into info->index, or, with searched
+// indexes (info->setlist), every fourth one (k % 4 == 0 in counted order) is
+// looked up in each.  block_digest_kernel leaves the counted features' SHA-1
+// words in feat (192 slots per block); feature k of launch block b has the
+// order key b * 192 + k, which follows the reference's order (inputs, then
+// blocks, then features).
+//
+// Insertion, in three passes.  The index bits are an OR, so any order sets
+// the same ones; the element count is the number of features that find one
+// of their positions clear, which for feature f means: some position of f
+// was clear before the call and no earlier feature of the call has it.  With
+// first[p] the smallest key of the features that hold a position p clear
+// before the call, f counts exactly when first[p] == key(f) for one of its
+// positions.  first[] (one word per index bit, ~0 when unused) is filled by
+// index_first_kernel, read by index_commit_kernel (which sets the bits and
+// counts), and reset by index_reset_kernel.
+
+// one thread per (block, feature slot) of the launch; false past the block's count
+__device__ __forceinline__ bool feat_slot(const BlockDesc* __restrict__ blocks, uint32_t nblk,
+                                          const uint16_t* __restrict__ ecf, uint64_t t)
+{
+    const uint64_t b = t / SDBFX_MAX_ELEM_DD;
+    return b < nblk && (t % SDBFX_MAX_ELEM_DD) < ecf[blocks[b].out];
+}
+
+__global__ void index_first_kernel(const BlockDesc* __restrict__ blocks, uint32_t nblk,
+                                   const uint16_t* __restrict__ ecf, const uint32_t* __restrict__ feat,
+                                   const uint32_t* __restrict__ ixw, uint32_t imask, uint32_t* __restrict__ first)
+{
+    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (!feat_slot(blocks, nblk, ecf, t))
+        return;
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+        const uint32_t pp = feat[t * 5 + j] & imask;
+        if (!((ixw[pp >> 5] >> (pp & 31)) & 1u))
+            atomicMin(&first[pp], (uint32_t)t);
+    }
+}
+
+__global__ void index_commit_kernel(const BlockDesc* __restrict__ blocks, uint32_t nblk,
+                                    const uint16_t* __restrict__ ecf, const uint32_t* __restrict__ feat,
+                                    uint32_t* __restrict__ ixw, uint32_t imask, const uint32_t* __restrict__ first,
+                                    unsigned long long* __restrict__ ielem)
+{
+    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const bool ok = feat_slot(blocks, nblk, ecf, t);
+    uint32_t pp[5];
+    bool mine = false;
+    if (ok) {
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+            pp[j] = feat[t * 5 + j] & imask;
+            mine |= first[pp[j]] == (uint32_t)t;
+        }
+        if (mine) {
+#pragma unroll
+            for (int j = 0; j < 5; j++)
+                atomicOr(&ixw[pp[j] >> 5], 1u << (pp[j] & 31));
+        }
+    }
+    const unsigned long long m = __ballot(mine);
+    if (m && (threadIdx.x & 63) == (uint32_t)__builtin_ctzll(m))
+        atomicAdd(ielem, (unsigned long long)__popcll(m));
+}
+
+__global__ void index_reset_kernel(const BlockDesc* __restrict__ blocks, uint32_t nblk,
+                                   const uint16_t* __restrict__ ecf, const uint32_t* __restrict__ feat,
+                                   uint32_t imask, uint32_t* __restrict__ first)
+{
+    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (!feat_slot(blocks, nblk, ecf, t))
+        return;
+#pragma unroll
+    for (int j = 0; j < 5; j++)
+        first[feat[t * 5 + j] & imask] = 0xFFFFFFFFu;
+}
+
+// fills n words with v (grid-stride)
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint64_t n, uint32_t v)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// A searched index (bloom_filter::query_sha1, bloom_filter.cc:334-359): its
+// words and bit mask.
+struct IndexRef {
+    const uint32_t* words;
+    uint32_t mask;
+    uint32_t pad;
+};
+
+// check_indexes (sdbf_core.cc:167-177, 482-493): counted features k % 4 == 0
+// of each block looked up in every searched index; matches[b * nix + m]
+// counts block b's hits in index m.  One thread per (block, looked-up
+// feature).
+__global__ void index_search_kernel(const BlockDesc* __restrict__ blocks, uint32_t nblk,
+                                    const uint16_t* __restrict__ ecf, const uint32_t* __restrict__ feat,
+                                    const IndexRef* __restrict__ ixs, uint32_t nix, uint32_t* __restrict__ matches)
+{
+    constexpr uint32_t PER = SDBFX_MAX_ELEM_DD / 4;
+    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const uint64_t b = t / PER;
+    const uint32_t k = 4 * (uint32_t)(t % PER);
+    if (b >= nblk || k >= ecf[blocks[b].out])
+        return;
+    uint32_t h[5];
+#pragma unroll
+    for (int j = 0; j < 5; j++)
+        h[j] = feat[(b * SDBFX_MAX_ELEM_DD + k) * 5 + j];
+    for (uint32_t m = 0; m < nix; m++) {
+        const IndexRef ix = ixs[m];
+        bool all = true;
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+            const uint32_t pp = h[j] & ix.mask;
+            all = all && ((ix.words[pp >> 5] >> (pp & 31)) & 1u);
+        }
+        if (all)
+            atomicAdd(&matches[b * nix + m], 1u);
     }
 }
 

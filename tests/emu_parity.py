@@ -271,7 +271,56 @@ def main(lib):
         assert C.string_at(eb.ptr.value, n * 2) == wec.tobytes()
         assert C.string_at(hb.ptr.value, n * 2) == whm.tobytes()
     print("noalloc ok", flush=True)
+
+    # indexes (--index, --index-search): stream inputs gated by the index,
+    # block inputs inserting, in the order given, sharing one small index
+    # (64 KiB, so that positions collide); then block inputs searching it
+    index_cases(eng, rng)
+    print("index ok", flush=True)
     eng.close()
+
+
+def index_cases(eng, rng):
+    img = synth.disk_image(61, 200 * KIB)
+    a = img[: 90 * KIB]
+    b = np.concatenate([img[40 * KIB: 120 * KIB], synth.segment_mix(rng, 30 * KIB)])
+    c = img[60 * KIB: 60 * KIB + 3 * 16 * KIB + 900]
+    d = np.concatenate([img[70 * KIB: 110 * KIB], img[: 20 * KIB]])
+    e = img[100 * KIB: 200 * KIB]
+    bufs = [a, b, c, d, e, img[:300]]
+    modes = [0, 0, 16384, 0, 4096, 0]
+    ix = eng.new_index(64 * KIB)
+    oix = oracle.Index(64 * KIB)
+    s = eng.new_set()
+    st, _ = s.digest_indexed(bufs, [b"i%d" % k for k in range(6)], dd_block=modes, index=ix)
+    assert list(st) == [0, 0, 0, 0, 0, -3], st
+    k = 0
+    for buf, m in zip(bufs[:5], modes):
+        od = oracle.digest_block_indexed(buf, m, index=oix)[0] if m else oracle.digest_stream_indexed(buf, oix)
+        same(s, k, od, "indexed input %d" % k)
+        k += 1
+    bits, ecount = ix.read()
+    assert np.array_equal(bits, oix.bf) and ecount == oix.elem_count, (ecount, oix.elem_count)
+    # without the index the stream digests differ (the gate skipped features)
+    plain = oracle.digest_stream(b)
+    assert plain.bf_count != s.info(1).bf_count or not np.array_equal(plain.filters, s.read(1)[0])
+    # search: the index above and a second one, loaded from the host
+    other = oracle.Index(4 * KIB)
+    for buf in (img[150 * KIB: 200 * KIB],):
+        oracle.digest_block_indexed(buf, 16384, index=other)
+    ix2 = eng.new_index(4 * KIB)
+    ix2.write(other.bf, other.elem_count)
+    sb = [img[: 100 * KIB], img[120 * KIB: 200 * KIB], synth.segment_mix(rng, 40 * KIB)]
+    s2 = eng.new_set()
+    st, got = s2.digest_indexed(sb, dd_block=[16384, 4096, 0], search=[ix, ix2])
+    assert np.all(st == 0)
+    want = [oracle.digest_block_indexed(x, m, search=[oix, other]) for x, m in zip(sb[:2], (16384, 4096))]
+    assert np.array_equal(got, np.concatenate([w[1] for w in want])), (got, want)
+    assert got.sum() > 0
+    for k2, (w, _) in enumerate(want):
+        same(s2, k2, w, "searching input %d" % k2)
+    ix.close()
+    ix2.close()
 
 
 if __name__ == "__main__":

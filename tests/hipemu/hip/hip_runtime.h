@@ -443,6 +443,15 @@ inline uint32_t atomicMin(uint32_t* p, uint32_t v)
     *p = v < o ? v : o;
     return o;
 }
+inline uint32_t atomicCAS(uint32_t* p, uint32_t cmp, uint32_t v)
+{
+    const uint32_t o = *p;
+    if (o == cmp)
+        *p = v;
+    return o;
+}
+#define __HIP_MEMORY_SCOPE_AGENT 3
+inline uint32_t __hip_atomic_load(const uint32_t* p, int, int) { return *p; }
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 
 inline uint32_t min(uint32_t a, uint32_t b) { return a < b ? a : b; }

@@ -36,6 +36,7 @@ def run(lib):
     assert "serve ok" in r.stdout
     assert "admission ok" in r.stdout
     assert "noalloc ok" in r.stdout
+    assert "index ok" in r.stdout
 
 
 @pytest.mark.parametrize("variant", [[], ["-DSDBFX_WARM=12", "-DSDBFX_INSERT_REPLAY=1"]],
