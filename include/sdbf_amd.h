@@ -6,9 +6,15 @@
 // header name).
 //
 // Differences from the reference, all outside the hot path:
-//   - index_info (--index / --index-search): the constructors take one
-//     without an index (index NULL, no set list), as the reference CLI
-//     passes it, and throw -1 for one that asks for indexing.
+//   - index_info (--index / --index-search): the indexes are used on the
+//     device; an info->index is copied to the device for each digest call
+//     and its bits and element count copied back afterwards, so its content
+//     is the reference's after every constructor or hash_batch call.  A set
+//     list entry without an index throws -1 (the reference dereferences it).
+//     bloom_filter::write_out writes the reference's .idx header and a valid
+//     LZ4 block of the bits (any LZ4 decoder, the reference's included,
+//     reads it back), which need not be byte-identical with lz4's own
+//     compressor output.
 //   - sampling (compare(other, sample), compare_to(..., sample_size, ...))
 //     gives the reference's scores: a calling digest with more than `sample`
 //     filters is scored on its first `sample` filters; the reference's
@@ -57,6 +63,12 @@ public:
     bloom_filter(uint64_t size, uint16_t hash_count, uint64_t max_elem, double max_fp);
     // copy of a 256-byte sdbf filter (sdbf/bloom_filter.cc:125-144)
     bloom_filter(uint8_t *data, uint64_t size, int id, int bf_elem_ct, uint16_t hamming);
+    // reads an index file written by write_out (sdbf/bloom_filter.cc:77-114):
+    // throws -1 when it cannot be read, -2 when it does not decompress
+    bloom_filter(std::string indexfilename);
+    // writes the index file (sdbf/bloom_filter.cc:208-228): -1 when the
+    // compression fails, -2 when the file cannot be written
+    int32_t write_out(std::string filename);
     ~bloom_filter();
     bool insert_sha1(uint32_t *sha1);
     bool query_sha1(uint32_t *sha1);
@@ -83,6 +95,7 @@ public:
     uint16_t hash_count;
 
 private:
+    friend class sdbf;
     bool query_and_set(uint32_t *sha1, bool mode_set);
     double max_fp = 0;
     uint64_t bf_elem_count = 0;
@@ -130,7 +143,7 @@ public:
     uint64_t input_size();
     int32_t compare(sdbf *other, uint32_t sample);
     std::string to_string() const;
-    std::string get_index_results() const { return std::string(); }
+    std::string get_index_results() const { return index_results; }
     uint8_t *clone_filter(uint32_t position);
     uint32_t filter_count();
     // fast mode (sdbf/sdbf_class.cc:426-439): folds every filter to 64 bytes
@@ -147,9 +160,11 @@ public:
     // digest's input_size() when it differs from the digested length, as
     // for the stream constructor, whose digest covers msize bytes of which
     // it read fewer.
+    // With an index_info (info), the inputs are digested in the order given
+    // against its indexes, as the constructors do one after another.
     static void hash_batch(const std::vector<std::string> &names, const std::vector<const uint8_t *> &bufs,
                            const std::vector<uint64_t> &lens, uint32_t dd_block_size, std::vector<sdbf *> &out,
-                           const std::vector<uint64_t> *orig_sizes = nullptr);
+                           const std::vector<uint64_t> *orig_sizes = nullptr, index_info *info = nullptr);
 
 public:
     uint8_t *buffer = nullptr;     // bf_count * 256 filter bytes
@@ -166,7 +181,13 @@ public:
 private:
     sdbf() = default;
     void from_device(sdbfx_set *set, uint32_t idx, const char *name);
-    void hash_buffer(const char *name, const uint8_t *buf, uint64_t len, uint32_t dd);
+    void hash_buffer(const char *name, const uint8_t *buf, uint64_t len, uint32_t dd, index_info *info);
+    // one device digest call (with info's indexes when it asks for them);
+    // results[k] gets the print_indexes lines of the k-th input
+    static sdbfx_set *digest_inputs(sdbfx_ctx *ctx, const std::vector<const char *> &names,
+                                    const std::vector<const uint8_t *> &bufs, const std::vector<uint64_t> &lens,
+                                    const std::vector<uint32_t> &dd, std::vector<int32_t> &status, index_info *info,
+                                    std::vector<std::string> &results);
 
     char *hashname = nullptr;
     uint32_t bf_count = 0;
@@ -178,6 +199,7 @@ private:
     uint32_t dd_block_size = 0;
     uint64_t orig_file_size = 0;
     bool fastmode = false;
+    std::string index_results;   // print_indexes lines of --index-search
 };
 
 class sdbf_set {
@@ -202,7 +224,7 @@ public:
     std::string compare_to_quiet(sdbf_set *other, int32_t threshold, uint32_t sample_size, int32_t thread_count,
                                  bool fast);
     std::string to_string() const;
-    std::string index_results() const { return std::string(); }
+    std::string index_results() const;
     std::string name() const;
     void set_name(std::string name);
     void set_separator(char sep);

@@ -18,8 +18,14 @@ _SO = os.environ.get("SDBF_ORACLE_SO") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 
+REF_LZ4 = os.path.join(_HERE, "_ref", "lz4check")
+
+
 def build() -> None:
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    # the reference's own LZ4 (index files), where the sdhash tree is present
+    if os.path.exists("/root/reference/lz4/lz4.cc"):
+        subprocess.run(["make", "-s", "-C", _HERE, "ref"], check=True)
 
 
 def lib():

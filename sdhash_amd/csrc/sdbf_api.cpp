@@ -120,6 +120,118 @@ void check(sdbfx_ctx *ctx, int rc)
     }
 }
 
+// LZ4 block format (the published "LZ4 Block Format Description"), which the
+// reference's index files hold (bloom_filter.cc:176-200 through lz4.cc).
+// A block is a run of sequences: a token (literal count in the high nibble,
+// match length - 4 in the low one; 15 continues in following bytes, each
+// added, until one below 255), the literals, a 2-byte little-endian offset
+// back into the output and the match length's continuation bytes.  The last
+// sequence holds literals only; the last 5 bytes are literals and no match
+// starts within the last 12 bytes.
+void lz4_length(std::vector<uint8_t> &o, size_t v)
+{
+    while (v >= 255) {
+        o.push_back(255);
+        v -= 255;
+    }
+    o.push_back((uint8_t)v);
+}
+
+// Greedy compressor: each 4-byte sequence is looked up in a table of the
+// last position with the same hash.
+std::vector<uint8_t> lz4_compress(const uint8_t *src, size_t n)
+{
+    std::vector<uint8_t> o;
+    o.reserve(n / 32 + 64);
+    std::vector<uint32_t> table(1u << 16, 0xFFFFFFFFu);
+    size_t anchor = 0, i = 0;
+    auto rd32 = [&](size_t p) {
+        uint32_t v;
+        memcpy(&v, src + p, 4);
+        return v;
+    };
+    auto emit = [&](size_t lit_end, size_t off, size_t mlen) {
+        const size_t ll = lit_end - anchor;
+        uint8_t tok = (uint8_t)((ll >= 15 ? 15 : ll) << 4);
+        if (mlen)
+            tok |= (uint8_t)(mlen - 4 >= 15 ? 15 : mlen - 4);
+        o.push_back(tok);
+        if (ll >= 15)
+            lz4_length(o, ll - 15);
+        o.insert(o.end(), src + anchor, src + lit_end);
+        if (mlen) {
+            o.push_back((uint8_t)(off & 0xFF));
+            o.push_back((uint8_t)(off >> 8));
+            if (mlen - 4 >= 15)
+                lz4_length(o, mlen - 4 - 15);
+        }
+    };
+    if (n > 12) {
+        const size_t limit = n - 12, mlimit = n - 5;
+        while (i < limit) {
+            const uint32_t v = rd32(i);
+            const uint32_t h = (v * 2654435761u) >> 16;
+            const uint32_t c = table[h];
+            table[h] = (uint32_t)i;
+            if (c != 0xFFFFFFFFu && i - c <= 65535 && rd32(c) == v) {
+                size_t m = 4;
+                while (i + m < mlimit && src[c + m] == src[i + m])
+                    m++;
+                emit(i, i - c, m);
+                i += m;
+                anchor = i;
+            } else {
+                i++;
+            }
+        }
+    }
+    emit(n, 0, 0);
+    return o;
+}
+
+// 0 when src[0, srclen) decodes to exactly dstlen bytes, else -1
+int lz4_decompress(const uint8_t *src, size_t srclen, uint8_t *dst, size_t dstlen)
+{
+    size_t ip = 0, op = 0;
+    auto more = [&](size_t &v) {
+        uint8_t b;
+        do {
+            if (ip >= srclen)
+                return false;
+            b = src[ip++];
+            v += b;
+        } while (b == 255);
+        return true;
+    };
+    while (ip < srclen) {
+        const uint8_t tok = src[ip++];
+        size_t ll = tok >> 4;
+        if (ll == 15 && !more(ll))
+            return -1;
+        if (ll > srclen - ip || ll > dstlen - op)
+            return -1;
+        memcpy(dst + op, src + ip, ll);
+        ip += ll;
+        op += ll;
+        if (ip == srclen)
+            break;
+        if (srclen - ip < 2)
+            return -1;
+        const size_t off = (size_t)src[ip] | (size_t)src[ip + 1] << 8;
+        ip += 2;
+        size_t ml = tok & 15;
+        if (ml == 15 && !more(ml))
+            return -1;
+        ml += 4;
+        if (off == 0 || off > op || ml > dstlen - op)
+            return -1;
+        for (size_t k = 0; k < ml; k++)
+            dst[op + k] = dst[op + k - off];
+        op += ml;
+    }
+    return op == dstlen ? 0 : -1;
+}
+
 }  // namespace
 
 // ----------------------------------------------------------------- bloom_filter
@@ -159,6 +271,55 @@ bloom_filter::bloom_filter(uint8_t *data, uint64_t size, int id, int bf_elem_ct,
     memcpy(bf, data, size);
     bf_elem_count = (uint64_t)bf_elem_ct;
     bl_id = id;
+}
+
+// sdbf/bloom_filter.cc:77-114: "sdbf-idx:<size>:<elements>:<hash count>:
+// <bit mask>:<compressed size>:<name>" on one line, then the compressed bits
+bloom_filter::bloom_filter(std::string indexfilename) : hamming(0), hamminglg(0), bf_size(0), max_elem(0), hash_count(5)
+{
+    std::ifstream ifs(indexfilename.c_str(), std::ios::in | std::ios::binary);
+    if (!ifs.is_open())
+        throw -1;
+    std::string f[6];
+    for (int k = 0; k < 6; k++)
+        std::getline(ifs, f[k], ':');
+    std::getline(ifs, setname);
+    uint64_t comp = 0;
+    try {
+        bf_size = std::stoull(f[1]);
+        bf_elem_count = std::stoull(f[2]);
+        hash_count = (uint16_t)std::stoul(f[3]);
+        bit_mask = std::stoull(f[4]);
+        comp = std::stoull(f[5]);
+    } catch (...) {
+        throw -2;
+    }
+    if (f[0] != "sdbf-idx" || !bf_size || (bf_size & (bf_size - 1)))
+        throw -2;
+    std::vector<uint8_t> c(comp);
+    ifs.read((char *)c.data(), (std::streamsize)comp);
+    if ((uint64_t)ifs.gcount() != comp)
+        throw -2;
+    bf = (uint8_t *)malloc(bf_size);
+    if (lz4_decompress(c.data(), comp, bf, bf_size) != 0) {
+        free(bf);
+        throw -2;
+    }
+}
+
+// sdbf/bloom_filter.cc:208-228
+int32_t bloom_filter::write_out(std::string filename)
+{
+    const std::vector<uint8_t> c = lz4_compress(bf, bf_size);
+    if (c.empty())
+        return -1;
+    std::ofstream os(filename.c_str(), std::ios::out | std::ios::binary);
+    if (!os.is_open())
+        return -2;
+    os << "sdbf-idx:" << bf_size << ":" << bf_elem_count << ":" << hash_count << ":" << bit_mask << ":" << c.size()
+       << ":" << setname << std::endl;
+    os.write((const char *)c.data(), (std::streamsize)c.size());
+    return os.good() ? 0 : -2;
 }
 
 bloom_filter::~bloom_filter() { free(bf); }
@@ -286,44 +447,132 @@ void sdbf::from_device(sdbfx_set *set, uint32_t idx, const char *name)
     }
 }
 
-void sdbf::hash_buffer(const char *name, const uint8_t *buf, uint64_t len, uint32_t dd)
+// The digests of inputs [0, n) in a new device set.  With an index_info
+// that asks for indexing (sdbf_core.cc:110-115, 157-190): info->index is
+// copied to the device, gates / takes the inserts of the inputs in order, and
+// is copied back; the indexes of info->setlist are looked up by block-mode
+// inputs, and each block with more than _FP_THRESHOLD (4) hits in an index
+// gives a print_indexes line "<name> [<block>] |<set name>|<hits>" (sdbf_core.
+// cc:458-472), unless search_first or search_deep is set (:184-187).
+sdbfx_set *sdbf::digest_inputs(sdbfx_ctx *ctx, const std::vector<const char *> &names,
+                               const std::vector<const uint8_t *> &bufs, const std::vector<uint64_t> &lens,
+                               const std::vector<uint32_t> &dd, std::vector<int32_t> &status, index_info *info,
+                               std::vector<std::string> &results)
+{
+    const uint32_t n = (uint32_t)bufs.size();
+    status.assign(n, 0);
+    results.assign(n, std::string());
+    sdbfx_set *set = nullptr;
+    check(ctx, sdbfx_set_create(ctx, &set));
+    if (!indexing(info)) {
+        const int rc = sdbfx_set_digest(ctx, set, n, bufs.data(), lens.data(), names.data(), dd.data(), status.data());
+        if (rc != 0) {
+            sdbfx_set_destroy(set);
+            check(ctx, rc);
+        }
+        return set;
+    }
+    sdbfx_index *ix = nullptr;
+    std::vector<sdbfx_index *> search;
+    std::vector<sdbf_set *> sets;
+    auto release = [&]() {
+        sdbfx_index_destroy(ix);
+        for (sdbfx_index *p : search)
+            sdbfx_index_destroy(p);
+    };
+    auto to_device = [&](bloom_filter *bf) {
+        sdbfx_index *d = nullptr;
+        if (sdbfx_index_create(ctx, bf->bf_size, &d) != 0 ||
+            sdbfx_index_write(ctx, d, bf->bf, bf->bf_elem_count) != 0) {
+            sdbfx_index_destroy(d);
+            return (sdbfx_index *)nullptr;
+        }
+        return d;
+    };
+    int rc = 0;
+    if (info->index && !(ix = to_device(info->index)))
+        rc = -1;
+    if (info->setlist)
+        for (sdbf_set *ss : *info->setlist) {
+            if (rc == 0 && (!ss || !ss->index))
+                rc = -1;
+            if (rc == 0) {
+                sdbfx_index *d = to_device(ss->index);
+                if (!d)
+                    rc = -1;
+                else
+                    search.push_back(d), sets.push_back(ss);
+            }
+        }
+    uint64_t nblk = 0;
+    for (uint32_t i = 0; i < n; i++)
+        if (dd[i] && lens[i] >= 512)
+            nblk += lens[i] / dd[i] + (lens[i] % dd[i] >= 512 ? 1 : 0);
+    const uint32_t ns = (uint32_t)search.size();
+    std::vector<uint32_t> matches(std::max<uint64_t>(nblk * ns, 1));
+    if (rc == 0)
+        rc = sdbfx_set_digest_indexed(ctx, set, n, bufs.data(), lens.data(), names.data(), dd.data(), ix,
+                                      (const sdbfx_index *const *)search.data(), ns, matches.data(), nblk,
+                                      status.data());
+    if (rc == 0 && ix) {
+        uint64_t e = 0;
+        rc = sdbfx_index_read(ctx, ix, info->index->bf, &e);
+        info->index->bf_elem_count = e;
+    }
+    release();
+    if (rc != 0) {
+        sdbfx_set_destroy(set);
+        check(ctx, rc);
+    }
+    if (ns && !info->search_first && !info->search_deep) {
+        uint64_t b = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            if (!dd[i] || lens[i] < 512)
+                continue;
+            const uint64_t cnt = lens[i] / dd[i] + (lens[i] % dd[i] >= 512 ? 1 : 0);
+            std::stringstream build;
+            for (uint64_t k = 0; k < cnt; k++, b++)
+                for (uint32_t m = 0; m < ns; m++)
+                    if (matches[b * ns + m] > 4)
+                        build << names[i] << " [" << k << "] |" << sets[m]->name() << "|" << matches[b * ns + m]
+                              << std::endl;
+            results[i] = build.str();
+        }
+    }
+    return set;
+}
+
+void sdbf::hash_buffer(const char *name, const uint8_t *buf, uint64_t len, uint32_t dd, index_info *info)
 {
     std::lock_guard<std::mutex> g(config->lock);
     sdbfx_ctx *ctx = config->ctx();
-    sdbfx_set *set = nullptr;
-    check(ctx, sdbfx_set_create(ctx, &set));
-    int32_t status = 0;
     const char *nm = name ? name : "";
-    const int rc = sdbfx_set_digest(ctx, set, 1, &buf, &len, &nm, &dd, &status);
-    if (rc != 0 || status != 0) {
+    std::vector<int32_t> status;
+    std::vector<std::string> res;
+    sdbfx_set *set = digest_inputs(ctx, {nm}, {buf}, {len}, {dd}, status, info, res);
+    if (status[0] != 0) {
         sdbfx_set_destroy(set);
-        if (rc != 0)
-            check(ctx, rc);
-        throw (int)status;
+        throw (int)status[0];
     }
     from_device(set, 0, nm);
+    index_results = res[0];
     sdbfx_set_destroy(set);
 }
 
 void sdbf::hash_batch(const std::vector<std::string> &names, const std::vector<const uint8_t *> &bufs,
                       const std::vector<uint64_t> &lens, uint32_t dd, std::vector<sdbf *> &out,
-                      const std::vector<uint64_t> *orig_sizes)
+                      const std::vector<uint64_t> *orig_sizes, index_info *info)
 {
     std::lock_guard<std::mutex> g(config->lock);
     sdbfx_ctx *ctx = config->ctx();
     const uint32_t n = (uint32_t)bufs.size();
     std::vector<const char *> nm(n);
     std::vector<uint32_t> ddv(n, dd);
-    std::vector<int32_t> status(n);
     for (uint32_t i = 0; i < n; i++)
         nm[i] = names[i].c_str();
-    sdbfx_set *set = nullptr;
-    check(ctx, sdbfx_set_create(ctx, &set));
-    const int rc = sdbfx_set_digest(ctx, set, n, bufs.data(), lens.data(), nm.data(), ddv.data(), status.data());
-    if (rc != 0) {
-        sdbfx_set_destroy(set);
-        check(ctx, rc);
-    }
+    std::vector<int32_t> status;
+    std::vector<std::string> res;
+    sdbfx_set *set = digest_inputs(ctx, nm, bufs, lens, ddv, status, info, res);
     out.assign(n, nullptr);
     uint32_t k = 0;
     for (uint32_t i = 0; i < n; i++) {
@@ -333,6 +582,7 @@ void sdbf::hash_batch(const std::vector<std::string> &names, const std::vector<c
         s->from_device(set, k++, nm[i]);
         if (orig_sizes)
             s->orig_file_size = (*orig_sizes)[i];
+        s->index_results = res[i];
         out[i] = s;
     }
     sdbfx_set_destroy(set);
@@ -349,32 +599,28 @@ sdbf::sdbf(const char *filename, uint32_t dd)
     is.read((char *)data.data(), (std::streamsize)data.size());
     if ((uint64_t)is.gcount() != (uint64_t)st.st_size)
         throw -1;
-    hash_buffer(filename, data.data(), data.size(), dd);
+    hash_buffer(filename, data.data(), data.size(), dd, nullptr);
 }
 
 // sdbf/sdbf_class.cc:71-102: reads msize bytes; fewer than 512 throw -3; the
 // digest covers all msize bytes (zero filled past the read)
 sdbf::sdbf(const char *name, std::istream *ifs, uint32_t dd, uint64_t msize, index_info *info)
 {
-    if (indexing(info))
-        throw -1;
     std::vector<uint8_t> data(msize, 0);
     ifs->read((char *)data.data(), (std::streamsize)msize);
     const uint64_t got = (uint64_t)ifs->gcount();
     if (got < 512)
         throw -3;
-    hash_buffer(name, data.data(), msize, dd);
+    hash_buffer(name, data.data(), msize, dd, info);
     orig_file_size = got;
 }
 
 // sdbf/sdbf_class.cc:115-136
 sdbf::sdbf(const char *name, char *str, uint32_t dd, uint64_t length, index_info *info)
 {
-    if (indexing(info))
-        throw -1;
     if (length < 512)
         throw -3;
-    hash_buffer(name, (const uint8_t *)str, length, dd);
+    hash_buffer(name, (const uint8_t *)str, length, dd, info);
 }
 
 // sdbf/sdbf_class.cc:144-215
@@ -656,6 +902,15 @@ std::ostream &operator<<(std::ostream &os, const sdbf_set &s) { return os << s.t
 std::ostream &operator<<(std::ostream &os, const sdbf_set *s) { return os << s->to_string(); }
 
 std::string sdbf_set::name() const { return setname; }
+
+// sdbf/sdbf_set.cc:161-167: the items' print_indexes lines, in set order
+std::string sdbf_set::index_results() const
+{
+    std::string r;
+    for (const sdbf *it : items)
+        r += it->get_index_results();
+    return r;
+}
 void sdbf_set::set_name(std::string name) { setname = name; }
 void sdbf_set::set_separator(char s) { sep = s; }
 

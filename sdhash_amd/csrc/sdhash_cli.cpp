@@ -9,14 +9,15 @@
 // -s/--sample-size N, -b/--block-size KB, -p/--threads N (accepted; the
 // device does the work),
 // -z/--segment-size MB, -o/--output BASE, --separator pipe|csv|tab,
-// --hash-name NAME (stdin), --fast, --validate, --verbose, --version,
-// -h/--help.
+// --hash-name NAME (stdin), --fast, --validate, --index, --index-search DIR,
+// --verbose, --version, -h/--help.
 // File policy as sdhash.cc:502-618: inputs below 16 MiB are stream-hashed
 // unless -b > 0; larger ones use 16 KiB blocks unless -b 0 (or -b N KiB),
 // cut into segments named "<file>.NNNNM" (sdhash_threads.cc:300-373).
 #include <dirent.h>
 #include <sys/stat.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -34,12 +35,12 @@ namespace {
 const uint64_t KB = 1024, MB = 1024 * 1024;
 
 struct Opts {
-    bool deep = false, gen = false, cmp = false, validate = false, verbose = false, fast = false;
+    bool deep = false, gen = false, cmp = false, validate = false, verbose = false, fast = false, index = false;
     int32_t threshold = 1;
     uint32_t sample = 0;           // -s: filters of each query digest compared (0 = all)
     int32_t dd = -1;               // KiB; -1 = automatic
     uint64_t segment = 128 * MB;
-    std::string output, separator = "pipe", hash_name;
+    std::string output, separator = "pipe", hash_name, index_dir;
     std::vector<std::string> inputs;
 };
 
@@ -60,6 +61,8 @@ void usage()
                  "  --hash-name NAME        name of the digest of stdin\n"
                  "  --fast                  shrink sdbf filters for speedup\n"
                  "  --validate              parse SDBF files and report their contents\n"
+                 "  --index                 generate indexes while hashing (needs -o)\n"
+                 "  --index-search DIR      search the reference indexes in DIR\n"
                  "  --verbose               progress output\n";
 }
 
@@ -104,6 +107,12 @@ bool parse(int argc, char **argv, Opts &o)
             o.cmp = true;
         } else if (a == "--validate") {
             o.validate = true;
+        } else if (a == "--index") {
+            o.index = true;
+        } else if (is(nullptr, "--index-search")) {
+            if (!need("index-search"))
+                return false;
+            o.index_dir = val;
         } else if (a == "--fast") {
             o.fast = true;   // sdhash.cc:268-273
         } else if (a == "--verbose") {
@@ -258,8 +267,9 @@ void segment_jobs(const std::string &name, const std::vector<uint8_t> &file, uin
     }
 }
 
-// Digests jobs in one device batch and appends them to `set` (or prints).
-int run_jobs(std::vector<Job> &jobs, uint32_t dd_bytes, sdbf_set *set)
+// Digests jobs in one device batch and appends them to `set` (or prints);
+// with info, against its indexes, in order.
+int run_jobs(std::vector<Job> &jobs, uint32_t dd_bytes, sdbf_set *set, index_info *info = nullptr)
 {
     std::vector<std::string> names;
     std::vector<const uint8_t *> bufs;
@@ -276,7 +286,7 @@ int run_jobs(std::vector<Job> &jobs, uint32_t dd_bytes, sdbf_set *set)
     }
     std::vector<sdbf *> out;
     if (!bufs.empty())
-        sdbf::hash_batch(names, bufs, lens, dd_bytes, out, &got);
+        sdbf::hash_batch(names, bufs, lens, dd_bytes, out, &got, info);
     for (size_t k = 0; k < out.size(); k++) {
         if (!out[k])
             continue;
@@ -302,6 +312,105 @@ int write_file(const std::string &s, const std::string &fn)
     return 0;
 }
 
+// sdhash_threads.cc:376-492 (hash_index_stringlist), run in the order given
+// (sdhash -p 1): files are grouped, a group closing once the sum of its file
+// sizes passes 640 MB or at the last file; each group is hashed against a new
+// 64 MiB index into <output><n>.sdbf, whose index goes to <output><n>.sdbf.idx.
+// Files below 16 MiB are stream-hashed; larger ones use 16 KiB blocks and
+// 128 MiB segments.  As in the reference, every group after the first also
+// starts with the first file of the first group (its file list is refilled
+// from the second slot on), and the numbering restarts at 0 for the large
+// files, whose outputs replace the small files' of the same number.
+int hash_index_list(const std::vector<std::string> &files, const std::string &output)
+{
+    std::vector<std::string> small, large;
+    std::vector<uint64_t> ssz, lsz;
+    for (const auto &f : files) {
+        uint64_t sz;
+        if (!regular(f, &sz))
+            continue;
+        if (sz < 16 * MB)
+            small.push_back(f), ssz.push_back(sz);
+        else
+            large.push_back(f), lsz.push_back(sz);
+    }
+    for (int pass = 0; pass < 2; pass++) {
+        const std::vector<std::string> &lst = pass ? large : small;
+        const std::vector<uint64_t> &szs = pass ? lsz : ssz;
+        std::vector<std::string> group;
+        uint64_t total = 0;
+        int count = 0;
+        for (size_t i = 0; i < lst.size(); i++) {
+            group.push_back(lst[i]);
+            total += szs[i];
+            if (!(total > 640 * MB || i == lst.size() - 1))
+                continue;
+            bloom_filter *index1 = new bloom_filter(64 * MB, 5, 0, 0.01);
+            index_info info{index1, nullptr, nullptr, false, false, false};
+            sdbf_set *set1 = new sdbf_set(index1);
+            std::vector<Job> jobs;
+            for (const auto &f : group) {
+                std::vector<uint8_t> d;
+                if (!read_file(f, d))
+                    continue;
+                if (pass) {
+                    // large files one at a time (bounded host memory); the
+                    // index carries over from call to call
+                    segment_jobs(f, d, 128 * MB, jobs);
+                    run_jobs(jobs, 16 * KB, set1, &info);
+                } else {
+                    const uint64_t got = d.size();
+                    jobs.push_back(Job{f, std::move(d), got});
+                }
+            }
+            if (!jobs.empty())
+                run_jobs(jobs, 0, set1, &info);
+            const std::string nm = output + std::to_string(count) + ".sdbf";
+            count++;
+            if (write_file(set1->to_string(), nm) != 0)
+                return -1;
+            set1->index->set_name(nm);
+            if (set1->index->write_out(nm + ".idx") == -2) {
+                std::cerr << "sdhash: ERROR cannot write to file " << nm + ".idx" << std::endl;
+                return -1;
+            }
+            for (uint64_t k = 0; k < set1->size(); k++)
+                delete set1->at((uint32_t)k);
+            delete set1;
+            delete index1;
+            group.assign(1, lst[0]);
+            total = 0;
+        }
+    }
+    return 0;
+}
+
+// --index-search DIR (sdhash.cc:241-264): every <name>.idx file in DIR (in
+// name order; the reference takes the directory's own order) is loaded with
+// the set of DIR/<name> that it indexes, named by that path.
+bool load_index_dir(const std::string &dir, std::vector<sdbf_set *> &setlist)
+{
+    DIR *d = opendir(dir.c_str());
+    if (!d)
+        return true;
+    std::vector<std::string> names;
+    while (dirent *e = readdir(d)) {
+        const std::string n = e->d_name;
+        uint64_t sz;
+        if (n.size() > 4 && n.compare(n.size() - 4, 4, ".idx") == 0 && regular(dir + "/" + n, &sz))
+            names.push_back(n);
+    }
+    closedir(d);
+    std::sort(names.begin(), names.end());
+    for (const auto &n : names) {
+        bloom_filter *b = new bloom_filter(dir + "/" + n);
+        sdbf_set *s = new sdbf_set((dir + "/" + n.substr(0, n.size() - 4)).c_str());
+        s->index = b;
+        setlist.push_back(s);
+    }
+    return true;
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
@@ -311,7 +420,26 @@ int main(int argc, char **argv)
         return 1;
     sdbf::config->warnings = o.verbose ? 1 : 0;
     const char sep = sep_char(o.separator);
+    // sdhash.cc:195-198
+    if (o.index && o.output.empty()) {
+        std::cerr << "sdhash:  ERROR: indexing requires output base filename " << std::endl;
+        return -1;
+    }
     try {
+        std::vector<sdbf_set *> setlist;
+        index_info search_info{nullptr, nullptr, &setlist, false, false, false};
+        index_info *info = nullptr;
+        if (!o.index_dir.empty()) {
+            // sdhash.cc:241-250
+            if (o.dd == 0) {
+                std::cerr << "ERROR: Index searching only supported in block mode" << std::endl;
+                return 0;
+            }
+            if (o.dd == -1)
+                o.dd = 16;
+            load_index_dir(o.index_dir, setlist);
+            info = &search_info;
+        }
         if (o.cmp) {
             if (o.inputs.size() == 1) {
                 sdbf_set *s1 = new sdbf_set(o.inputs[0].c_str());
@@ -348,7 +476,25 @@ int main(int argc, char **argv)
             }
             return 0;
         }
-        const bool keep = o.gen || !o.output.empty();
+        if (o.index) {
+            // sdhash.cc:556-561: files below 16 MiB, then the larger ones
+            std::vector<std::string> files, small, large;
+            for (const auto &in : o.inputs) {
+                std::vector<std::string> found;
+                struct stat st;
+                if (o.deep && stat(in.c_str(), &st) == 0 && S_ISDIR(st.st_mode))
+                    walk(in, found);
+                uint64_t sz;
+                if (regular(in, &sz))
+                    found.push_back(in);
+                for (const auto &f : found)
+                    if (regular(f, &sz))
+                        (sz < 16 * MB ? small : large).push_back(f);
+            }
+            const int a = hash_index_list(small, o.output), b = hash_index_list(large, o.output);
+            return a + b;
+        }
+        const bool keep = o.gen || !o.output.empty() || info;
         sdbf_set *set1 = keep ? new sdbf_set() : nullptr;
         std::vector<Job> stream_jobs, block_jobs;
         uint32_t block_bytes = 0;
@@ -368,7 +514,7 @@ int main(int argc, char **argv)
                 ns.fill('0');
                 ns << base << "." << std::setw(4) << j * o.segment / MB << "M";
                 std::vector<Job> jobs{Job{ns.str(), std::vector<uint8_t>(buf.begin(), buf.begin() + got), got}};
-                run_jobs(jobs, dd * KB, set1);
+                run_jobs(jobs, dd * KB, set1, info);
                 if (got != buf.size())
                     break;
             }
@@ -410,9 +556,9 @@ int main(int argc, char **argv)
                     add_block(f, (uint32_t)o.dd);
             }
             if (!stream_jobs.empty())
-                run_jobs(stream_jobs, 0, set1);
+                run_jobs(stream_jobs, 0, set1, info);
             if (!block_jobs.empty())
-                run_jobs(block_jobs, block_bytes, set1);
+                run_jobs(block_jobs, block_bytes, set1, info);
             for (const auto &f : large) {
                 if (o.dd == 0)
                     add_stream(f);
@@ -420,9 +566,9 @@ int main(int argc, char **argv)
                     add_block(f, o.dd == -1 ? 16 : (uint32_t)o.dd);
                 // large inputs are digested one file at a time (bounded host memory)
                 if (!stream_jobs.empty())
-                    run_jobs(stream_jobs, 0, set1);
+                    run_jobs(stream_jobs, 0, set1, info);
                 if (!block_jobs.empty())
-                    run_jobs(block_jobs, block_bytes, set1);
+                    run_jobs(block_jobs, block_bytes, set1, info);
             }
         }
         if (o.gen) {
@@ -431,6 +577,12 @@ int main(int argc, char **argv)
                 write_file(set1->compare_all_quiet(o.threshold, 0, o.fast), o.output + ".compare");
             else
                 set1->compare_all(o.threshold, o.fast);
+        } else if (info) {
+            // sdhash.cc:640-652
+            if (!o.output.empty())
+                write_file(set1->index_results(), o.output + ".idx-result");
+            else
+                std::cerr << set1->index_results();
         } else if (!o.output.empty()) {
             write_file(set1->to_string(), o.output + ".sdbf");
         }

@@ -9,6 +9,8 @@
 //   caller_compat all <threshold> <fast 0|1> set.sdbf      compare_all_quiet
 //   caller_compat two <threshold> <sample> <fast 0|1> q.sdbf t.sdbf
 //   caller_compat api                                      member checks
+//   caller_compat idxw <size> <n> <seed> <file.idx>        index of n inserts, written out
+//   caller_compat idxr <file.idx> <bits.bin>               index file read back
 #include <sys/stat.h>
 
 #include <cstdlib>
@@ -230,6 +232,38 @@ int main(int argc, char **argv)
         delete set2;
         delete set1;
         delete index1;
+    } else if (mode == "idxw" && argc == 6) {
+        // bloom_filter as an index (sdhash_threads.cc:409,430-437): inserts of
+        // xorshift64 words, then write_out
+        bloom_filter *ix = new bloom_filter((uint64_t)atoll(argv[2]), 5, 0, 0.01);
+        uint64_t x = (uint64_t)atoll(argv[4]);
+        for (long k = 0; k < atol(argv[3]); k++) {
+            uint32_t h[5];
+            for (int j = 0; j < 5; j++) {
+                x ^= x << 13;
+                x ^= x >> 7;
+                x ^= x << 17;
+                h[j] = (uint32_t)x;
+            }
+            ix->insert_sha1(h);
+        }
+        ix->set_name("set.sdbf");
+        if (ix->write_out(argv[5]) != 0)
+            return 9;
+        std::cout << ix->elem_count() << std::endl;
+        delete ix;
+    } else if (mode == "idxr" && argc == 4) {
+        bloom_filter *ix;
+        try {
+            ix = new bloom_filter(std::string(argv[2]));
+        } catch (int e) {
+            std::cout << "throw " << e << std::endl;
+            return 0;
+        }
+        std::ofstream os(argv[3], std::ios::binary);
+        os.write((const char *)ix->bf, (std::streamsize)ix->bf_size);
+        std::cout << ix->bf_size << " " << ix->elem_count() << " " << ix->bit_mask << " " << ix->name() << std::endl;
+        delete ix;
     } else {
         return 2;
     }

@@ -123,3 +123,58 @@ def test_cli_hash_and_compare(tmp_path):
     d = oracle.digest_block(files[0], 16384, b"stdin.0000M")
     assert out == d.to_string()
     assert run([caller, "api"], tmp_path).decode().startswith("api ok")
+
+
+def idx_bits(caller, path, tmp_path):
+    out = subprocess.run([caller, "idxr", path, str(tmp_path / "idx.bin")], capture_output=True, text=True,
+                         check=True).stdout.split()
+    return np.fromfile(str(tmp_path / "idx.bin"), np.uint8), int(out[1]), out[3]
+
+
+@pytest.mark.gpu
+def test_cli_index_and_index_search(tmp_path):
+    # --index (sdhash_threads.cc:376-492): small files stream-hashed in order
+    # against one 64 MiB index -> out0.sdbf and out0.sdbf.idx; a large file
+    # in 16 KiB blocks -> big0.sdbf(.idx); then --index-search of block-mode
+    # queries against the directory (sdhash.cc:241-264, 640-652)
+    caller = build_caller(str(tmp_path / "caller_compat"))
+    img = synth.disk_image(310, 3 * MIB)
+    rng = np.random.Generator(np.random.PCG64(311))
+    small = [img[:700 * KIB], img[500 * KIB:1300 * KIB], np.concatenate([img[:200 * KIB], synth.segment_mix(rng, 300 * KIB)]),
+             img[2 * MIB:]]
+    names = ["s%d.bin" % k for k in range(len(small))]
+    for n, f in zip(names, small):
+        (tmp_path / n).write_bytes(f.tobytes())
+    big = synth.disk_image(312, 17 * MIB)
+    (tmp_path / "big.bin").write_bytes(big.tobytes())
+    (tmp_path / "ix").mkdir()
+    run([CLI, "--index", "-o", "ix/out"] + names, tmp_path)
+    oix = oracle.Index(64 << 20)
+    want = "".join(oracle.digest_stream_indexed(f, oix, n.encode()).to_string() for f, n in zip(small, names))
+    assert (tmp_path / "ix" / "out0.sdbf").read_text() == want
+    bits, ec, nm = idx_bits(caller, str(tmp_path / "ix" / "out0.sdbf.idx"), tmp_path)
+    assert ec == oix.elem_count and nm == "ix/out0.sdbf" and np.array_equal(bits, oix.bf)
+    run([CLI, "--index", "-o", "big"] + ["big.bin"], tmp_path)
+    bix = oracle.Index(64 << 20)
+    d = oracle.digest_block_indexed(big, 16384, b"big.bin", index=bix)[0]
+    assert (tmp_path / "big0.sdbf").read_text() == d.to_string()
+    bits, ec, _ = idx_bits(caller, str(tmp_path / "big0.sdbf.idx"), tmp_path)
+    assert ec == bix.elem_count and np.array_equal(bits, bix.bf)
+    # search: queries cut from the indexed data, and fresh data.  (s1 and
+    # s2 begin with data that is already indexed: the gate skips those
+    # features after they set their filter bits, so the filter fills up and
+    # nothing after it reaches the index, as in the reference.)
+    qs = [img[100 * KIB:600 * KIB], img[2 * MIB + 100 * KIB:2 * MIB + 600 * KIB], synth.segment_mix(rng, 200 * KIB)]
+    qn = ["q%d.bin" % k for k in range(len(qs))]
+    for n, q in zip(qn, qs):
+        (tmp_path / n).write_bytes(q.tobytes())
+    r = subprocess.run([CLI, "--index-search", "ix"] + qn, cwd=tmp_path, capture_output=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    lines = []
+    for n, q in zip(qn, qs):
+        _, m = oracle.digest_block_indexed(q, 16384, n.encode(), search=[oix])
+        lines += ["%s [%d] |ix/out0.sdbf|%d" % (n, k, c) for k, c in enumerate(m[:, 0]) if c > 4]
+    assert len(lines) > 10
+    assert r.stderr.decode().splitlines() == lines
+    run([CLI, "--index-search", "ix", "-o", "res"] + qn, tmp_path)
+    assert (tmp_path / "res.idx-result").read_text().splitlines() == lines
