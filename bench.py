@@ -470,6 +470,135 @@ def allpairs(eng, args) -> dict:
     return out
 
 
+def index_section(eng, dimg, img) -> dict:
+    """sdhash --index / --index-search on the device, over the 1 GiB image in
+    HBM: (1) block-mode insertion of its eight 128 MiB segments at -b 16 into
+    a 64 MiB index (the --index hashing of large files); (2) the stream-mode
+    gate over 64 files of 1 MiB cut at random offsets of the image (so that
+    later files repeat features of earlier ones), in order, into a fresh
+    index; (3) the image at -b 16 searched against four 64 MiB indexes.  Rates
+    are input bytes per second of wall time around each call (inputs resident,
+    index scratch allocated by a warm-up call).  Each is checked against the
+    oracle: (1) and (3) on a 64 MiB slice of the image, (2) on all 64 files;
+    the oracle's times on those are the CPU baselines (one thread: the
+    reference's index inserts are ordered)."""
+    import oracle
+    from sdhash_amd import synth
+    nseg = IMAGE // SEGMENT
+    seg_off = [i * SEGMENT for i in range(nseg)]
+    names = [b"image.%04dM" % (i * SEGMENT // MIB) for i in range(nseg)]
+    zeros = np.zeros(64 << 20, np.uint8)
+    out = {}
+
+    def timed(fn, reps=3):
+        fn()
+        eng.sync()
+        best = 1e9
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            eng.sync()
+            best = min(best, time.perf_counter() - t)
+        return best
+
+    # (1) block-mode insertion
+    ix = eng.new_index(64 << 20)
+
+    def blk():
+        ix.write(zeros, 0)
+        s = eng.new_set()
+        s.digest_device_indexed(dimg, seg_off, [SEGMENT] * nseg, names, dd_block=16384, index=ix)
+        s.close()
+    t = timed(blk)
+    t_write = timed(lambda: ix.write(zeros, 0))
+    blk_s = t - t_write
+    # parity and CPU baseline on a 64 MiB slice
+    sl = 64 * MIB
+    ix.write(zeros, 0)
+    s = eng.new_set()
+    s.digest_device_indexed(dimg, [0], [sl], [b"slice"], dd_block=16384, index=ix)
+    gbits, gec = ix.read()
+    oix = oracle.Index(64 << 20)
+    tc = time.perf_counter()
+    od = oracle.digest_block_indexed(img[:sl], 16384, b"slice", index=oix)[0]
+    cpu_blk = time.perf_counter() - tc
+    if gec != oix.elem_count or not np.array_equal(gbits, oix.bf) or not np.array_equal(s.read(0)[0], od.filters):
+        raise RuntimeError("indexed block digest differs from the oracle's")
+    s.close()
+    out["block_insert"] = {"bytes": IMAGE, "wall_s": blk_s, "GBps": IMAGE / blk_s / 1e9,
+                           "cpu_oracle_MBps": sl / cpu_blk / 1e6, "cpu_sample": "64 MiB slice, 1 thread",
+                           "identical_sample": True}
+    # (2) the stream gate, 64 files of 1 MiB
+    rng = np.random.Generator(np.random.PCG64(61))
+    offs = [int(rng.integers(0, (IMAGE - MIB) // 256)) * 256 for _ in range(64)]
+    fnames = [b"f%02d" % k for k in range(64)]
+    six = eng.new_index(64 << 20)
+
+    def strm():
+        six.write(zeros, 0)
+        s = eng.new_set()
+        s.digest_device_indexed(dimg, offs, [MIB] * 64, fnames, dd_block=0, index=six)
+        s.close()
+    t = timed(strm)
+    t_write = timed(lambda: six.write(zeros, 0))
+    st_s = t - t_write
+    six.write(zeros, 0)
+    s = eng.new_set()
+    s.digest_device_indexed(dimg, offs, [MIB] * 64, fnames, dd_block=0, index=six)
+    gbits, gec = six.read()
+    oix = oracle.Index(64 << 20)
+    tc = time.perf_counter()
+    ods = [oracle.digest_stream_indexed(img[o:o + MIB], oix, n) for o, n in zip(offs, fnames)]
+    cpu_st = time.perf_counter() - tc
+    same = gec == oix.elem_count and np.array_equal(gbits, oix.bf)
+    for k, od in enumerate(ods):
+        same = same and s.info(k).bf_count == od.bf_count and np.array_equal(s.read(k)[0], od.filters)
+    if not same:
+        raise RuntimeError("indexed stream digests differ from the oracle's")
+    s.close()
+    out["stream_gate"] = {"files": 64, "bytes": 64 * MIB, "wall_s": st_s, "MBps": 64 * MIB / st_s / 1e6,
+                          "cpu_oracle_MBps": 64 * MIB / cpu_st / 1e6, "cpu_sample": "all 64 files, 1 thread",
+                          "identical": True}
+    # (3) search against four indexes: the image's own (1) and three built
+    # from 256 MiB regions of fresh device data
+    ixs = [ix]
+    gen = eng.alloc(256 * MIB)
+    for k in range(3):
+        synth.device_fill(gen.ptr.value, 256 * MIB, seed=71 + k)
+        x = eng.new_index(64 << 20)
+        s = eng.new_set()
+        s.digest_device_indexed(gen, [0, 128 * MIB], [128 * MIB] * 2, dd_block=16384, index=x)
+        s.close()
+        ixs.append(x)
+    gen.free()
+    ix.write(zeros, 0)
+    s = eng.new_set()
+    s.digest_device_indexed(dimg, seg_off, [SEGMENT] * nseg, names, dd_block=16384, index=ix)
+    s.close()
+
+    def srch():
+        s = eng.new_set()
+        st, m = s.digest_device_indexed(dimg, seg_off, [SEGMENT] * nseg, names, dd_block=16384, search=ixs)
+        s.close()
+        return m
+    sr_s = timed(srch)
+    m = srch()
+    host = [x.read() for x in ixs]
+    hix = [oracle.Index(64 << 20, b, e) for b, e in host]
+    tc = time.perf_counter()
+    _, om = oracle.digest_block_indexed(img[:sl], 16384, b"q", search=hix)
+    cpu_sr = time.perf_counter() - tc
+    if not np.array_equal(m[:om.shape[0]], om):
+        raise RuntimeError("index search counts differ from the oracle's")
+    for x in ixs:
+        x.close()
+    six.close()
+    out["search"] = {"bytes": IMAGE, "indexes": 4, "wall_s": sr_s, "GBps": IMAGE / sr_s / 1e9,
+                     "blocks_over_4": int((m > 4).sum()), "cpu_oracle_MBps": sl / cpu_sr / 1e6,
+                     "cpu_sample": "64 MiB slice against the four indexes, 1 thread", "identical_sample": True}
+    return out
+
+
 def plain_allpairs(eng) -> dict:
     """The plain method: 64 files of 1 MiB of segment mix with 0-50 %
     planted cross-file chunks (host-generated), uploaded to HBM, stream
@@ -552,6 +681,7 @@ def main() -> int:
     ap.add_argument("--no-allpairs", action="store_true", help="skip the all-pairs measurement")
     ap.add_argument("--allpairs-mfilters", type=float, default=2.0)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain 64 x 1 MiB all-pairs measurement")
+    ap.add_argument("--no-index", action="store_true", help="skip the --index / --index-search measurement")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -689,6 +819,9 @@ def main() -> int:
                              "algorithmic_ops_per_launch": pairs * 4096},
     }
     del tset, qset
+    if not args.no_index:
+        log("index ...")
+        out["index"] = index_section(eng, dimg, img)
     if not args.no_plain:
         log("plain all-pairs ...")
         out["plain_allpairs"] = plain_allpairs(eng)
