@@ -47,7 +47,8 @@ def build(verbose: bool = False) -> str:
     os.replace(SYNTH_LIB + ".tmp", SYNTH_LIB)
     # the C++ sdbf / sdbf_set classes over the C ABI, and the sdhash CLI
     cxx = ["g++", "-std=c++17", "-O2", "-fPIC", "-Wall", "-I", INCLUDE]
-    _run([*cxx, "-shared", os.path.join(CSRC, "sdbf_api.cpp"), "-o", CXXLIB + ".tmp", "-L", HERE,
+    _run([*cxx, "-shared", os.path.join(CSRC, "sdbf_api.cpp"), os.path.join(CSRC, "sdhash_srv.cpp"), "-o",
+          CXXLIB + ".tmp", "-L", HERE,
           "-l:libsdbfx.so", "-Wl,-rpath,$ORIGIN"], verbose)
     os.replace(CXXLIB + ".tmp", CXXLIB)
     _run([*cxx, os.path.join(CSRC, "sdhash_cli.cpp"), "-o", CLI + ".tmp", "-L", HERE, "-l:libsdbf_amd.so",
