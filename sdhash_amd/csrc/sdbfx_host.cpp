@@ -25,8 +25,13 @@
 
 namespace {
 
-// Largest input volume digested by one device pass (bounds scratch memory).
+// Largest input volume digested by one device pass (bounds scratch memory:
+// about 1.4 bytes per byte of stream-mode input).  Inputs from host memory
+// are staged into device memory pass by pass; device-resident inputs take
+// larger passes, since each pass ends with the ordered stream insertion,
+// whose length is that of its largest input's serial chain.
 constexpr uint64_t kPassBytes = 4ull << 30;
+constexpr uint64_t kPassBytesDevice = 16ull << 30;
 
 struct DevBuf {
     void* p = nullptr;
@@ -591,7 +596,8 @@ int digest_common(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t n, const uint8_t* d_b
         // with an index, a pass holds inputs of one mode only, so that the
         // index sees stream and block inputs in the order given
         auto blk = [&](uint32_t i) { return dd && dd[i]; };
-        while (i1 < n && (i1 == i0 || bytes + lens[i1] <= kPassBytes) &&
+        const uint64_t pass_bytes = host_bufs ? kPassBytes : kPassBytesDevice;
+        while (i1 < n && (i1 == i0 || bytes + lens[i1] <= pass_bytes) &&
                !(ic && ic->ix && i1 > i0 && blk(i1) != blk(i0))) {
             bytes += (lens[i1] + SDBFX_PAD - 1) / SDBFX_PAD * SDBFX_PAD;
             i1++;

@@ -1303,17 +1303,47 @@ __device__ __forceinline__ void stream_insert_body(
         wave_lds_sync();
     };
 
-    for (uint32_t t = sf.tile_begin; t < sf.tile_end; t++) {
-        if (tiles[t].first)
-            bigfi = 0;
-        const uint32_t C = tstate[t].count;
-        const uint32_t* cw = cand + (uint64_t)t * 5 * SDBFX_CAND_CAP;
-        for (uint32_t c0 = 0; c0 < C; c0 += 64) {
+    // The batches of 64 candidates in order: (tile t, first candidate c0).
+    // Each batch's SHA-1 words are loaded one batch ahead, so that their
+    // latency overlaps the batch before; a tile's restart flag (the first
+    // tile of a 32 MiB unit resets the de-duplication count) is applied
+    // when the first batch at or after it is taken.
+    auto load_batch = [&](uint32_t tt, uint32_t cc, uint32_t CC, uint32_t h[5]) {
+        const uint32_t* cw = cand + (uint64_t)tt * 5 * SDBFX_CAND_CAP;
+#pragma unroll
+        for (int j = 0; j < 5; j++)
+            h[j] = (cc + lane < CC) ? cw[j * SDBFX_CAND_CAP + cc + lane] : 0u;
+    };
+    // the first tile from tt on with candidates (sf.tile_end if none), and its count
+    auto next_tile = [&](uint32_t tt, uint32_t& CC) {
+        CC = 0;
+        while (tt < sf.tile_end && (CC = tstate[tt].count) == 0)
+            tt++;
+        return tt;
+    };
+    uint32_t C;
+    uint32_t t = next_tile(sf.tile_begin, C), c0 = 0, applied = sf.tile_begin;
+    uint32_t hnext[5] = {0, 0, 0, 0, 0};
+    if (t < sf.tile_end)
+        load_batch(t, 0, C, hnext);
+    while (t < sf.tile_end) {
+        for (; applied <= t; applied++)
+            if (tiles[applied].first)
+                bigfi = 0;
+        {
             const uint32_t cnt = min(64u, C - c0);
             uint32_t myh[5];
 #pragma unroll
             for (int j = 0; j < 5; j++)
-                myh[j] = (c0 + lane < C) ? cw[j * SDBFX_CAND_CAP + c0 + lane] : 0u;
+                myh[j] = hnext[j];
+            // the next batch, loaded now
+            uint32_t tn = t, c0n = c0 + 64, Cn = C;
+            if (c0n >= C) {
+                tn = next_tile(t + 1, Cn);
+                c0n = 0;
+            }
+            if (tn < sf.tile_end)
+                load_batch(tn, c0n, Cn, hnext);
             // positions and which of them repeat an earlier one of the lane
             uint32_t p[5], q[5], ip[5], pdup = 0, qdup = 0, idup = 0;
 #pragma unroll
@@ -1526,6 +1556,9 @@ __device__ __forceinline__ void stream_insert_body(
                     counted(k);
                 s = cut + 1;
             }
+            t = tn;
+            c0 = c0n;
+            C = Cn;
         }
     }
     flush(bfc - 1);
