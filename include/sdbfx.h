@@ -222,6 +222,29 @@ int sdbfx_compare_to_sampled(sdbfx_ctx *ctx, const sdbfx_set *q, const sdbfx_set
 #define SDBFX_CMP_MFMA 2
 int sdbfx_set_compare_method(sdbfx_ctx *ctx, int method);
 
+/*
+ * Autotuning of the comparison launches (no counterpart in the reference,
+ * whose comparison has a single CPU form).  With autotuning on and the
+ * method AUTO, a call whose shape class (log2 of its reference-filter
+ * instances, of its largest target digest's filter count and of its target
+ * digest count) is new runs every candidate configuration on the call
+ * itself, twice each -- the VALU kernel, and the MFMA kernel with 1024 to
+ * 16384 target filters per work item -- checks that their scores agree, returns them, and
+ * keeps the fastest for later calls of the class.  Without it (the default)
+ * the engine's cost model and chunk rule decide.
+ */
+int sdbfx_set_autotune(sdbfx_ctx *ctx, int on);
+typedef struct {
+    uint64_t instances;        /* reference-filter instances of the tuning call */
+    uint64_t max_target;       /* filters of its largest target digest */
+    uint32_t target_digests;
+    int32_t method;            /* SDBFX_CMP_VALU or SDBFX_CMP_MFMA */
+    uint32_t chunk;            /* target filters per work item */
+    float ms;                  /* the chosen configuration's time on that call */
+} sdbfx_tune_entry;
+/* The tuned classes so far: *n gets their number, entries[0 .. min(cap, n)) the entries. */
+int sdbfx_autotune_table(const sdbfx_ctx *ctx, sdbfx_tune_entry *entries, uint32_t cap, uint32_t *n);
+
 /* ---- serving -------------------------------------------------------- */
 
 /* ---- indexes (sdhash --index / --index-search) ------------------------- */

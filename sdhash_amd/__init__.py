@@ -28,7 +28,7 @@ SYMBOLS = (
     "sdbfx_serve_create", "sdbfx_serve_destroy", "sdbfx_serve_error", "sdbfx_serve_ref_size",
     "sdbfx_serve_submit", "sdbfx_serve_poll", "sdbfx_serve_wait", "sdbfx_serve_stats", "sdbfx_serve_admission",
     "sdbfx_index_create", "sdbfx_index_destroy", "sdbfx_index_size", "sdbfx_index_read", "sdbfx_index_write",
-    "sdbfx_set_digest_indexed", "sdbfx_set_digest_device_indexed",
+    "sdbfx_set_digest_indexed", "sdbfx_set_digest_device_indexed", "sdbfx_set_autotune", "sdbfx_autotune_table",
 )
 
 # comparison kernels (sdbfx_set_compare_method)
@@ -44,6 +44,11 @@ class SdbfxError(RuntimeError):
 
 
 EBUDGET = -4   # a call's scratch would exceed the context's memory budget
+
+
+class _TuneEntry(C.Structure):
+    _fields_ = [("instances", C.c_uint64), ("max_target", C.c_uint64), ("target_digests", C.c_uint32),
+                ("method", C.c_int32), ("chunk", C.c_uint32), ("ms", C.c_float)]
 
 
 class _Info(C.Structure):
@@ -111,6 +116,8 @@ def load():
     L.sdbfx_serve_wait.argtypes = [P, U64, P, C.POINTER(I32), C.POINTER(U64)]
     L.sdbfx_serve_stats.argtypes = [P, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64)]
     L.sdbfx_serve_admission.argtypes = [P, C.POINTER(U64), C.POINTER(U64)]
+    L.sdbfx_set_autotune.argtypes = [P, C.c_int]
+    L.sdbfx_autotune_table.argtypes = [P, P, U32, C.POINTER(U32)]
     L.sdbfx_index_create.argtypes = [P, U64, C.POINTER(P)]
     L.sdbfx_index_destroy.argtypes = [P]
     L.sdbfx_index_destroy.restype = None
@@ -130,7 +137,8 @@ def load():
                  "sdbfx_reset_kernel_times", "sdbfx_set_compare_method", "sdbfx_digest_blocks_noalloc", "sdbfx_serve_create",
                  "sdbfx_serve_submit", "sdbfx_serve_poll", "sdbfx_serve_wait", "sdbfx_serve_stats",
                  "sdbfx_serve_admission", "sdbfx_index_create", "sdbfx_index_read", "sdbfx_index_write",
-                 "sdbfx_set_digest_indexed", "sdbfx_set_digest_device_indexed"):
+                 "sdbfx_set_digest_indexed", "sdbfx_set_digest_device_indexed", "sdbfx_set_autotune",
+                 "sdbfx_autotune_table"):
         getattr(L, name).restype = I32
     _lib = L
     return L
@@ -424,6 +432,22 @@ class Engine:
         """CMP_AUTO, CMP_VALU (popcount on the vector ALUs) or CMP_MFMA (int8
         matrix cores on {0,1} bit-planes); the scores are the same."""
         self._check(self.lib.sdbfx_set_compare_method(self.ctx, method))
+
+    def autotune(self, on: bool) -> None:
+        """Times every comparison configuration on the first call of each
+        shape class and keeps the fastest (scores unchanged)."""
+        self._check(self.lib.sdbfx_set_autotune(self.ctx, 1 if on else 0))
+
+    def tuned(self) -> list[dict]:
+        """The autotuned classes: instances, largest target, target digests,
+        method (CMP_VALU / CMP_MFMA), chunk, ms."""
+        n = C.c_uint32()
+        self._check(self.lib.sdbfx_autotune_table(self.ctx, None, 0, C.byref(n)))
+        arr = (_TuneEntry * max(n.value, 1))()
+        self._check(self.lib.sdbfx_autotune_table(self.ctx, arr, n.value, C.byref(n)))
+        return [{"instances": e.instances, "max_target": e.max_target, "target_digests": e.target_digests,
+                 "method": "mfma" if e.method == CMP_MFMA else "valu", "chunk": e.chunk, "ms": e.ms}
+                for e in arr[:n.value]]
 
     def profiling(self, on: bool) -> None:
         self._check(self.lib.sdbfx_set_profiling(self.ctx, 1 if on else 0))

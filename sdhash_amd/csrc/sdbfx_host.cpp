@@ -48,6 +48,11 @@ struct sdbfx_ctx {
     std::string err;
     bool prof = false;
     int cmp_method = SDBFX_CMP_AUTO;
+    // compare autotuning (sdbfx_set_autotune): per shape class, the launch
+    // configuration found fastest by timing every candidate on the class's
+    // first call
+    bool autotune = false;
+    std::map<uint64_t, sdbfx_tune_entry> tuned;
     struct Ev {
         const char* name;
         hipEvent_t a, b;
@@ -660,6 +665,20 @@ bool use_mfma(const sdbfx_ctx* ctx, const std::vector<uint64_t>& inst, const std
     return mfma < valu;
 }
 
+// whether the MFMA kernel can take these target digests: it lists a
+// candidate's target filter (its index in the digest) in 25 bits
+bool use_mfma_possible(const std::vector<uint64_t>& tgt)
+{
+    for (uint64_t t : tgt)
+        if (t >= (1ull << 25))
+            return false;
+    return true;
+}
+
+int launch_pairs(sdbfx_ctx* ctx, const sdbfx_set* const S[2], bool all, uint32_t na, uint64_t npairs,
+                 const std::vector<PairRef>& ref, const std::vector<uint32_t>& order, const std::vector<uint32_t>& kc,
+                 uint32_t nkeys, int mode, bool mfma, uint32_t chunk, int32_t* sorted);
+
 // Scoring mode of one oriented pair.  sdbf_max_score takes the cutoff table
 // and bit count from the reference digest's fastmode flag only
 // (sdbf/sdbf_core.cc:422-437), so a pair whose reference digest is folded
@@ -704,24 +723,98 @@ int compare_pairs(sdbfx_ctx* ctx, const sdbfx_set* const S[2], bool all, uint32_
         ginst.push_back(ic);
         gtgt.push_back(T->bfc[q >= na ? q - na : q]);
     }
-    const bool mfma = mode == CMP_MODE_PLAIN && use_mfma(ctx, ginst, gtgt);
-    const uint32_t rows = mfma ? SDBFX_MROWS : SDBFX_CMP_ROWS;
+    const bool mfma_ok = mode == CMP_MODE_PLAIN && use_mfma_possible(gtgt);
+    bool mfma = mode == CMP_MODE_PLAIN && use_mfma(ctx, ginst, gtgt);
     // target filters per work item.  The MFMA kernel's fixed costs (row
     // staging, publishing maxima) are per work item and its pruning works
-    // across the tiles of one, so it takes the largest chunk that still
-    // leaves a work item for every one of its 1024 wave slots.
+    // across the tiles of one, so by default it takes the largest chunk that
+    // still leaves a work item for every one of its 1024 wave slots.
     uint32_t chunk = SDBFX_CMP_CHUNK;
     if (mfma) {
         for (uint32_t ch = 16384; ch > SDBFX_CMP_CHUNK; ch >>= 1) {
             uint64_t n = 0;
             for (size_t g = 0; g < ginst.size(); g++)
-                n += (ginst[g] + rows - 1) / rows * ((gtgt[g] + ch - 1) / ch);
+                n += (ginst[g] + SDBFX_MROWS - 1) / SDBFX_MROWS * ((gtgt[g] + ch - 1) / ch);
             if (n >= 1024) {
                 chunk = ch;
                 break;
             }
         }
     }
+    // Autotuning: for a shape class not seen before, time the VALU kernel
+    // and the MFMA kernel at every chunk size on this call, twice each (all give
+    // the same scores, which is checked), and keep the fastest for the class.
+    if (ctx->autotune && ctx->cmp_method == SDBFX_CMP_AUTO && mode == CMP_MODE_PLAIN && mfma_ok) {
+        uint64_t itot = 0, tmax = 0;
+        for (size_t g = 0; g < ginst.size(); g++) {
+            itot += ginst[g];
+            tmax = std::max<uint64_t>(tmax, gtgt[g]);
+        }
+        auto lg = [](uint64_t v) { return (uint64_t)(v ? 64 - __builtin_clzll(v) : 0); };
+        const uint64_t key = lg(itot) << 16 | lg(tmax) << 8 | lg(ginst.size());
+        auto it = ctx->tuned.find(key);
+        if (it == ctx->tuned.end()) {
+            const uint32_t cands[6] = {0, 1024, 2048, 4096, 8192, 16384};   // 0: the VALU kernel
+            sdbfx_tune_entry best{};
+            best.ms = 1e30;
+            std::vector<int32_t> keep(npairs), tmp(npairs);
+            for (int k = 0; k < 6; k++) {
+                // the better of two runs of each configuration
+                float ms = 1e30f;
+                for (int rep = 0; rep < 2; rep++) {
+                    hipEvent_t e0 = get_event(ctx), e1 = get_event(ctx);
+                    if (!e0 || !e1)
+                        return fail(ctx, "hipEventCreate failed");
+                    HIPC(ctx, hipEventRecord(e0, ctx->stream));
+                    const int rc = launch_pairs(ctx, S, all, na, npairs, ref, order, kc, nkeys, mode, cands[k] != 0,
+                                                cands[k] ? cands[k] : SDBFX_CMP_CHUNK, tmp.data());
+                    if (rc != 0)
+                        return rc;
+                    HIPC(ctx, hipEventRecord(e1, ctx->stream));
+                    HIPC(ctx, hipEventSynchronize(e1));
+                    float m = 0.f;
+                    HIPC(ctx, hipEventElapsedTime(&m, e0, e1));
+                    ctx->evpool.push_back(e0);
+                    ctx->evpool.push_back(e1);
+                    ms = std::min(ms, m);
+                }
+                if (k == 0)
+                    keep = tmp;
+                else if (tmp != keep)
+                    return fail(ctx, "autotuning: the comparison configurations disagree");
+                if (ms < best.ms) {
+                    best.ms = ms;
+                    best.method = cands[k] ? SDBFX_CMP_MFMA : SDBFX_CMP_VALU;
+                    best.chunk = cands[k] ? cands[k] : SDBFX_CMP_CHUNK;
+                }
+            }
+            best.instances = itot;
+            best.max_target = tmax;
+            best.target_digests = (uint32_t)ginst.size();
+            ctx->tuned[key] = best;
+            for (uint64_t q = 0; q < npairs; q++)
+                out[order[q]] = keep[q];
+            return 0;
+        }
+        mfma = it->second.method == SDBFX_CMP_MFMA;
+        chunk = it->second.chunk;
+    }
+    std::vector<int32_t> sorted(npairs);
+    if (int rc = launch_pairs(ctx, S, all, na, npairs, ref, order, kc, nkeys, mode, mfma, chunk, sorted.data()))
+        return rc;
+    for (uint64_t q = 0; q < npairs; q++)
+        out[order[q]] = sorted[q];
+    return 0;
+}
+
+// One comparison launch sequence with a given kernel and chunk size over the
+// pairs grouped by target digest (order / kc from compare_pairs); the scores
+// in grouped order go to sorted.
+int launch_pairs(sdbfx_ctx* ctx, const sdbfx_set* const S[2], bool all, uint32_t na, uint64_t npairs,
+                 const std::vector<PairRef>& ref, const std::vector<uint32_t>& order, const std::vector<uint32_t>& kc,
+                 uint32_t nkeys, int mode, bool mfma, uint32_t chunk, int32_t* sorted)
+{
+    const uint32_t rows = mfma ? SDBFX_MROWS : SDBFX_CMP_ROWS;
     std::vector<CmpPair> pairs(npairs);
     std::vector<CmpGroup> groups;
     uint64_t inst = 0, wi = 0, partial_n = 0;
@@ -872,16 +965,13 @@ int compare_pairs(sdbfx_ctx* ctx, const sdbfx_set* const S[2], bool all, uint32_
         LAUNCH(ctx, pair_sum_kernel, (uint32_t)((npairs + 255) / 256), 256, (const CmpPair*)ctx->pairs.p,
                (uint32_t)npairs, va.ecf, vb.ecf, (const uint32_t*)ctx->imax.p, (int32_t*)ctx->scores.p);
     }
-    std::vector<int32_t> sorted(npairs);
-    HIPC(ctx, hipMemcpyAsync(sorted.data(), ctx->scores.p, npairs * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(ctx, hipMemcpyAsync(sorted, ctx->scores.p, npairs * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPC(ctx, hipStreamSynchronize(ctx->stream));
-    for (uint64_t s = 0; s < npairs; s++) {
+    for (uint64_t s = 0; s < npairs; s++)
         if (sorted[s] == SDBFX_SCORE_NONFINITE)
             return fail(ctx, "a pair's score is not finite (a plain reference filter against a folded target "
                              "filter whose Hamming weight equals the cutoff): the reference aborts on it "
                              "(boost::math::round of infinity)");
-        out[order[s]] = sorted[s];
-    }
     return 0;
 }
 
@@ -1664,6 +1754,28 @@ int sdbfx_set_compare_method(sdbfx_ctx* ctx, int method)
     if (method != SDBFX_CMP_AUTO && method != SDBFX_CMP_VALU && method != SDBFX_CMP_MFMA)
         return fail(ctx, "unknown compare method");
     ctx->cmp_method = method;
+    return 0;
+}
+
+int sdbfx_set_autotune(sdbfx_ctx* ctx, int on)
+{
+    if (!ctx)
+        return -1;
+    ctx->autotune = on != 0;
+    return 0;
+}
+
+int sdbfx_autotune_table(const sdbfx_ctx* ctx, sdbfx_tune_entry* entries, uint32_t cap, uint32_t* n)
+{
+    if (!ctx || !n)
+        return -1;
+    uint32_t k = 0;
+    for (const auto& kv : ctx->tuned) {
+        if (entries && k < cap)
+            entries[k] = kv.second;
+        k++;
+    }
+    *n = k;
     return 0;
 }
 

@@ -317,3 +317,34 @@ def test_compare_hand_derived_rules(cmp_engine):
     for k, (x, y, want) in enumerate(cases):
         assert oracle.score(x, y) == want, k
         assert cmp_engine.compare_to(upload_oracle(cmp_engine, [x]), upload_oracle(cmp_engine, [y]))[0, 0] == want, k
+
+
+def test_compare_autotuned(engine):
+    # autotuning times the VALU kernel and the MFMA kernel at every chunk size
+    # on the first call of a shape class (their scores must agree), returns
+    # the scores, and reuses the fastest configuration afterwards
+    import sdhash_amd
+    img = synth.disk_image(35, 6 * MIB)
+    frags = synth.fragments(36, img, 700)
+    names = [b"img.%04dM" % k for k in range(6)]
+    tgt = [oracle.digest_block(img[k * MIB:(k + 1) * MIB], 16384, n) for k, n in enumerate(names)]
+    q = [oracle.digest_stream(f, b"frag%04d" % i) for i, f in enumerate(frags)]
+    want = oracle.compare_to(q, tgt, threads=oracle.threads_available())
+    gq = engine.new_set()
+    assert np.all(gq.digest(frags, [d.name for d in q]) == 0)
+    gt = engine.new_set()
+    assert np.all(gt.digest([img[k * MIB:(k + 1) * MIB] for k in range(6)], names, dd_block=16384) == 0)
+    engine.compare_method(sdhash_amd.CMP_AUTO)
+    engine.autotune(True)
+    try:
+        first = engine.compare_to(gq, gt)
+        table = engine.tuned()
+        again = engine.compare_to(gq, gt)
+        assert engine.tuned() == table
+    finally:
+        engine.autotune(False)
+    assert np.array_equal(first, want) and np.array_equal(again, want)
+    assert len(table) == 1 and table[0]["method"] in ("valu", "mfma") and table[0]["chunk"] in (1024, 2048, 4096,
+                                                                                                   8192, 16384)
+    # every query digest is the reference side against each of the 6 targets
+    assert table[0]["instances"] == 6 * sum(d.bf_count for d in q) and table[0]["target_digests"] == 6
