@@ -352,6 +352,8 @@ int sdbfx_upload(sdbfx_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 
 /* ---- no device allocation ------------------------------------------- */
 
+typedef struct sdbfx_graph sdbfx_graph;
+
 /*
  * Block-mode digest of one input already in device memory, into device
  * memory the caller provides, with no device allocation: no hipMalloc or
@@ -372,6 +374,23 @@ int sdbfx_upload(sdbfx_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 int sdbfx_digest_blocks_noalloc(sdbfx_ctx *ctx, const void *d_input, uint64_t len, uint32_t dd_block,
                                 uint8_t *d_filters, uint16_t *d_elem_counts, uint16_t *d_hamming,
                                 uint32_t *nfilters);
+
+/*
+ * The launches of sdbfx_digest_blocks_noalloc for one (d_input, len,
+ * dd_block, outputs), captured once into a HIP graph on the context's stream
+ * (no counterpart in the reference).  sdbfx_graph_launch replays it on the
+ * context's stream: a loop that digests successive contents of one device
+ * buffer (refilled by the caller in stream order between replays) then costs
+ * one graph launch per call and no host work.  The graph holds no memset and
+ * allocates nothing; it shares the no-allocation entry point's scratch array
+ * (replays that overlap another no-allocation call on the GPU race with it)
+ * and its limits.  *nfilters gets the filter count of every replay.
+ */
+int sdbfx_graph_blocks_create(sdbfx_ctx *ctx, const void *d_input, uint64_t len, uint32_t dd_block,
+                              uint8_t *d_filters, uint16_t *d_elem_counts, uint16_t *d_hamming, uint32_t *nfilters,
+                              sdbfx_graph **out);
+int sdbfx_graph_launch(sdbfx_ctx *ctx, sdbfx_graph *g);
+void sdbfx_graph_destroy(sdbfx_graph *g);
 
 /* ---- kernel timing -------------------------------------------------- */
 

@@ -29,6 +29,7 @@ SYMBOLS = (
     "sdbfx_serve_submit", "sdbfx_serve_poll", "sdbfx_serve_wait", "sdbfx_serve_stats", "sdbfx_serve_admission",
     "sdbfx_index_create", "sdbfx_index_destroy", "sdbfx_index_size", "sdbfx_index_read", "sdbfx_index_write",
     "sdbfx_set_digest_indexed", "sdbfx_set_digest_device_indexed", "sdbfx_set_autotune", "sdbfx_autotune_table",
+    "sdbfx_graph_blocks_create", "sdbfx_graph_launch", "sdbfx_graph_destroy",
 )
 
 # comparison kernels (sdbfx_set_compare_method)
@@ -116,6 +117,10 @@ def load():
     L.sdbfx_serve_wait.argtypes = [P, U64, P, C.POINTER(I32), C.POINTER(U64)]
     L.sdbfx_serve_stats.argtypes = [P, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64)]
     L.sdbfx_serve_admission.argtypes = [P, C.POINTER(U64), C.POINTER(U64)]
+    L.sdbfx_graph_blocks_create.argtypes = [P, P, U64, U32, P, P, P, C.POINTER(U32), C.POINTER(P)]
+    L.sdbfx_graph_launch.argtypes = [P, P]
+    L.sdbfx_graph_destroy.argtypes = [P]
+    L.sdbfx_graph_destroy.restype = None
     L.sdbfx_set_autotune.argtypes = [P, C.c_int]
     L.sdbfx_autotune_table.argtypes = [P, P, U32, C.POINTER(U32)]
     L.sdbfx_index_create.argtypes = [P, U64, C.POINTER(P)]
@@ -138,7 +143,7 @@ def load():
                  "sdbfx_serve_submit", "sdbfx_serve_poll", "sdbfx_serve_wait", "sdbfx_serve_stats",
                  "sdbfx_serve_admission", "sdbfx_index_create", "sdbfx_index_read", "sdbfx_index_write",
                  "sdbfx_set_digest_indexed", "sdbfx_set_digest_device_indexed", "sdbfx_set_autotune",
-                 "sdbfx_autotune_table"):
+                 "sdbfx_autotune_table", "sdbfx_graph_blocks_create", "sdbfx_graph_launch"):
         getattr(L, name).restype = I32
     _lib = L
     return L

@@ -1801,29 +1801,40 @@ int sdbfx_kernel_time(sdbfx_ctx* ctx, const char* name, double* ms, uint64_t* la
     return 0;
 }
 
-int sdbfx_digest_blocks_noalloc(sdbfx_ctx* ctx, const void* d_input, uint64_t len, uint32_t dd_block,
-                                uint8_t* d_filters, uint16_t* d_elem_counts, uint16_t* d_hamming, uint32_t* nfilters)
+}  // extern "C"
+
+namespace {
+
+// The checks of sdbfx_digest_blocks_noalloc; its dd-block count in *nblk.
+int noalloc_check(sdbfx_ctx* ctx, const void* d_input, uint64_t len, uint32_t dd_block, const void* d_filters,
+                  const void* d_elem_counts, const void* d_hamming, uint64_t* nblk)
 {
-    if (!ctx || !d_input || !d_filters || !d_elem_counts || !d_hamming || !nfilters)
+    if (!ctx || !d_input || !d_filters || !d_elem_counts || !d_hamming)
         return fail(ctx, "null argument");
     if ((uintptr_t)d_input % SDBFX_PAD)
         return fail(ctx, "the input must be 256-byte aligned");
     if (dd_block < SDBFX_MIN_FILE || dd_block > SDBFX_TILE || dd_block % 16)
         return fail(ctx, "dd block size must be a multiple of 16 in [512, 16384]");
-    *nfilters = 0;
     if (len < SDBFX_MIN_FILE) {
         ctx->err = "input too small";
         return -3;
     }
-    const uint64_t nblk = block_count(len, dd_block);
-    if (nblk > SDBFX_NOALLOC_BLOCKS)
-        return fail(ctx, "input of " + std::to_string(nblk) + " dd-blocks: the no-allocation digest takes at most " +
+    *nblk = block_count(len, dd_block);
+    if (*nblk > SDBFX_NOALLOC_BLOCKS)
+        return fail(ctx, "input of " + std::to_string(*nblk) + " dd-blocks: the no-allocation digest takes at most " +
                              std::to_string(SDBFX_NOALLOC_BLOCKS));
-    void* sp = scratch_ptr(ctx->device);
-    if (!sp)
+    if (!scratch_ptr(ctx->device))
         return fail(ctx, "no device address for the scratch array");
-    HIPC(ctx, hipSetDevice(ctx->device));
-    BlockDesc* bd = (BlockDesc*)sp;
+    return 0;
+}
+
+// The launches of sdbfx_digest_blocks_noalloc: the work list into the fixed
+// scratch array, then one workgroup per dd-block.  Nothing else is queued,
+// so the sequence can be captured into a graph.
+int noalloc_launch(sdbfx_ctx* ctx, const void* d_input, uint64_t len, uint32_t dd_block, uint64_t nblk,
+                   uint8_t* d_filters, uint16_t* d_elem_counts, uint16_t* d_hamming)
+{
+    BlockDesc* bd = (BlockDesc*)scratch_ptr(ctx->device);
     hipLaunchKernelGGL(block_desc_kernel, dim3((uint32_t)((nblk + 255) / 256)), dim3(256), 0, ctx->stream, bd,
                        len, dd_block, (uint32_t)nblk);
     HIPC(ctx, hipGetLastError());
@@ -1831,6 +1842,81 @@ int sdbfx_digest_blocks_noalloc(sdbfx_ctx* ctx, const void* d_input, uint64_t le
                        (const uint8_t*)d_input, (const BlockDesc*)bd, d_filters, d_elem_counts, d_hamming,
                        (uint32_t*)nullptr);
     HIPC(ctx, hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+struct sdbfx_graph {
+    int device = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int sdbfx_graph_blocks_create(sdbfx_ctx* ctx, const void* d_input, uint64_t len, uint32_t dd_block,
+                              uint8_t* d_filters, uint16_t* d_elem_counts, uint16_t* d_hamming, uint32_t* nfilters,
+                              sdbfx_graph** out)
+{
+    if (!out || !nfilters)
+        return fail(ctx, "null argument");
+    *out = nullptr;
+    *nfilters = 0;
+    uint64_t nblk = 0;
+    if (int rc = noalloc_check(ctx, d_input, len, dd_block, d_filters, d_elem_counts, d_hamming, &nblk))
+        return rc;
+    HIPC(ctx, hipSetDevice(ctx->device));
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    sdbfx_graph* g = new sdbfx_graph;
+    g->device = ctx->device;
+    if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+        delete g;
+        return fail(ctx, "hipStreamBeginCapture failed");
+    }
+    const int rc = noalloc_launch(ctx, d_input, len, dd_block, nblk, d_filters, d_elem_counts, d_hamming);
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g->graph);
+    if (rc != 0 || e != hipSuccess || hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess) {
+        sdbfx_graph_destroy(g);
+        return rc != 0 ? rc : fail(ctx, "graph capture failed");
+    }
+    *nfilters = (uint32_t)nblk;
+    *out = g;
+    return 0;
+}
+
+int sdbfx_graph_launch(sdbfx_ctx* ctx, sdbfx_graph* g)
+{
+    if (!ctx || !g || g->device != ctx->device)
+        return fail(ctx, "null graph or graph of another device");
+    HIPC(ctx, hipGraphLaunch(g->exec, ctx->stream));
+    return 0;
+}
+
+void sdbfx_graph_destroy(sdbfx_graph* g)
+{
+    if (!g)
+        return;
+    (void)hipSetDevice(g->device);
+    if (g->exec)
+        (void)hipGraphExecDestroy(g->exec);
+    if (g->graph)
+        (void)hipGraphDestroy(g->graph);
+    delete g;
+}
+
+int sdbfx_digest_blocks_noalloc(sdbfx_ctx* ctx, const void* d_input, uint64_t len, uint32_t dd_block,
+                                uint8_t* d_filters, uint16_t* d_elem_counts, uint16_t* d_hamming, uint32_t* nfilters)
+{
+    if (!nfilters)
+        return fail(ctx, "null argument");
+    *nfilters = 0;
+    uint64_t nblk = 0;
+    if (int rc = noalloc_check(ctx, d_input, len, dd_block, d_filters, d_elem_counts, d_hamming, &nblk))
+        return rc;
+    HIPC(ctx, hipSetDevice(ctx->device));
+    if (int rc = noalloc_launch(ctx, d_input, len, dd_block, nblk, d_filters, d_elem_counts, d_hamming))
+        return rc;
     *nfilters = (uint32_t)nblk;
     return 0;
 }
