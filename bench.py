@@ -33,10 +33,12 @@ FRAG = 4096
 HBM_PEAK_GBPS = 8000.0
 # MI355X: 256 CUs x 4 SIMDs at 2.4 GHz.  VALU: a wave64 instruction per 4
 # cycles per SIMD, 2 instructions (AND, popcount-accumulate) per 32-bit word,
-# 64 words per filter pair -> 128 lane-instructions per pair.  MFMA: 5.0e15
-# dense int8 ops/s = 2.5e15 MACs/s, 2048 MACs per pair.
+# 64 words per filter pair -> 128 lane-instructions per pair.  MFMA: 10.1e15
+# dense FP4 ops/s = 5.05e15 MACs/s, 2048 MACs per pair (one per bit pair);
+# the int8 form: 5.0e15 dense int8 ops/s.
 VALU_PEAK_PAIRS = 256 * 4 * 2.4e9 * 16 / 128
-MFMA_PEAK_PAIRS = 5.0e15 / 2 / 2048
+MFMA_PEAK_PAIRS = 10.1e15 / 2 / 2048
+MFMA_I8_PEAK_PAIRS = 5.0e15 / 2 / 2048
 
 
 def log(msg: str) -> None:
@@ -419,6 +421,7 @@ def allpairs(eng, args) -> dict:
            "image_bytes": img_blocks * 16384, "segment_bytes": int(sizes.sum())}
     ref = None
     for label, method, kname in (("mfma", sdhash_amd.CMP_MFMA, "cmp_mfma_kernel"),
+                                 ("mfma_i8", sdhash_amd.CMP_MFMA_I8, "cmp_mfma_i8_kernel"),
                                  ("valu", sdhash_amd.CMP_VALU, "cmp_kernel")):
         eng.compare_method(method)
         eng.sync()
@@ -796,14 +799,15 @@ def main() -> int:
     cmp_ms = sum(k.get(n, (0.0, 0))[0] for n in ("cmp_kernel", "col_order_kernel", "cmp_prep_kernel",
                                                  "cmp_mfma_kernel", "chunk_max_kernel", "pair_sum_kernel")) / args.steps
 
-    # the two comparison kernels on the same sets: same scores, rate of each
+    # the comparison kernels on the same sets: same scores, rate of each
     # against the peak pair rate of its unit (2048 AND-popcount bit pairs per
     # filter pair: VALU 2 instructions per 32-bit word per lane; MFMA 2048
-    # int8 MACs)
+    # FP4 MACs, or 2048 int8 MACs in the byte-plane form)
     forms = {}
     for label, method, kname, peak in (
             ("valu", sdhash_amd.CMP_VALU, "cmp_kernel", VALU_PEAK_PAIRS),
-            ("mfma", sdhash_amd.CMP_MFMA, "cmp_mfma_kernel", MFMA_PEAK_PAIRS)):
+            ("mfma", sdhash_amd.CMP_MFMA, "cmp_mfma_kernel", MFMA_PEAK_PAIRS),
+            ("mfma_i8", sdhash_amd.CMP_MFMA_I8, "cmp_mfma_i8_kernel", MFMA_I8_PEAK_PAIRS)):
         eng.compare_method(method)
         got = eng.compare_to(qset, tset)
         if not np.array_equal(got, scores):
@@ -865,10 +869,11 @@ def main() -> int:
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic("block_digest_kernel"),
                      "kernel": "block_digest_kernel",
                      "algorithmic_bytes_per_launch": IMAGE * args.steps // max(blk_n, 1)},
-        # the kernel behind `value`: 2048 int8 MACs (4096 ops) per filter pair
-        "roofline_compare": {"bound": "mfma", "kernel": "cmp_mfma_kernel", "unit": "TOPS",
-                             "achieved": mfma_rate * 4096 / 1e12, "peak": 5000.0,
-                             "frac": mfma_rate * 4096 / 1e12 / 5000.0,
+        # the kernel behind `value`: 2048 FP4 MACs (4096 ops) per filter pair
+        "roofline_compare": {"bound": "mfma", "kernel": "cmp_mfma_kernel", "unit": "TFLOP/s",
+                             "dtype": "fp4",
+                             "achieved": mfma_rate * 4096 / 1e12, "peak": 10100.0,
+                             "frac": mfma_rate * 4096 / 1e12 / 10100.0,
                              "traffic": measured_traffic("cmp_mfma_kernel"),
                              "algorithmic_ops_per_launch": pairs * 4096,
                              "launch_ms": pairs / mfma_rate * 1e3},

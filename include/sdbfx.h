@@ -211,15 +211,19 @@ int sdbfx_compare_to_sampled(sdbfx_ctx *ctx, const sdbfx_set *q, const sdbfx_set
  * context.  Both give identical scores:
  *   SDBFX_CMP_VALU  AND + popcount on the vector ALUs, one reference filter
  *                   per lane;
- *   SDBFX_CMP_MFMA  exact int8 matrix-core products of {0,1} bit-planes,
- *                   128 x 128 filter pairs per wave;
- *   SDBFX_CMP_AUTO  (default) picks one from the shape of the call.
+ *   SDBFX_CMP_MFMA  exact matrix-core products of the filter bits as FP4
+ *                   (E2M1) elements 0 / 0.5 with block scale 2, f32
+ *                   accumulation, 128 x 128 filter pairs per wave;
+ *   SDBFX_CMP_MFMA_I8  the same tiling with int8 {0,1} byte planes and
+ *                   int32 accumulation (half the FP4 form's rate);
+ *   SDBFX_CMP_AUTO  (default) picks VALU or MFMA from the shape of the call.
  * No counterpart in the reference, whose bf_bitcount_cut_256
  * (sdbf/bf_utils.cc:115-206) is a single CPU loop.
  */
 #define SDBFX_CMP_AUTO 0
 #define SDBFX_CMP_VALU 1
 #define SDBFX_CMP_MFMA 2
+#define SDBFX_CMP_MFMA_I8 3
 int sdbfx_set_compare_method(sdbfx_ctx *ctx, int method);
 
 /*

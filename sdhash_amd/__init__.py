@@ -33,7 +33,7 @@ SYMBOLS = (
 )
 
 # comparison kernels (sdbfx_set_compare_method)
-CMP_AUTO, CMP_VALU, CMP_MFMA = 0, 1, 2
+CMP_AUTO, CMP_VALU, CMP_MFMA, CMP_MFMA_I8 = 0, 1, 2, 3
 
 _lib = None
 
@@ -434,8 +434,9 @@ class Engine:
         self._check(self.lib.sdbfx_sync(self.ctx))
 
     def compare_method(self, method: int) -> None:
-        """CMP_AUTO, CMP_VALU (popcount on the vector ALUs) or CMP_MFMA (int8
-        matrix cores on {0,1} bit-planes); the scores are the same."""
+        """CMP_AUTO, CMP_VALU (popcount on the vector ALUs), CMP_MFMA (matrix
+        cores, filter bits as FP4 elements) or CMP_MFMA_I8 (matrix cores,
+        int8 {0,1} byte planes); the scores are the same."""
         self._check(self.lib.sdbfx_set_compare_method(self.ctx, method))
 
     def autotune(self, on: bool) -> None:

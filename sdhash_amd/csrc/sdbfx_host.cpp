@@ -645,9 +645,9 @@ struct PairRef {
 // Kernel choice for SDBFX_CMP_AUTO from the padded work of each tiling, per
 // target digest with I reference-filter instances and T target filters:
 // VALU ceil(I/64)*64 * T pair slots at about 8 SIMD cycles each (128
-// instructions per lane-pair), MFMA ceil(I/128)*128 * ceil(T/128)*128 slots
-// at about 2.75 cycles each (measured on MI355X, candidate re-checks
-// included).
+// instructions per lane-pair), MFMA (FP4 form) ceil(I/128)*128 *
+// ceil(T/128)*128 slots at about 2 cycles each (measured on MI355X,
+// candidate re-checks included).
 bool use_mfma(const sdbfx_ctx* ctx, const std::vector<uint64_t>& inst, const std::vector<uint64_t>& tgt)
 {
     // the MFMA kernel lists a candidate's target filter (its index in the
@@ -656,11 +656,11 @@ bool use_mfma(const sdbfx_ctx* ctx, const std::vector<uint64_t>& inst, const std
         if (t >= (1ull << 25))
             return false;
     if (ctx->cmp_method != SDBFX_CMP_AUTO)
-        return ctx->cmp_method == SDBFX_CMP_MFMA;
+        return ctx->cmp_method == SDBFX_CMP_MFMA || ctx->cmp_method == SDBFX_CMP_MFMA_I8;
     double valu = 0, mfma = 0;
     for (size_t g = 0; g < inst.size(); g++) {
         valu += 8.0 * (double)((inst[g] + 63) / 64 * 64) * (double)tgt[g];
-        mfma += 2.75 * (double)((inst[g] + 127) / 128 * 128) * (double)((tgt[g] + 127) / 128 * 128);
+        mfma += 2.0 * (double)((inst[g] + 127) / 128 * 128) * (double)((tgt[g] + 127) / 128 * 128);
     }
     return mfma < valu;
 }
@@ -937,9 +937,14 @@ int launch_pairs(sdbfx_ctx* ctx, const sdbfx_set* const S[2], bool all, uint32_t
         // one wave per SIMD (the 128 x 128 tile holds 256 accumulators per
         // lane): a 4-wave workgroup per CU, a multiple of the 8 XCDs
         const uint32_t grid = (uint32_t)std::min<uint64_t>(((wi + 3) / 4 + 7) & ~7ull, 256);
-        LAUNCH(ctx, cmp_mfma_kernel, grid, 256, va, vb, (const CmpGroup*)ctx->groups.p, (uint32_t)groups.size(),
-               (const uint64_t*)ctx->rmap.p, (const uint32_t*)ctx->rcode.p, wi, (uint32_t*)ctx->partial.p,
-               (unsigned long long*)ctx->gbest.p);
+        if (ctx->cmp_method == SDBFX_CMP_MFMA_I8)
+            LAUNCH(ctx, cmp_mfma_i8_kernel, grid, 256, va, vb, (const CmpGroup*)ctx->groups.p,
+                   (uint32_t)groups.size(), (const uint64_t*)ctx->rmap.p, (const uint32_t*)ctx->rcode.p, wi,
+                   (uint32_t*)ctx->partial.p, (unsigned long long*)ctx->gbest.p);
+        else
+            LAUNCH(ctx, cmp_mfma_kernel, grid, 256, va, vb, (const CmpGroup*)ctx->groups.p,
+                   (uint32_t)groups.size(), (const uint64_t*)ctx->rmap.p, (const uint32_t*)ctx->rcode.p, wi,
+                   (uint32_t*)ctx->partial.p, (unsigned long long*)ctx->gbest.p);
     } else if (wi) {
         const uint32_t grid = (uint32_t)std::min<uint64_t>(wi, 16384);
         if (mode == CMP_MODE_FAST)
@@ -1751,7 +1756,8 @@ int sdbfx_set_compare_method(sdbfx_ctx* ctx, int method)
 {
     if (!ctx)
         return -1;
-    if (method != SDBFX_CMP_AUTO && method != SDBFX_CMP_VALU && method != SDBFX_CMP_MFMA)
+    if (method != SDBFX_CMP_AUTO && method != SDBFX_CMP_VALU && method != SDBFX_CMP_MFMA &&
+        method != SDBFX_CMP_MFMA_I8)
         return fail(ctx, "unknown compare method");
     ctx->cmp_method = method;
     return 0;

@@ -1978,33 +1978,37 @@ __global__ void fold_kernel(uint32_t* __restrict__ filt, uint16_t* __restrict__ 
 //
 // The same per-(instance, chunk) maxima as cmp_kernel, with the AND-popcount of
 // 128 x 128 filter pairs per wave computed on the matrix cores: every filter
-// is expanded into {0,1} bytes ("bit-planes") and multiplied as int8 with
-// int32 accumulation (v_mfma_i32_32x32x32_i8, a 4 x 4 grid of 32 x 32
-// blocks), so the 2048-element dot product of two expanded filters is their
-// exact AND-popcount.
+// bit becomes one FP4 (E2M1) element, 0 or 0.5, and the block-scaled MFMA
+// v_mfma_scale_f32_32x32x64_f8f6f4 (both scales 2^1, f32 accumulation; a
+// 4 x 4 grid of 32 x 32 blocks) multiplies them, so the 2048-element dot
+// product of two expanded filters is their exact AND-popcount.  FP4 is the
+// densest operand type of the matrix cores: 64 K elements per lane pair per
+// instruction, twice the int8 form at the same issue rate.
 //
-// Bit-planes: plane j (0..7) of filter word w is (w >> j) & 0x01010101, the
-// bits j, 8+j, 16+j, 24+j of w as bytes.  A K-slice of one MFMA is 32 bytes
-// per row: lanes 0-31 give planes of one word and lanes 32-63 of another (16
-// bytes each).  Over 8 K-slices the two lane halves cover words 8q..8q+3 and
-// 8q+4..8q+7; the reference rows (A) and target columns (B) use the same
-// byte-to-bit assignment, so the product sums each bit position exactly once.
+// Operands: a lane gives 32 elements (4 registers, MfOps below), all 32
+// bits of one filter word; lanes 0-31 give a word of the chunk's first half,
+// lanes 32-63 the same word of its second half.  Over 4 K-steps the two lane
+// halves cover words 8q..8q+3 and 8q+4..8q+7; the reference rows (A) and
+// target columns (B) use the same nibble-to-bit assignment, so the product
+// sums each bit position exactly once.
 //
 // Only the full 2048-bit count comes out of the matrix cores.  A pair can
 // score above 0 only if that count exceeds its cutoff cut(s1 + s2), so every
 // accumulator starts at -(cutoff + 1) of its pair (read from an LDS table
 // indexed by the sum of the two filters' size codes, size_code below) and
-// ends >= 0 exactly for those pairs.  These pairs are listed (768 per round)
-// and recomputed exactly on the VALU with the early-exit checkpoints of
-// bf_bitcount_cut_256;
-// their fractions are merged with a 64-bit LDS max on floor(num * 2^22 / den)
-// (strictly monotone in num/den for denominators below 2^11) followed by
-// num << 16 | den.
-// Result D of one 32x32x32 MFMA: lane l, register i holds row
+// ends >= 0 exactly for those pairs; all values are whole numbers below
+// 2^24, exact in f32, and a sum that ends at zero is +0.  These pairs are
+// listed (MF_CAND_CAP per round) and recomputed exactly on the VALU with the
+// early-exit checkpoints of bf_bitcount_cut_256; their fractions are merged
+// with a 64-bit LDS max on floor(num * 2^22 / den) (strictly monotone in
+// num/den for denominators below 2^11) followed by num << 16 | den.
+// Result D of one 32x32 MFMA: lane l, register i holds row
 // 8 * (i / 4) + 4 * (l / 32) + i % 4, column l % 32.
 
 typedef int sdbfx_v4i __attribute__((vector_size(16)));
+typedef int sdbfx_v8i __attribute__((vector_size(32)));
 typedef int sdbfx_v16i __attribute__((vector_size(64)));
+typedef float sdbfx_v16f __attribute__((vector_size(64)));
 
 #define MF_B 4                                  // 32-wide blocks per tile side
 #define MF_CAND_CAP 704                         // candidates listed per tile
@@ -2014,14 +2018,65 @@ __device__ __forceinline__ uint32_t word_of(const uint4& v, int k)
     return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
 }
 
-// planes sh..sh+3 of w
-__device__ __forceinline__ sdbfx_v4i planes4(uint32_t w, int sh)
+// The operands of v_mfma_scale_f32_32x32x64_f8f6f4 in FP4 (E2M1): each lane
+// gives 32 elements in 4 registers.  Register j of word w is (w >> j) &
+// 0x11111111: 8 nibbles, bit j + 4k of w in nibble k, each 0 or 0b0001 =
+// 0.5, so the four registers of a word hold all its 32 bits once.  With both
+// block scales 2^1, a product is 1.0 exactly when both bits are set, and the
+// f32 accumulator sums whole numbers (at most 2048 plus the initial value,
+// all exact).
+#define MF_SCALE 128   // E8M0 block scale 2^1
+
+// The two operand forms of cmp_mfma_body.  FP4: the form above, 4 K-steps
+// of 64 per 8 filter words.  Int8 (v_mfma_i32_32x32x32_i8, int32
+// accumulation): byte planes, plane j of word w is (w >> j) & 0x01010101,
+// the bits j, 8+j, 16+j, 24+j of w as bytes; a lane gives 16 bytes (planes
+// sh..sh+3 of one word), so a word takes 2 K-steps of 32 and 8 words take 8.
+template <bool FP4>
+struct MfOps;
+template <>
+struct MfOps<true> {
+    typedef sdbfx_v8i Op;
+    typedef sdbfx_v16f Acc;
+    typedef float El;
+    static constexpr int kSteps = 4;
+    static constexpr uint32_t kMask = 0x11111111u;
+    __device__ static __forceinline__ Acc mfma(const Op& a, const Op& b, const Acc& c)
+    {
+        return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 4, 4, 0, MF_SCALE, 0, MF_SCALE);
+    }
+    __device__ static __forceinline__ uint32_t bits(El v) { return __builtin_bit_cast(uint32_t, v); }
+    // the word and first plane of step t of a chunk's 4 words
+    __device__ static __forceinline__ int word(int t) { return t; }
+    __device__ static __forceinline__ int shift(int) { return 0; }
+};
+template <>
+struct MfOps<false> {
+    typedef sdbfx_v4i Op;
+    typedef sdbfx_v16i Acc;
+    typedef int El;
+    static constexpr int kSteps = 8;
+    static constexpr uint32_t kMask = 0x01010101u;
+    __device__ static __forceinline__ Acc mfma(const Op& a, const Op& b, const Acc& c)
+    {
+        return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
+    }
+    __device__ static __forceinline__ uint32_t bits(El v) { return (uint32_t)v; }
+    __device__ static __forceinline__ int word(int t) { return t >> 1; }
+    __device__ static __forceinline__ int shift(int t) { return 4 * (t & 1); }
+};
+
+// the operand registers of step t (planes shift(t) .. shift(t) + 3 of w; the
+// FP4 form's registers 4-7 are not read and stay 0)
+template <bool FP4>
+__device__ __forceinline__ typename MfOps<FP4>::Op mf_operand(uint32_t w, int t)
 {
-    sdbfx_v4i r;
-    r[0] = (int)((w >> sh) & 0x01010101u);
-    r[1] = (int)((w >> (sh + 1)) & 0x01010101u);
-    r[2] = (int)((w >> (sh + 2)) & 0x01010101u);
-    r[3] = (int)((w >> (sh + 3)) & 0x01010101u);
+    typedef MfOps<FP4> M;
+    typename M::Op r = {};
+    const int sh = M::shift(t);
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        r[j] = (int)((w >> (sh + j)) & M::kMask);
     return r;
 }
 
@@ -2029,8 +2084,10 @@ __device__ __forceinline__ sdbfx_v4i planes4(uint32_t w, int sh)
 // so that its initial values are not held in VGPRs up to the loop (plain form
 // under the CPU emulation).
 #if defined(HIPEMU)
+__device__ __forceinline__ void in_agprs(sdbfx_v16f&) {}
 __device__ __forceinline__ void in_agprs(sdbfx_v16i&) {}
 #else
+__device__ __forceinline__ void in_agprs(sdbfx_v16f& v) { asm volatile("" : "+a"(v)); }
 __device__ __forceinline__ void in_agprs(sdbfx_v16i& v) { asm volatile("" : "+a"(v)); }
 #endif
 
@@ -2311,11 +2368,14 @@ __global__ __launch_bounds__(1024) void cmp_prep_kernel(const uint16_t* __restri
 //
 // Latency.  The next tile's column data (element counts, Hamming weights) is
 // loaded before a tile's K loop and used after it.
-__global__ __launch_bounds__(256) void cmp_mfma_kernel(
+template <bool FP4>
+__device__ __forceinline__ void cmp_mfma_body(
     SetView A, SetView B, const CmpGroup* __restrict__ groups, uint32_t ngroups,
     const uint64_t* __restrict__ rmap, const uint32_t* __restrict__ rcode, uint64_t nwork,
     uint32_t* __restrict__ partial, unsigned long long* __restrict__ gbest)
 {
+    typedef MfOps<FP4> M;
+    typedef typename M::El El;
     __shared__ int16_t ncut[4097];                  // -(cutoff + 1) by size_code(s1) + size_code(s2)
     __shared__ MfmaWaveSmem wsm[4];
 
@@ -2496,16 +2556,16 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
             // count - cutoff > m.  m is computed with a 24-bit multiply; when
             // min(e1, e2) <= cutoff (d <= 0) the pair cannot pass whatever m
             // is, and the wrapped product still gives m >= 0.
-            sdbfx_v16i acc[MF_B][MF_B];
+            typename M::Acc acc[MF_B][MF_B];
             uint32_t rb = h;
             opaque(rb);
 #pragma unroll
             for (int bm = 0; bm < MF_B; bm++) {
                 if (uniform) {
-                    int v[16];
+                    El v[16];
 #pragma unroll
                     for (int i = 0; i < 16; i++)
-                        v[i] = (int)bfix[4 * rb + 32 * bm + 8 * (i >> 2) + (i & 3)];
+                        v[i] = (El)(int)bfix[4 * rb + 32 * bm + 8 * (i >> 2) + (i & 3)];
 #pragma unroll
                     for (int bn = 0; bn < MF_B; bn++) {
 #pragma unroll
@@ -2530,7 +2590,7 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
                         for (int i = 0; i < 16; i++) {
                             const int n = *(const int16_t*)(ncutb + rc[i] + ccode[bn]);
                             const uint32_t d = min(e1c[i], e2c[bn]) + (uint32_t)n;
-                            acc[bm][bn][i] = n - (int)(((bq[i] & 0xFFFFFFu) * (d & 0xFFFFFFu)) >> 16);
+                            acc[bm][bn][i] = (El)(n - (int)(((bq[i] & 0xFFFFFFu) * (d & 0xFFFFFFu)) >> 16));
                         }
                         in_agprs(acc[bm][bn]);
                         __builtin_amdgcn_sched_barrier(0);
@@ -2538,12 +2598,12 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
                 }
             }
 
-            // bit-planes of the first step
-            sdbfx_v4i av[MF_B], bv[MF_B];
+            // operands of the first step
+            typename M::Op av[MF_B], bv[MF_B];
 #pragma unroll
             for (int k = 0; k < MF_B; k++) {
-                av[k] = planes4(aw[k].x, 0);
-                bv[k] = planes4(bw[k].x, 0);
+                av[k] = mf_operand<FP4>(aw[k].x, 0);
+                bv[k] = mf_operand<FP4>(bw[k].x, 0);
             }
 #pragma unroll 1
             for (int q = 0; q < 8; q++) {
@@ -2558,27 +2618,32 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
                     bnx[k] = tbase[cbt[k] + 2 * qn];
                 }
                 __builtin_amdgcn_sched_barrier(0);
-                // step t: word t / 2, planes 4 (t % 2) .. 4 (t % 2) + 3.  Each
-                // MFMA of step t is followed by two plane elements (four VALU
-                // instructions) of step t + 1, so that they run in its shadow.
+                // step t: word M::word(t) of the chunk, planes M::shift(t) ..
+                // M::shift(t) + 3 (FP4: all 32 bits of word t, 64 K elements
+                // with the other lane half's word).  Each MFMA of step t is
+                // followed by two operand registers (four VALU instructions)
+                // of step t + 1, so that they run in its shadow.
 #pragma unroll
-                for (int t = 0; t < 8; t++) {
-                    const int shn = 4 * ((t + 1) & 1);
+                for (int t = 0; t < M::kSteps; t++) {
+                    const int shn = M::shift(t + 1);
                     uint32_t wn[2 * MF_B];
 #pragma unroll
                     for (int k = 0; k < MF_B; k++) {
-                        wn[k] = t < 7 ? word_of(aw[k], (t + 1) >> 1) : an[k].x;
-                        wn[MF_B + k] = t < 7 ? word_of(bw[k], (t + 1) >> 1) : bnx[k].x;
+                        wn[k] = t < M::kSteps - 1 ? word_of(aw[k], M::word(t + 1)) : an[k].x;
+                        wn[MF_B + k] = t < M::kSteps - 1 ? word_of(bw[k], M::word(t + 1)) : bnx[k].x;
                     }
-                    sdbfx_v4i nv[2 * MF_B];
+                    typename M::Op nv[2 * MF_B];
+#pragma unroll
+                    for (int k = 0; k < 2 * MF_B; k++)
+                        nv[k] = typename M::Op{};
 #pragma unroll
                     for (int gi = 0; gi < MF_B * MF_B; gi++) {
                         const int bm = gi / MF_B, bn = gi % MF_B;
-                        acc[bm][bn] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[bm], bv[bn], acc[bm][bn], 0, 0, 0);
+                        acc[bm][bn] = M::mfma(av[bm], bv[bn], acc[bm][bn]);
 #pragma unroll
                         for (int e2i = 0; e2i < 2; e2i++) {
                             const int el = 2 * gi + e2i, p = el >> 2, e = el & 3;
-                            nv[p][e] = (int)((wn[p] >> (shn + e)) & 0x01010101u);
+                            nv[p][e] = (int)((wn[p] >> (shn + e)) & M::kMask);
                         }
                         __builtin_amdgcn_sched_barrier(0);
                     }
@@ -2618,9 +2683,13 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
                     // a new value of the block, defined here in AGPRs, so that
                     // its copies to VGPRs are made here and not all at once
                     in_agprs(acc[bm][bn]);
+                    // (each element copied out first: a bit cast of the
+                    // subscripted vector itself reads element 0 for every i)
 #pragma unroll
-                    for (int i = 0; i < 16; i++)
-                        x = __builtin_amdgcn_alignbit(x, (uint32_t)acc[bm][bn][i], 31);
+                    for (int i = 0; i < 16; i++) {
+                        const El v = acc[bm][bn][i];
+                        x = __builtin_amdgcn_alignbit(x, M::bits(v), 31);
+                    }
                     __builtin_amdgcn_sched_barrier(0);
                 }
                 cbits[wi] = ~x;
@@ -2691,6 +2760,24 @@ __global__ __launch_bounds__(256) void cmp_mfma_kernel(
         }
         wave_lds_sync();
     }
+}
+
+// The matrix-core comparison with FP4 operands (the default form) and with
+// int8 byte planes; both give the same partial maxima.
+__global__ __launch_bounds__(256) void cmp_mfma_kernel(
+    SetView A, SetView B, const CmpGroup* __restrict__ groups, uint32_t ngroups,
+    const uint64_t* __restrict__ rmap, const uint32_t* __restrict__ rcode, uint64_t nwork,
+    uint32_t* __restrict__ partial, unsigned long long* __restrict__ gbest)
+{
+    cmp_mfma_body<true>(A, B, groups, ngroups, rmap, rcode, nwork, partial, gbest);
+}
+
+__global__ __launch_bounds__(256) void cmp_mfma_i8_kernel(
+    SetView A, SetView B, const CmpGroup* __restrict__ groups, uint32_t ngroups,
+    const uint64_t* __restrict__ rmap, const uint32_t* __restrict__ rcode, uint64_t nwork,
+    uint32_t* __restrict__ partial, unsigned long long* __restrict__ gbest)
+{
+    cmp_mfma_body<false>(A, B, groups, ngroups, rmap, rcode, nwork, partial, gbest);
 }
 
 // Per reference-filter instance: the largest fraction over its target

@@ -193,7 +193,7 @@ struct sdbfx_server {
 
     // True when the MFMA kernel's padded work on (batch x sample) is under
     // half the VALU kernel's, by the engine's cost model (sdbfx_host.cpp,
-    // use_mfma: 2.75 vs 8 SIMD cycles per padded pair slot; query filters
+    // use_mfma: 2 vs 8 SIMD cycles per padded pair slot; query filters
     // padded to 128 and 64 rows, target filters to 128 and 1); then the
     // trial is skipped.  The batch's digests all face every sample digest,
     // so the instances per target digest are the batch's filter count.
@@ -207,7 +207,7 @@ struct sdbfx_server {
                 return false;
             const double t = info.bf_count;
             valu += 8.0 * std::ceil(nq / 64) * 64 * t;
-            mfma += 2.75 * std::ceil(nq / 128) * 128 * std::ceil(t / 128) * 128;
+            mfma += 2.0 * std::ceil(nq / 128) * 128 * std::ceil(t / 128) * 128;
         }
         return mfma < 0.5 * valu;
     }

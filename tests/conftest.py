@@ -20,10 +20,12 @@ def engine():
     e.close()
 
 
-@pytest.fixture(params=["valu", "mfma"])
+@pytest.fixture(params=["valu", "mfma", "mfma_i8"])
 def cmp_engine(request, engine):
-    """The engine with one of its two comparison kernels selected."""
+    """The engine with one of its comparison kernels selected: VALU
+    popcount, MFMA on FP4 bit elements, MFMA on int8 byte planes."""
     import sdhash_amd
-    engine.compare_method(sdhash_amd.CMP_VALU if request.param == "valu" else sdhash_amd.CMP_MFMA)
+    engine.compare_method({"valu": sdhash_amd.CMP_VALU, "mfma": sdhash_amd.CMP_MFMA,
+                           "mfma_i8": sdhash_amd.CMP_MFMA_I8}[request.param])
     yield engine
     engine.compare_method(sdhash_amd.CMP_AUTO)

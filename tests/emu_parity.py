@@ -142,7 +142,7 @@ def main(lib):
     for x in reps:
         gr.add(x.name, x.orig_size, x.bf_count, x.last_count, 0, 160, x.filters, None)
     want_rep = oracle.compare_all(reps)
-    for method in (sdhash_amd.CMP_MFMA, sdhash_amd.CMP_VALU):
+    for method in (sdhash_amd.CMP_MFMA, sdhash_amd.CMP_MFMA_I8, sdhash_amd.CMP_VALU):
         eng.compare_method(method)
         assert np.array_equal(eng.compare_all(s), want_all), method
         assert np.array_equal(eng.compare_to(gq, gt), want), method
@@ -270,6 +270,24 @@ def main(lib):
         assert C.string_at(fb.ptr.value, n * 256) == wf.tobytes()
         assert C.string_at(eb.ptr.value, n * 2) == wec.tobytes()
         assert C.string_at(hb.ptr.value, n * 2) == whm.tobytes()
+    # the same launches captured into a graph and replayed twice
+    size, dd = 3 * 16 * KIB + 700, 16384
+    d = eng.alloc(size)
+    d.upload(0, img[:size])
+    n = size // dd + 1
+    fb, eb, hb = eng.alloc(n * 256), eng.alloc(n * 2), eng.alloc(n * 2)
+    g = C.c_void_p()
+    cnt = C.c_uint32()
+    assert eng.lib.sdbfx_graph_blocks_create(eng.ctx, d.ptr, size, dd, fb.ptr, eb.ptr, hb.ptr, C.byref(cnt),
+                                             C.byref(g)) == 0
+    for _ in range(2):
+        assert eng.lib.sdbfx_graph_launch(eng.ctx, g) == 0
+        eng.sync()
+        ns = eng.new_set()
+        assert np.all(ns.digest_device(d, [0], [size], [b"x"], dd_block=dd) == 0)
+        wf, _, _ = ns.read(0)
+        assert C.string_at(fb.ptr.value, n * 256) == wf.tobytes()
+    eng.lib.sdbfx_graph_destroy(g)
     print("noalloc ok", flush=True)
 
     # indexes (--index, --index-search): stream inputs gated by the index,

@@ -392,6 +392,50 @@ inline hipemu_v16i __builtin_amdgcn_mfma_i32_32x32x32_i8(hipemu_v4i a, hipemu_v4
     return d;
 }
 
+// v_mfma_scale_f32_32x32x64_f8f6f4 with FP4 (E2M1) A and B (cbsz = blgp = 4):
+// A lane l = row l%32, 32 nibbles (registers 0-3, low nibble first) of
+// K-half l/32; B lane l = column l%32, same K assignment; each element times
+// its block scale 2^(scale - 127); D lane l register i = row
+// 8*(i/4) + 4*(l/32) + i%4, column l%32.
+typedef int hipemu_v8i __attribute__((vector_size(32)));
+typedef float hipemu_v16f __attribute__((vector_size(64)));
+inline float hipemu_fp4(uint32_t nib)
+{
+    static const float mag[8] = {0.f, 0.5f, 1.f, 1.5f, 2.f, 3.f, 4.f, 6.f};
+    return (nib & 8) ? -mag[nib & 7] : mag[nib & 7];
+}
+inline hipemu_v16f __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(hipemu_v8i a, hipemu_v8i b, hipemu_v16f c,
+                                                                   int cbsz, int blgp, int, int sa, int, int sb)
+{
+    if (cbsz != 4 || blgp != 4)
+        abort();   // only the FP4 form is emulated
+    hipemu::Block* bk = hipemu::tl_block;
+    const int base = hipemu::wave() * hipemu::kWave, l = hipemu::lane();
+    auto& me = bk->mslot[threadIdx.x];
+    for (int k = 0; k < 4; k++) {
+        me[k] = a[k];
+        me[4 + k] = b[k];
+    }
+    hipemu::wave_sync();
+    const float s = std::ldexp(1.0f, sa - 127) * std::ldexp(1.0f, sb - 127);
+    hipemu_v16f d;
+    const int n = l % 32;
+    for (int i = 0; i < 16; i++) {
+        const int m = 8 * (i / 4) + 4 * (l / 32) + i % 4;
+        float acc = c[i];
+        for (int hh = 0; hh < 2; hh++) {
+            const int32_t* pa = bk->mslot[base + 32 * hh + m].data();
+            const int32_t* pb = bk->mslot[base + 32 * hh + n].data() + 4;
+            for (int j = 0; j < 32; j++)
+                acc += hipemu_fp4(((uint32_t)pa[j / 8] >> (4 * (j % 8))) & 15u) *
+                       hipemu_fp4(((uint32_t)pb[j / 8] >> (4 * (j % 8))) & 15u) * s;
+        }
+        d[i] = acc;
+    }
+    hipemu::wave_sync();
+    return d;
+}
+
 inline uint32_t __builtin_amdgcn_mbcnt_lo(uint32_t m, uint32_t base)
 {
     const int l = hipemu::lane();
@@ -505,12 +549,56 @@ inline hipError_t hipHostFree(void* p) { free(p); return hipSuccess; }
 #define HIP_SYMBOL(x) (&(x))
 inline hipError_t hipGetSymbolAddress(void** p, const void* sym) { *p = const_cast<void*>(sym); return hipSuccess; }
 
+// graphs: launches on a capturing stream are recorded (arguments copied) and
+// run by hipGraphLaunch
+struct hipemu_graph {
+    std::vector<std::function<void()>> ops;
+};
+typedef hipemu_graph* hipGraph_t;
+typedef hipemu_graph* hipGraphExec_t;
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal, hipStreamCaptureModeThreadLocal, hipStreamCaptureModeRelaxed };
+inline hipemu_graph*& hipemu_capturing()
+{
+    static thread_local hipemu_graph* g = nullptr;
+    return g;
+}
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode)
+{
+    hipemu_capturing() = new hipemu_graph;
+    return hipSuccess;
+}
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g)
+{
+    *g = hipemu_capturing();
+    hipemu_capturing() = nullptr;
+    return hipSuccess;
+}
+inline hipError_t hipGraphInstantiate(hipGraphExec_t* x, hipGraph_t g, void*, void*, size_t)
+{
+    *x = new hipemu_graph(*g);
+    return hipSuccess;
+}
+inline hipError_t hipGraphLaunch(hipGraphExec_t x, hipStream_t)
+{
+    for (auto& op : x->ops)
+        op();
+    return hipSuccess;
+}
+inline hipError_t hipGraphDestroy(hipGraph_t g) { delete g; return hipSuccess; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t x) { delete x; return hipSuccess; }
+
 #define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...)                          \
     do {                                                                                     \
         if (getenv("HIPEMU_TRACE"))                                                          \
             fprintf(stderr, "[hipemu] %s grid %u block %u\n", #kernel, (unsigned)dim3(grid).x, \
                     (unsigned)dim3(block).x);                                                \
-        hipemu::launch((grid), (block), [&]() { kernel(__VA_ARGS__); });                     \
+        if (hipemu_capturing()) {                                                            \
+            const dim3 g_ = dim3(grid), b_ = dim3(block);                                    \
+            hipemu_capturing()->ops.push_back(                                               \
+                [=]() { hipemu::launch(g_, b_, [&]() { kernel(__VA_ARGS__); }); });          \
+        } else {                                                                             \
+            hipemu::launch((grid), (block), [&]() { kernel(__VA_ARGS__); });                 \
+        }                                                                                    \
     } while (0)
 
 #endif

@@ -1,6 +1,7 @@
 """Comparison parity on the GPU: every 0-100 score (and -1) identical with
 the CPU oracle, including the reference's orientation rule, for both
-comparison kernels (VALU popcount and int8 MFMA on bit-planes)."""
+comparison kernels (VALU popcount, MFMA on FP4 bit elements and MFMA on
+int8 byte planes)."""
 import numpy as np
 import pytest
 
