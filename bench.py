@@ -732,6 +732,7 @@ def main() -> int:
     ap.add_argument("--allpairs-mfilters", type=float, default=2.0)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain 64 x 1 MiB all-pairs measurement")
     ap.add_argument("--no-index", action="store_true", help="skip the --index / --index-search measurement")
+    ap.add_argument("--no-graph", action="store_true", help="skip the HIP graph replay measurement")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -882,7 +883,8 @@ def main() -> int:
     del tset, qset
     # the other sections use the engine's cost model, as the server does
     eng.autotune(False)
-    out["graph_replay"] = graph_replay(eng, dimg)
+    if not args.no_graph:
+        out["graph_replay"] = graph_replay(eng, dimg)
     if not args.no_index:
         log("index ...")
         out["index"] = index_section(eng, dimg, img)
