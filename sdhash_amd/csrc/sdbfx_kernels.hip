@@ -1996,8 +1996,9 @@ __global__ void fold_kernel(uint32_t* __restrict__ filt, uint16_t* __restrict__ 
 // score above 0 only if that count exceeds its cutoff cut(s1 + s2), so every
 // accumulator starts at -(cutoff + 1) of its pair (read from an LDS table
 // indexed by the sum of the two filters' size codes, size_code below) and
-// ends >= 0 exactly for those pairs; all values are whole numbers below
-// 2^24, exact in f32, and a sum that ends at zero is +0.  These pairs are
+// ends >= 0 exactly for those pairs; all values are whole numbers (plus
+// one half in the FP4 form, so that no sum is 0) below 2^24, exact in f32,
+// and the sign bit of each sum is the test.  These pairs are
 // listed (MF_CAND_CAP per round) and recomputed exactly on the VALU with the
 // early-exit checkpoints of bf_bitcount_cut_256; their fractions are merged
 // with a 64-bit LDS max on floor(num * 2^22 / den) (strictly monotone in
@@ -2023,8 +2024,8 @@ __device__ __forceinline__ uint32_t word_of(const uint4& v, int k)
 // 0x11111111: 8 nibbles, bit j + 4k of w in nibble k, each 0 or 0b0001 =
 // 0.5, so the four registers of a word hold all its 32 bits once.  With both
 // block scales 2^1, a product is 1.0 exactly when both bits are set, and the
-// f32 accumulator sums whole numbers (at most 2048 plus the initial value,
-// all exact).
+// f32 accumulator adds whole numbers (at most 2048) to the initial value, all
+// exact.
 #define MF_SCALE 128   // E8M0 block scale 2^1
 
 // The two operand forms of cmp_mfma_body.  FP4: the form above, 4 K-steps
@@ -2046,6 +2047,10 @@ struct MfOps<true> {
         return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 4, 4, 0, MF_SCALE, 0, MF_SCALE);
     }
     __device__ static __forceinline__ uint32_t bits(El v) { return __builtin_bit_cast(uint32_t, v); }
+    // an accumulator's initial value x as x + 0.5: the sums are then never
+    // 0, and x + count >= 0 exactly when the sign bit is clear, whatever the
+    // sign of a zero result would be
+    __device__ static __forceinline__ El initial(int x) { return (float)x + 0.5f; }
     // the word and first plane of step t of a chunk's 4 words
     __device__ static __forceinline__ int word(int t) { return t; }
     __device__ static __forceinline__ int shift(int) { return 0; }
@@ -2062,6 +2067,7 @@ struct MfOps<false> {
         return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
     }
     __device__ static __forceinline__ uint32_t bits(El v) { return (uint32_t)v; }
+    __device__ static __forceinline__ El initial(int x) { return x; }
     __device__ static __forceinline__ int word(int t) { return t >> 1; }
     __device__ static __forceinline__ int shift(int t) { return 4 * (t & 1); }
 };
@@ -2565,7 +2571,7 @@ __device__ __forceinline__ void cmp_mfma_body(
                     El v[16];
 #pragma unroll
                     for (int i = 0; i < 16; i++)
-                        v[i] = (El)(int)bfix[4 * rb + 32 * bm + 8 * (i >> 2) + (i & 3)];
+                        v[i] = M::initial((int)bfix[4 * rb + 32 * bm + 8 * (i >> 2) + (i & 3)]);
 #pragma unroll
                     for (int bn = 0; bn < MF_B; bn++) {
 #pragma unroll
@@ -2590,7 +2596,7 @@ __device__ __forceinline__ void cmp_mfma_body(
                         for (int i = 0; i < 16; i++) {
                             const int n = *(const int16_t*)(ncutb + rc[i] + ccode[bn]);
                             const uint32_t d = min(e1c[i], e2c[bn]) + (uint32_t)n;
-                            acc[bm][bn][i] = (El)(n - (int)(((bq[i] & 0xFFFFFFu) * (d & 0xFFFFFFu)) >> 16));
+                            acc[bm][bn][i] = M::initial(n - (int)(((bq[i] & 0xFFFFFFu) * (d & 0xFFFFFFu)) >> 16));
                         }
                         in_agprs(acc[bm][bn]);
                         __builtin_amdgcn_sched_barrier(0);

@@ -85,6 +85,44 @@ def test_compare_hand_cases(cmp_engine):
     assert got[3] == -1          # a|e: every target filter sparse
 
 
+def test_compare_extreme_bit_patterns(cmp_engine):
+    # Filters whose bits sit in single operand registers of the MFMA forms
+    # (only bit 0, only bit 3 of every nibble; byte planes 0 and 7), full
+    # and empty words at the ends of the K steps, all 2048 bits set (the
+    # largest accumulator sum), disjoint halves (counts of 0) and prefixes
+    # that pass or fail the early exits of bf_bitcount_cut_256.
+    def rep(b):
+        return np.full(256, b, np.uint8)
+    def words(*idx):
+        v = np.zeros(256, np.uint8)
+        for i in idx:
+            v[4 * i:4 * i + 4] = 0xFF
+        return v
+    def prefix(n):
+        v = np.zeros(256, np.uint8)
+        v[:n] = 0xFF
+        return v
+    pats = [rep(0xFF), rep(0x55), rep(0xAA), rep(0x11), rep(0x88), rep(0x01), rep(0x80),
+            words(0, 63), words(*range(0, 64, 2)), words(*range(1, 64, 2)), prefix(32), prefix(64),
+            prefix(128), prefix(160), np.concatenate([rep(0xFF)[:128], rep(0x0F)[:128]])]
+    groups = [(0, 1, 2, 3), (4, 5, 6, 7, 8), (9, 10, 11, 12), (13, 14, 0), (1, 4, 9, 13), (2, 5, 10, 14), (0,)]
+    ds = []
+    for k, g in enumerate(groups):
+        f = np.concatenate([pats[i] for i in g])
+        ds.append(oracle.Digest(b"x%d" % k, f, len(g), 160, None, 100000, 0))
+    s = upload_oracle(cmp_engine, ds)
+    want = oracle.compare_all(ds)
+    got = cmp_engine.compare_all(s)
+    assert np.array_equal(got, want), (got, want)
+    assert (want == 100).any() and (want > 0).sum() >= 5
+    # and one against many: the all-ones filter facing every pattern
+    q = [oracle.Digest(b"q", rep(0xFF), 1, 160, None, 1000, 0)]
+    t = [oracle.Digest(b"t%02d" % i, p, 1, 160, None, 1000, 0) for i, p in enumerate(pats)]
+    want = oracle.compare_to(q, t)
+    got = cmp_engine.compare_to(upload_oracle(cmp_engine, q), upload_oracle(cmp_engine, t))
+    assert np.array_equal(got, want), (got, want)
+
+
 def test_compare_large_target_chunks(cmp_engine):
     # a target digest of > 1024 filters is split over several work items
     img = synth.disk_image(35, 20 * MIB)
