@@ -68,6 +68,13 @@ struct sdbfx_ctx {
     hipEvent_t pin_ev[2] = {nullptr, nullptr};
     bool pin_busy[2] = {false, false};
     int pin_next = 0;
+    // pinned ring for the small uploads (work lists, descriptors); its
+    // space is reused after ring_ev, recorded after the latest of them
+    uint8_t* ring = nullptr;
+    uint64_t ring_head = 0;
+    hipEvent_t ring_ev = nullptr;
+    // pinned buffer for the small read-backs (counts, element counts, scores)
+    uint8_t* rb = nullptr;
 };
 
 struct sdbfx_set {
@@ -215,14 +222,73 @@ int grow_keep(sdbfx_ctx* ctx, DevBuf& b, size_t bytes, size_t keep)
     return 0;
 }
 
+constexpr uint64_t kRingBytes = 16ull << 20;
+// Copies below this size are made by a copy kernel even between pinned host
+// memory and the device; small transfers are padded to it (the buffers on
+// both sides hold at least this much) so that they are DMA copies.
+constexpr uint64_t kDmaMin = 32ull << 10;
+
+// A host-to-device copy of host data through the pinned ring, as a DMA copy:
+// a copy kernel (what a copy from pageable memory, or a small one, becomes)
+// waits for compute units behind a comparison running on another stream, and
+// holds up this stream and the calling thread with it.  dst must hold
+// max(bytes, kDmaMin).  Copies larger than the ring go directly.
+int upload_small(sdbfx_ctx* ctx, void* dst, const void* src, uint64_t bytes)
+{
+    const uint64_t xfer = std::max(bytes, kDmaMin);
+    const uint64_t need = (xfer + 255) & ~255ull;
+    if (need > kRingBytes) {
+        HIPC(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        return 0;
+    }
+    if (!ctx->ring) {
+        HIPC(ctx, hipHostMalloc((void**)&ctx->ring, kRingBytes, hipHostMallocDefault));
+        HIPC(ctx, hipEventCreate(&ctx->ring_ev));
+        ctx->ring_head = 0;
+    } else if (ctx->ring_head + need > kRingBytes) {
+        // every earlier copy out of the ring is done once the latest is
+        HIPC(ctx, hipEventSynchronize(ctx->ring_ev));
+        ctx->ring_head = 0;
+    }
+    uint8_t* p = ctx->ring + ctx->ring_head;
+    ctx->ring_head += need;
+    memcpy(p, src, bytes);
+    HIPC(ctx, hipMemcpyAsync(dst, p, xfer, hipMemcpyHostToDevice, ctx->stream));
+    HIPC(ctx, hipEventRecord(ctx->ring_ev, ctx->stream));
+    return 0;
+}
+
+constexpr uint64_t kRbBytes = 4ull << 20;
+
+// A device-to-host read-back that waits for the stream, through the pinned
+// read-back buffer so that it is a DMA copy (into pageable memory, or when
+// small, it would be a copy kernel, held up behind a comparison on another
+// stream); `avail` bytes are readable at src, and up to kDmaMin of them are
+// read.  Larger read-backs go directly.
+int download(sdbfx_ctx* ctx, void* dst, const void* src, uint64_t bytes, uint64_t avail)
+{
+    if (bytes > kRbBytes) {
+        HIPC(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        HIPC(ctx, hipStreamSynchronize(ctx->stream));
+        return 0;
+    }
+    if (!ctx->rb)
+        HIPC(ctx, hipHostMalloc((void**)&ctx->rb, kRbBytes, hipHostMallocDefault));
+    if (bytes)
+        HIPC(ctx, hipMemcpyAsync(ctx->rb, src, std::max(bytes, std::min(avail, kDmaMin)), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    memcpy(dst, ctx->rb, bytes);
+    return 0;
+}
+
 template <typename T>
 int upload(sdbfx_ctx* ctx, DevBuf& b, const std::vector<T>& v)
 {
-    if (int rc = ensure(ctx, b, v.size() * sizeof(T) + 16))
+    if (int rc = ensure(ctx, b, std::max<uint64_t>(v.size() * sizeof(T) + 16, kDmaMin)))
         return rc;
     if (!v.empty())
-        HIPC(ctx, hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice,
-                                 ctx->stream));
+        return upload_small(ctx, b.p, v.data(), v.size() * sizeof(T));
     return 0;
 }
 
@@ -322,9 +388,9 @@ int block_index_pass(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t nblk, IndexCall* i
         const uint64_t nt = (uint64_t)nblk * (SDBFX_MAX_ELEM_DD / 4);
         LAUNCH(ctx, index_search_kernel, (uint32_t)((nt + 255) / 256), 256, bl, nblk, ecf, feat,
                (const IndexRef*)ctx->irefs.p, nix, (uint32_t*)ctx->imatch.p);
-        HIPC(ctx, hipMemcpyAsync(ic->matches + ic->mpos * nix, ctx->imatch.p, (uint64_t)nblk * nix * 4,
-                                 hipMemcpyDeviceToHost, ctx->stream));
-        HIPC(ctx, hipStreamSynchronize(ctx->stream));
+        if (int rc2 = download(ctx, ic->matches + ic->mpos * nix, ctx->imatch.p, (uint64_t)nblk * nix * 4,
+                               ctx->imatch.cap))
+            return rc2;
         ic->mpos += nblk;
         return 0;
     }
@@ -420,9 +486,8 @@ int digest_pass(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t i0, uint32_t i1, const 
                    (const TileDesc*)ctx->tiles.p, (const TileState*)ctx->tstate.p,
                    (const uint32_t*)ctx->cand.p, (uint8_t*)ctx->sfilt.p, (uint16_t*)ctx->sham.p,
                    (uint32_t*)ctx->counts.p);
-        HIPC(ctx, hipMemcpyAsync(counts.data(), ctx->counts.p, counts.size() * 4, hipMemcpyDeviceToHost,
-                                 ctx->stream));
-        HIPC(ctx, hipStreamSynchronize(ctx->stream));
+        if ((rc = download(ctx, counts.data(), ctx->counts.p, counts.size() * 4, ctx->counts.cap)))
+            return rc;
         for (size_t f = 0; f < sfiles.size(); f++)
             if (counts[3 * f + 2])
                 return fail(ctx, "stream digest exceeded its filter capacity");
@@ -502,9 +567,9 @@ int digest_pass(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t i0, uint32_t i1, const 
     // ---- metadata (block digests need their last element count on the host)
     std::vector<uint16_t> ec_all(total);
     if (total) {
-        HIPC(ctx, hipMemcpyAsync(ec_all.data(), (uint16_t*)set->ecf.p + set->nfilt, total * 2,
-                                 hipMemcpyDeviceToHost, ctx->stream));
-        HIPC(ctx, hipStreamSynchronize(ctx->stream));
+        if (int rc = download(ctx, ec_all.data(), (uint16_t*)set->ecf.p + set->nfilt, total * 2,
+                              set->ecf.cap - set->nfilt * 2))
+            return rc;
     }
     size_t f = 0;
     for (uint32_t i = i0; i < i1; i++) {
@@ -555,6 +620,51 @@ int upload_host(sdbfx_ctx* ctx, uint8_t* dst, const uint8_t* src, uint64_t len)
         const uint64_t c = std::min(kPinBytes, len - o);
         memcpy(ctx->pin[k], src + o, c);
         HIPC(ctx, hipMemcpyAsync(dst + o, ctx->pin[k], c, hipMemcpyHostToDevice, ctx->stream));
+        HIPC(ctx, hipEventRecord(ctx->pin_ev[k], ctx->stream));
+        ctx->pin_busy[k] = true;
+    }
+    return 0;
+}
+
+// The padded device image of a pass's host inputs, bytes [0, end) of dst:
+// input i at hoff[i] (i in [i0, i1)), zeros elsewhere.  It is assembled on
+// the host in windows of the two pinned buffers, each window one copy, so
+// that every transfer is a DMA copy from pinned memory: nothing on the
+// stream needs compute units (a copy kernel or a fill would wait for them
+// behind a comparison running on another stream, and the staging with it).
+int upload_pass(sdbfx_ctx* ctx, uint8_t* dst, uint32_t i0, uint32_t i1, const std::vector<uint64_t>& hoff,
+                const uint8_t* const* host_bufs, const uint64_t* lens, uint64_t end)
+{
+    if (!ctx->pin[0])
+        for (int k = 0; k < 2; k++) {
+            HIPC(ctx, hipHostMalloc(&ctx->pin[k], kPinBytes, hipHostMallocDefault));
+            HIPC(ctx, hipEventCreate(&ctx->pin_ev[k]));
+        }
+    uint32_t j = i0;   // the first input that ends after the window's start
+    for (uint64_t w0 = 0; w0 < end; w0 += kPinBytes) {
+        const uint64_t w1 = std::min(end, w0 + kPinBytes);
+        const int k = ctx->pin_next;
+        ctx->pin_next ^= 1;
+        if (ctx->pin_busy[k])
+            HIPC(ctx, hipEventSynchronize(ctx->pin_ev[k]));
+        uint8_t* pb = (uint8_t*)ctx->pin[k];
+        uint64_t c = w0;
+        while (c < w1) {
+            while (j < i1 && hoff[j] + lens[j] <= c)
+                j++;
+            if (j == i1 || hoff[j] >= w1) {
+                memset(pb + (c - w0), 0, w1 - c);
+                break;
+            }
+            if (c < hoff[j]) {
+                memset(pb + (c - w0), 0, hoff[j] - c);
+                c = hoff[j];
+            }
+            const uint64_t e = std::min(w1, hoff[j] + lens[j]);
+            memcpy(pb + (c - w0), host_bufs[j] + (c - hoff[j]), e - c);
+            c = e;
+        }
+        HIPC(ctx, hipMemcpyAsync(dst + w0, pb, w1 - w0, hipMemcpyHostToDevice, ctx->stream));
         HIPC(ctx, hipEventRecord(ctx->pin_ev[k], ctx->stream));
         ctx->pin_busy[k] = true;
     }
@@ -619,12 +729,8 @@ int digest_common(sdbfx_ctx* ctx, sdbfx_set* set, uint32_t n, const uint8_t* d_b
             }
             if (int rc = ensure(ctx, ctx->stage, pos + SDBFX_PAD))
                 return rollback(rc);
-            if (hipMemsetAsync(ctx->stage.p, 0, SDBFX_PAD, ctx->stream) != hipSuccess)
-                return rollback(fail(ctx, "hipMemsetAsync failed"));
-            for (uint32_t i = i0; i < i1; i++)
-                if (lens[i])
-                    if (int rc = upload_host(ctx, (uint8_t*)ctx->stage.p + hoff[i], host_bufs[i], lens[i]))
-                        return rollback(rc);
+            if (int rc = upload_pass(ctx, (uint8_t*)ctx->stage.p, i0, i1, hoff, host_bufs, lens, pos + SDBFX_PAD))
+                return rollback(rc);
             base = (const uint8_t*)ctx->stage.p;
             o = hoff.data();
         }
@@ -970,8 +1076,8 @@ int launch_pairs(sdbfx_ctx* ctx, const sdbfx_set* const S[2], bool all, uint32_t
         LAUNCH(ctx, pair_sum_kernel, (uint32_t)((npairs + 255) / 256), 256, (const CmpPair*)ctx->pairs.p,
                (uint32_t)npairs, va.ecf, vb.ecf, (const uint32_t*)ctx->imax.p, (int32_t*)ctx->scores.p);
     }
-    HIPC(ctx, hipMemcpyAsync(sorted, ctx->scores.p, npairs * 4, hipMemcpyDeviceToHost, ctx->stream));
-    HIPC(ctx, hipStreamSynchronize(ctx->stream));
+    if ((rc = download(ctx, sorted, ctx->scores.p, npairs * 4, ctx->scores.cap)))
+        return rc;
     for (uint64_t s = 0; s < npairs; s++)
         if (sorted[s] == SDBFX_SCORE_NONFINITE)
             return fail(ctx, "a pair's score is not finite (a plain reference filter against a folded target "
@@ -1011,8 +1117,8 @@ int compare_impl(sdbfx_ctx* ctx, const sdbfx_set* A, const sdbfx_set* B, bool al
             need |= A->dd[i] && A->bfc[i] > sample;
         if (need) {
             ecA.resize(A->nfilt);
-            HIPC(ctx, hipMemcpyAsync(ecA.data(), A->ecf.p, A->nfilt * 2, hipMemcpyDeviceToHost, ctx->stream));
-            HIPC(ctx, hipStreamSynchronize(ctx->stream));
+            if (int rc = download(ctx, ecA.data(), A->ecf.p, A->nfilt * 2, A->ecf.cap))
+                return rc;
         }
     }
     // orientation of every pair, in output order (sdbf_core.cc:349-359)
@@ -1095,6 +1201,12 @@ void sdbfx_ctx_destroy(sdbfx_ctx* ctx)
         if (ctx->pin_ev[k])
             (void)hipEventDestroy(ctx->pin_ev[k]);
     }
+    if (ctx->ring)
+        (void)hipHostFree(ctx->ring);
+    if (ctx->rb)
+        (void)hipHostFree(ctx->rb);
+    if (ctx->ring_ev)
+        (void)hipEventDestroy(ctx->ring_ev);
     for (auto& ev : ctx->pending) {
         (void)hipEventDestroy(ev.a);
         (void)hipEventDestroy(ev.b);

@@ -1224,6 +1224,7 @@ struct InsertSmem {
     uint32_t twice[SDBFX_BIG_BYTES / 4];    // positions of the batch set by two lanes
     uint32_t own[SDBFX_BF * 8];             // lowest lane setting each filter bit
     uint32_t fw[SDBFX_BFW];                 // current filter
+    uint32_t tinfo[64];                     // tiles [tw0, tw0 + 64): count | first << 31
 };
 #define IX_SLOTS 1024                        // index positions of one batch: at most 64 * 5
 struct InsertSmemIx : InsertSmem {
@@ -1307,7 +1308,23 @@ __device__ __forceinline__ void stream_insert_body(
     // Each batch's SHA-1 words are loaded one batch ahead, so that their
     // latency overlaps the batch before; a tile's restart flag (the first
     // tile of a 32 MiB unit resets the de-duplication count) is applied
-    // when the first batch at or after it is taken.
+    // when the first batch at or after it is taken.  The tiles' candidate
+    // counts and restart flags are read 64 tiles at a time (one load per
+    // lane) into LDS, so that moving to the next tile waits for no load.
+    uint32_t tw0 = 0xFFFFFFFFu;
+    auto tile_info = [&](uint32_t tt) {
+        if (tw0 == 0xFFFFFFFFu || tt < tw0 || tt >= tw0 + 64) {
+            const uint32_t k = tt + (uint32_t)lane;
+            uint32_t v = 0;
+            if (k < sf.tile_end)
+                v = tstate[k].count | (tiles[k].first ? 0x80000000u : 0u);
+            wave_lds_sync();
+            ism.tinfo[lane] = v;
+            wave_lds_sync();
+            tw0 = tt;
+        }
+        return ism.tinfo[tt - tw0];
+    };
     auto load_batch = [&](uint32_t tt, uint32_t cc, uint32_t CC, uint32_t h[5]) {
         const uint32_t* cw = cand + (uint64_t)tt * 5 * SDBFX_CAND_CAP;
 #pragma unroll
@@ -1317,7 +1334,7 @@ __device__ __forceinline__ void stream_insert_body(
     // the first tile from tt on with candidates (sf.tile_end if none), and its count
     auto next_tile = [&](uint32_t tt, uint32_t& CC) {
         CC = 0;
-        while (tt < sf.tile_end && (CC = tstate[tt].count) == 0)
+        while (tt < sf.tile_end && (CC = tile_info(tt) & 0x7FFFFFFFu) == 0)
             tt++;
         return tt;
     };
@@ -1328,7 +1345,7 @@ __device__ __forceinline__ void stream_insert_body(
         load_batch(t, 0, C, hnext);
     while (t < sf.tile_end) {
         for (; applied <= t; applied++)
-            if (tiles[applied].first)
+            if (tile_info(applied) >> 31)
                 bigfi = 0;
         {
             const uint32_t cnt = min(64u, C - c0);
@@ -1361,21 +1378,28 @@ __device__ __forceinline__ void stream_insert_body(
             uint32_t s = 0;   // first candidate of the chunk not yet committed
             while (s < cnt) {
                 const bool act = (uint32_t)lane >= s && (uint32_t)lane < cnt;
-                // 1. new bits in the current filter
+                // 1. new bits in the current filter (a repeated position
+                // takes a minimum with ~0, which changes nothing)
                 if (act) {
 #pragma unroll
                     for (int j = 0; j < 5; j++)
-                        if (!((pdup >> j) & 1))
-                            atomicMin(&ism.own[p[j]], (uint32_t)lane);
+                        atomicMin(&ism.own[p[j]], (pdup >> j) & 1 ? 0xFFFFFFFFu : (uint32_t)lane);
                 }
                 wave_lds_sync();
+                // (the filter words and owners of all five positions are read
+                // together, so that their LDS latencies overlap)
                 uint32_t nnew = 0;
                 if (act) {
+                    uint32_t fv[5], ov[5];
+#pragma unroll
+                    for (int j = 0; j < 5; j++) {
+                        fv[j] = fw[p[j] >> 5];
+                        ov[j] = ism.own[p[j]];
+                    }
 #pragma unroll
                     for (int j = 0; j < 5; j++)
-                        if (!((pdup >> j) & 1) && !((fw[p[j] >> 5] >> (p[j] & 31)) & 1u) &&
-                            ism.own[p[j]] == (uint32_t)lane)
-                            nnew++;
+                        nnew += (!((pdup >> j) & 1) && !((fv[j] >> (p[j] & 31)) & 1u) && ov[j] == (uint32_t)lane)
+                                    ? 1u : 0u;
                 }
                 const bool reach = act && nnew > 0;
                 // 1b. the index gate for those with new bits: positions clear
@@ -1432,23 +1456,40 @@ __device__ __forceinline__ void stream_insert_body(
                 // lanes that go on to the de-duplication filter
                 const bool reach2 = INDEXED ? (reach && iclr != 0) : reach;
                 // 2. the de-duplication filter for those
+                // Every distinct position of the lane is marked in `seen` (and
+                // in `twice` when another lane marked it first), clear or not:
+                // whether a position was clear before the batch is the same
+                // for every lane, so marks on set positions never make a
+                // clear one look shared.  The five reads and the five marks
+                // are issued together (an OR with 0 for a repeated position).
                 uint32_t clr = 0;   // positions clear at the start of the batch
+                uint32_t mk[5];     // the bit each position marks (0: repeated)
                 if (reach2) {
+                    uint32_t bv[5], old[5];
 #pragma unroll
                     for (int j = 0; j < 5; j++)
-                        if (!((qdup >> j) & 1) && !((big[q[j] >> 5] >> (q[j] & 31)) & 1u)) {
+                        bv[j] = big[q[j] >> 5];
+#pragma unroll
+                    for (int j = 0; j < 5; j++) {
+                        mk[j] = (qdup >> j) & 1 ? 0u : 1u << (q[j] & 31);
+                        if (mk[j] && !(bv[j] & mk[j]))
                             clr |= 1u << j;
-                            const uint32_t bit = 1u << (q[j] & 31);
-                            if (atomicOr(&ism.seen[q[j] >> 5], bit) & bit)
-                                atomicOr(&ism.twice[q[j] >> 5], bit);
-                        }
+                        old[j] = atomicOr(&ism.seen[q[j] >> 5], mk[j]);
+                    }
+#pragma unroll
+                    for (int j = 0; j < 5; j++)
+                        atomicOr(&ism.twice[q[j] >> 5], old[j] & mk[j]);
                 }
                 wave_lds_sync();
                 uint32_t shared = 0;
                 if (reach2) {
+                    uint32_t tv[5];
 #pragma unroll
                     for (int j = 0; j < 5; j++)
-                        if (((clr >> j) & 1) && ((ism.twice[q[j] >> 5] >> (q[j] & 31)) & 1u))
+                        tv[j] = ism.twice[q[j] >> 5];
+#pragma unroll
+                    for (int j = 0; j < 5; j++)
+                        if (((clr >> j) & 1) && ((tv[j] >> (q[j] & 31)) & 1u))
                             shared |= 1u << j;
                 }
                 // a rejection decided inside the batch: replay it in order
@@ -1458,17 +1499,15 @@ __device__ __forceinline__ void stream_insert_body(
                 wave_lds_sync();
                 if (reach2) {
 #pragma unroll
-                    for (int j = 0; j < 5; j++)
-                        if ((clr >> j) & 1) {
-                            atomicAnd(&ism.seen[q[j] >> 5], ~(1u << (q[j] & 31)));
-                            atomicAnd(&ism.twice[q[j] >> 5], ~(1u << (q[j] & 31)));
-                        }
+                    for (int j = 0; j < 5; j++) {
+                        atomicAnd(&ism.seen[q[j] >> 5], ~mk[j]);
+                        atomicAnd(&ism.twice[q[j] >> 5], ~mk[j]);
+                    }
                 }
                 if (act) {
 #pragma unroll
                     for (int j = 0; j < 5; j++)
-                        if (!((pdup >> j) & 1))
-                            ism.own[p[j]] = 0xFFFFFFFFu;
+                        ism.own[p[j]] = 0xFFFFFFFFu;
                 }
                 wave_lds_sync();
                 if (replay) {
