@@ -791,53 +791,48 @@ __device__ __forceinline__ uint32_t tile_core(TileSmem& sm, const uint8_t* __res
     return sm.misc[1];
 }
 
-// Ordered compaction of owned positions [0, cnt) whose score satisfies pred
-// (pred(0) must be false) into sm.u.s.cand as (position | score << 16).  256 positions per chunk, four
-// per lane (one score word); each wave takes a contiguous range of chunks, so
-// one running offset per wave orders them.  A first pass counts each wave's
-// candidates from ballots, a second writes them: a lane's place within a
-// chunk is the number of candidates in the lanes below it (mbcnt over the
-// four byte ballots) plus its own earlier bytes.  Returns the count.
-template <typename Pred>
-__device__ __forceinline__ uint32_t compact(TileSmem& sm, uint32_t cnt, Pred pred)
+// Ordered compaction of the owned positions whose score is >= TH into
+// sm.u.s.cand as (position | score << 16); returns the count.  Every score
+// byte of the tile is read: bytes past the owned positions are 0 (the F
+// phase clears them and credits stay inside), which TH >= 1 rejects.  Each
+// wave takes 4096 positions as 4 chunks of 1024 in which lane l holds
+// positions 16 l .. 16 l + 15 (one 16-byte LDS load), so that the order
+// inside a chunk is lane order and one wave-wide scan per chunk places every
+// lane.  A score byte b (<= 64) is >= TH exactly when bit 7 of
+// b + 0x80 - TH is set, four bytes per 32-bit add.
+template <uint32_t TH>
+__device__ __forceinline__ uint32_t compact(TileSmem& sm)
 {
-    constexpr int MAXC = SDBFX_TILE / 256 / NWAVE;   // chunks per wave (cnt <= SDBFX_TILE)
+    static_assert(TH >= 1 && TH <= 64, "threshold");
+    constexpr uint32_t ADD = (0x80u - TH) * 0x01010101u;
+    constexpr int NCH = SDBFX_TILE / 1024 / NWAVE;
+    static_assert(NCH * 1024 * NWAVE == SDBFX_TILE, "chunks cover the tile");
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t nchunk = (cnt + 255) / 256;
-    const uint32_t per = (nchunk + NWAVE - 1) / NWAVE;
-    const uint32_t c0 = min((uint32_t)wave * per, nchunk), c1 = min(c0 + per, nchunk);
-    // the wave's score words, loaded once for both passes; bytes past the
-    // range read as score 0, which pred rejects for every caller
-    uint32_t word[MAXC];
+    uint4 sw[NCH];
 #pragma unroll
-    for (int k = 0; k < MAXC; k++) {
-        const uint32_t q0 = (c0 + k) * 256 + lane * 4;
-        const bool ok = c0 + k < c1 && q0 < cnt;
-        const uint32_t w = sm.u.s.score[ok ? q0 >> 2 : 0u];
-        const uint32_t rem = cnt - q0;
-        word[k] = ok ? (rem >= 4 ? w : w & ((1u << (8 * rem)) - 1)) : 0u;
+    for (int k = 0; k < NCH; k++)
+        sw[k] = ((const uint4*)sm.u.s.score)[(wave * NCH + k) * 64 + lane];   // words 4 l .. 4 l + 3 of the chunk
+    // per chunk: the lane's 16 flags (bit 4j + i: byte i of word j, the
+    // position order), its place among the chunk's kept positions and the
+    // chunk's count
+    uint32_t fl[NCH], pre[NCH], ctot[NCH], mine = 0;
+#pragma unroll
+    for (int k = 0; k < NCH; k++) {
+        const uint32_t w[4] = {sw[k].x, sw[k].y, sw[k].z, sw[k].w};
+        uint32_t f = 0, c = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t t = (w[j] + ADD) & 0x80808080u;
+            c = (uint32_t)__popc(t) + c;
+            // bits 7, 15, 23, 31 to bits 21..24 (distinct partial products)
+            f |= ((((t >> 7) * 0x00204081u) >> 21) & 0xFu) << (4 * j);
+        }
+        fl[k] = f;
+        pre[k] = wave_excl_sum(c, lane, &ctot[k]);
+        mine += ctot[k];
     }
-    auto ballots = [&](int k, unsigned long long b[4]) {
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-            b[j] = __ballot(pred((word[k] >> (8 * j)) & 0xFFu));
-    };
-    uint32_t mine = 0;
-#pragma unroll
-    for (int k = 0; k < MAXC; k++) {
-        unsigned long long b[4];
-        ballots(k, b);
-        mine += __popcll(b[0]) + __popcll(b[1]) + __popcll(b[2]) + __popcll(b[3]);
-        __builtin_amdgcn_sched_barrier(0);   // one chunk's ballots live at a time
-    }
-    opaque(mine);   // counted here, not inside the branch that stores it
     if (lane == 0)
         sm.chunk_cnt[wave] = mine;
-    // the second pass takes its ballots again rather than keeping 4 * MAXC
-    // of them in scalar registers
-#pragma unroll
-    for (int k = 0; k < MAXC; k++)
-        opaque(word[k]);
     __syncthreads();
     uint32_t o = 0, total = 0;
 #pragma unroll
@@ -847,24 +842,59 @@ __device__ __forceinline__ uint32_t compact(TileSmem& sm, uint32_t cnt, Pred pre
         total += t;
     }
 #pragma unroll
-    for (int k = 0; k < MAXC; k++) {
-        unsigned long long b[4];
-        ballots(k, b);
-        uint32_t at = o + lanes_below(b[0]) + lanes_below(b[1]) + lanes_below(b[2]) + lanes_below(b[3]);
-        const uint32_t q0 = (c0 + k) * 256 + lane * 4;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            if ((b[j] >> lane) & 1)
-                sm.u.s.cand[at++] = (q0 + j) | (((word[k] >> (8 * j)) & 0xFFu) << 16);
+    for (int k = 0; k < NCH; k++) {
+        const uint32_t q0 = (uint32_t)(wave * NCH + k) * 1024 + (uint32_t)lane * 16;
+        uint32_t at = o + pre[k], f = fl[k];
+        while (f) {
+            const uint32_t bit = (uint32_t)__builtin_ctz(f);
+            const uint32_t j = bit >> 2;
+            const uint32_t wv = j == 0 ? sw[k].x : j == 1 ? sw[k].y : j == 2 ? sw[k].z : sw[k].w;
+            sm.u.s.cand[at++] = (q0 + bit) | (((wv >> (8 * (bit & 3))) & 0xFFu) << 16);
+            f &= f - 1;
         }
-        o += __popcll(b[0]) + __popcll(b[1]) + __popcll(b[2]) + __popcll(b[3]);
-        __builtin_amdgcn_sched_barrier(0);
+        o += ctot[k];
     }
     __syncthreads();
     return total;
 }
 
 // ------------------------------------------------------------------ block mode
+
+// Keeps the entries of sm.u.s.cand[0, n) whose score is >= k (k > 0), in
+// order, at the front of the list; returns their number.  Each thread takes
+// BCAND_CAP / SDBFX_NT consecutive entries (a stride coprime with the 64
+// banks); all are read before any is written.
+__device__ __forceinline__ uint32_t keep_cand(TileSmem& sm, uint32_t n, uint32_t k)
+{
+    constexpr int PER = (BCAND_CAP + SDBFX_NT - 1) / SDBFX_NT;
+    static_assert(PER % 2 == 1, "entries per thread must be odd (bank stride)");
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t v[PER], cnt = 0;
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const uint32_t i = (uint32_t)tid * PER + j;
+        v[j] = i < n ? sm.u.s.cand[i] : 0u;   // score 0 past the list: not kept
+        cnt += (v[j] >> 16) >= k ? 1u : 0u;
+    }
+    uint32_t tot;
+    uint32_t o = wave_excl_sum(cnt, lane, &tot);
+    if (lane == 0)
+        sm.chunk_cnt[wave] = tot;
+    __syncthreads();
+    uint32_t total = 0;
+#pragma unroll
+    for (int w = 0; w < NWAVE; w++) {
+        const uint32_t t = sm.chunk_cnt[w];
+        o += w < wave ? t : 0u;
+        total += t;
+    }
+#pragma unroll
+    for (int j = 0; j < PER; j++)
+        if ((v[j] >> 16) >= k)
+            sm.u.s.cand[o++] = v[j];
+    __syncthreads();
+    return total;
+}
 
 // One workgroup per dd-block (sdbf_core.cc:252-282 per block, :305-319 tail).
 // With feat (not null), the SHA-1 words of the block's counted features, the
@@ -884,20 +914,16 @@ __global__ __launch_bounds__(SDBFX_NT, 2) void block_digest_kernel(
 
     tile_core(sm, u, n, 0, T, -1, 0xFFFFFFFFu);
 
-    // histogram of scores >= 16 over positions [0, n-64)
+    // The positions of [0, n-64) with score >= 15, the lowest threshold the
+    // walk below can settle on, in order; then the histogram of their scores
+    // >= 16 (the walk's counts) from that list.
     if (tid < 68)
         sm.histo[tid] = 0;
-    __syncthreads();
-    for (uint32_t w = tid; w * 4 < T; w += SDBFX_NT) {
-        const uint32_t word = sm.u.s.score[w];
-        if (word & 0xF0F0F0F0u) {   // some byte >= 16
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const uint32_t sc = (word >> (8 * j)) & 0xFFu;
-                if (w * 4 + j < T && sc >= SDBFX_THRESHOLD)
-                    atomicAdd(&sm.histo[sc], 1u);
-            }
-        }
+    const uint32_t C15 = compact<SDBFX_THRESHOLD - 1>(sm);
+    for (uint32_t i = tid; i < C15; i += SDBFX_NT) {
+        const uint32_t sc = sm.u.s.cand[i] >> 16;
+        if (sc >= SDBFX_THRESHOLD)
+            atomicAdd(&sm.histo[sc], 1u);
     }
     __syncthreads();
     if (wave == 0) {
@@ -931,7 +957,7 @@ __global__ __launch_bounds__(SDBFX_NT, 2) void block_digest_kernel(
     __syncthreads();
     const uint32_t k = sm.misc[3];
     int32_t allowed = (int32_t)sm.misc[4];
-    const uint32_t C = compact(sm, T, [k](uint32_t sc) { return sc >= k; });
+    const uint32_t C = keep_cand(sm, C15, k);
 
     // Ordered insertion (gen_block_hash, sdbf_core.cc:157-183), 64 candidates
     // per step on wave 0.  Each bit of a batch is owned by the first selected
@@ -1086,7 +1112,7 @@ __device__ __forceinline__ void stream_tile(TileSmem& sm, const uint8_t* __restr
     if (tid == 0)
         sm.misc[2] = 0;
     const uint32_t cert = tile_core(sm, u, td.n, td.t0, td.T, forced, hand_i);
-    const uint32_t C = compact(sm, td.T, [](uint32_t sc) { return sc > SDBFX_THRESHOLD; });
+    const uint32_t C = compact<SDBFX_THRESHOLD + 1>(sm);
     uint32_t* cw = cand + (uint64_t)tile * 5 * SDBFX_CAND_CAP;
     for (uint32_t c = tid; c < C; c += SDBFX_NT) {
         uint32_t h[5];
