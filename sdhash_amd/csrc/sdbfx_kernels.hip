@@ -292,8 +292,10 @@ __device__ __forceinline__ uint32_t select_bit(unsigned long long m, uint32_t r)
 
 // Runs the score state machine over steps [i0, i1) from state m (i0 - base
 // is a multiple of 4).  With CREDIT each step adds SIGN to the score of the
-// position it elects, when that position is owned ([t0, t0+T)); consecutive
-// credits to one position are added at once.  With HAND, the canonical state
+// position it elects, when that position is owned ([t0, t0+T)), with one LDS
+// atomic per step (adding a run of equal positions at once needs a branch on
+// the change of position, and some lane of the wave takes it at most steps).
+// With HAND, the canonical state
 // before step hand_i+1 is stored to sm.misc[2] when step hand_i runs.  Returns
 // the state after step i1-1.  The words a step reads do not depend on the
 // state, so they are loaded ahead of the transitions: the two words of each
@@ -308,12 +310,6 @@ __device__ __forceinline__ uint32_t sm_run(TileSmem& sm, uint32_t base, uint32_t
         return m;
     const uint16_t* P = sm.P;
     uint32_t rm = m ? rank_at(P, base, m) : 0u;
-    uint32_t cpos = 0xFFFFFFFFu, ccnt = 0;
-    auto flush = [&]() {
-        const uint32_t rel = cpos - t0;
-        if (ccnt && cpos >= t0 && rel < T)
-            atomicAdd(&sm.u.s.score[rel >> 2], ((uint32_t)SIGN * ccnt) << (8 * (rel & 3)));
-    };
     const uint32_t ilast = i0 + ((i1 - i0 - 1) & ~3u);   // first step of the last group
     auto words = [&](uint32_t i, uint64_t& W, uint64_t& R) {
         const uint32_t x = pix(i - base);
@@ -341,12 +337,9 @@ __device__ __forceinline__ uint32_t sm_run(TileSmem& sm, uint32_t base, uint32_t
                 const bool has = cheap || rf != 0;
                 rm = cheap ? rm : rf;
                 if (CREDIT && has) {
-                    if (m != cpos) {
-                        flush();
-                        cpos = m;
-                        ccnt = 0;
-                    }
-                    ccnt++;
+                    const uint32_t rel = m - t0;
+                    if (m >= t0 && rel < T)
+                        atomicAdd(&sm.u.s.score[rel >> 2], ((uint32_t)SIGN) << (8 * (rel & 3)));
                 }
                 if (HAND && ii == hand_i)
                     sm.misc[2] = canon(P, base, ii + 1, m);
@@ -391,8 +384,6 @@ __device__ __forceinline__ uint32_t sm_run(TileSmem& sm, uint32_t base, uint32_t
             rfa[j] = rfb[j];
     }
     group(i, W0, R0, rfa, i1 - i);
-    if (CREDIT)
-        flush();
     return m;
 }
 
