@@ -488,9 +488,15 @@ __device__ __forceinline__ uint32_t tile_core(TileSmem& sm, const uint8_t* __res
     // ---- 1. ranks: each lane rolls a byte histogram over a run of at most
     // RUN_CHUNKS * 16 windows that starts at a multiple of 16; the run's bytes
     // and the 64 after it are loaded into registers before the first use.
-    // The count of byte value b for lane t is byte b & 3 of word
-    // (b >> 2) * RANK_LANES + t, so the lanes of a wave always touch 64
-    // different banks.  Counts change by word atomics that return the word
+    // The count of byte value b for lane t is byte t & 3 of word
+    // b * RANK_LANES / 4 + t / 4: the increment and the byte a lane reads
+    // back are fixed per lane and the address is one multiply-add (a layout
+    // with the byte chosen by b & 3 keeps every lane of a wave on its own
+    // bank but needs four more instructions per update, and the ranks phase
+    // is bound by instructions).  Lanes sharing a word change different
+    // bytes, and no count passes 64, so the bytes never carry into each
+    // other, whatever order the LDS applies one instruction's lanes in.
+    // Counts change by word atomics that return the word
     // before the change: the LDS applies one wave's operations in order, so
     // the 32 updates of a 16-window chunk are all in flight at once and each
     // still sees every update before it.  Removing the outgoing byte before
@@ -537,7 +543,8 @@ __device__ __forceinline__ uint32_t tile_core(TileSmem& sm, const uint8_t* __res
     __syncthreads();
 
     if (rank_lane) {
-        uint32_t* H = (uint32_t*)sm.u.hist + rl;
+        uint32_t* H = (uint32_t*)sm.u.hist + (rl >> 2);
+        const uint32_t hsh = 8 * (rl & 3), hinc = 1u << hsh;   // this lane's byte of a word
         if (p0 < pv) {
             // the first window's 64 increments (in two halves) and its
             // entropy from the counts they returned
@@ -546,23 +553,19 @@ __device__ __forceinline__ uint32_t tile_core(TileSmem& sm, const uint8_t* __res
             for (int h = 0; h < 2; h++) {
                 uint32_t w[32];
 #pragma unroll
-                for (int k = 0; k < 32; k++) {
-                    const uint32_t b = byte_of(X[2 * h + (k >> 4)], k & 15);
-                    w[k] = atomicAdd(H + (b >> 2) * RANK_LANES, 1u << (8 * (b & 3)));
-                }
+                for (int k = 0; k < 32; k++)
+                    w[k] = atomicAdd(H + byte_of(X[2 * h + (k >> 4)], k & 15) * (RANK_LANES / 4), hinc);
 #pragma unroll
-                for (int k = 0; k < 32; k++) {
-                    const uint32_t b = byte_of(X[2 * h + (k >> 4)], k & 15);
-                    e += sm.dtab[((w[k] >> (8 * (b & 3))) & 0xFFu) + 1];
-                }
+                for (int k = 0; k < 32; k++)
+                    e += sm.dtab[((w[k] >> hsh) & 0xFFu) + 1];
             }
             auto update = [&](int c, uint32_t* wo_, uint32_t* wi_) {
 #pragma unroll
                 for (int j = (c == 0); j < 16; j++) {
                     const uint32_t bo = j ? byte_of(X[c], j - 1) : byte_of(X[c - 1], 15);
                     const uint32_t bi = j ? byte_of(X[c + 4], j - 1) : byte_of(X[c + 3], 15);
-                    wo_[j] = atomicSub(H + (bo >> 2) * RANK_LANES, 1u << (8 * (bo & 3)));
-                    wi_[j] = atomicAdd(H + (bi >> 2) * RANK_LANES, 1u << (8 * (bi & 3)));
+                    wo_[j] = atomicSub(H + bo * (RANK_LANES / 4), hinc);
+                    wi_[j] = atomicAdd(H + bi * (RANK_LANES / 4), hinc);
                 }
             };
             uint16_t* Pc = sm.P + pix(r0);
@@ -574,12 +577,8 @@ __device__ __forceinline__ uint32_t tile_core(TileSmem& sm, const uint8_t* __res
                     uint32_t rk[8];
 #pragma unroll
                     for (int j = 0; j < 16; j++) {
-                        if (c > 0 || j > 0) {
-                            const uint32_t bo = j ? byte_of(X[c], j - 1) : byte_of(X[c - 1], 15);
-                            const uint32_t bi = j ? byte_of(X[c + 4], j - 1) : byte_of(X[c + 3], 15);
-                            e += sm.dtab[((wi_[j] >> (8 * (bi & 3))) & 0xFFu) + 1] -
-                                 sm.dtab[(wo_[j] >> (8 * (bo & 3))) & 0xFFu];
-                        }
+                        if (c > 0 || j > 0)
+                            e += sm.dtab[((wi_[j] >> hsh) & 0xFFu) + 1] - sm.dtab[(wo_[j] >> hsh) & 0xFFu];
                         // windows from pv on get their rank 0 below
                         const uint32_t rv = sm.rtab[e >> 10];
                         rk[j >> 1] = (j & 1) ? (rk[j >> 1] | rv << 16) : rv;
