@@ -765,14 +765,17 @@ def main() -> int:
     seg_off = [i * SEGMENT for i in range(nseg)]
     seg_len = [SEGMENT] * nseg
     seg_names = [b"image.%04dM" % (i * SEGMENT // MIB) for i in range(nseg)]
-    fr_off = [i * fpitch for i in range(NFRAG)]
-    fr_names = [b"frag%04d" % i for i in range(NFRAG)]
+    # the fragments' offsets, lengths and names as the arrays the library
+    # takes, built once rather than converted from Python lists every step
+    fr_off = np.arange(NFRAG, dtype=np.uint64) * fpitch
+    fr_len = np.full(NFRAG, FRAG, np.uint64)
+    fr_names = sdhash_amd.name_array([b"frag%04d" % i for i in range(NFRAG)])
 
     def step():
         t = eng.new_set()
         st = t.digest_device(dimg, seg_off, seg_len, seg_names, dd_block=16384)
         q = eng.new_set()
-        sq = q.digest_device(dfr, fr_off, [FRAG] * NFRAG, fr_names, dd_block=0)
+        sq = q.digest_device(dfr, fr_off, fr_len, fr_names, dd_block=0)
         scores = eng.compare_to(q, t)
         return t, q, scores, st, sq
 

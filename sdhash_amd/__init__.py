@@ -153,6 +153,14 @@ def _ptr(a: np.ndarray):
     return C.c_void_p(a.ctypes.data)
 
 
+def name_array(names) -> C.Array:
+    """The digest calls' name list as the C array they pass to the library.
+    Built once, it can be handed to many digest calls over the same inputs
+    in place of a list (the array holds the name bytes)."""
+    keep = [x if isinstance(x, bytes) else str(x).encode() for x in names]
+    return (C.c_char_p * max(len(keep), 1))(*keep)
+
+
 def _bytes_array(items) -> tuple[list, C.Array]:
     arrs = [np.ascontiguousarray(np.frombuffer(x, np.uint8) if isinstance(x, (bytes, bytearray))
                                  else x, dtype=np.uint8) for x in items]
@@ -333,6 +341,11 @@ class DigestSet:
         return np.ascontiguousarray(dd_block, np.uint32)
 
     def _names(self, names, n):
+        if isinstance(names, C.Array):   # from name_array(): passed as is
+            if len(names) < n:
+                raise ValueError("name array holds %d names for %d inputs" % (len(names), n))
+            self._keep = names
+            return names
         if names is None:
             names = [b"f%06d" % i for i in range(n)]
         self._keep = [x if isinstance(x, bytes) else str(x).encode() for x in names]

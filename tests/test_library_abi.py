@@ -35,3 +35,22 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     except sdhash_amd.SdbfxError:
         return
     raise AssertionError("load() must raise without the library")
+
+
+def test_name_array_is_passed_as_built():
+    """A prebuilt name array goes to the library as is; a list is converted;
+    an array with fewer names than inputs is refused."""
+    import types
+    names = [b"frag%04d" % i for i in range(5)]
+    arr = sdhash_amd.name_array(names)
+    holder = types.SimpleNamespace()
+    assert sdhash_amd.DigestSet._names(holder, arr, 5) is arr
+    assert [arr[i] for i in range(5)] == names
+    conv = sdhash_amd.DigestSet._names(holder, names, 5)
+    assert [conv[i] for i in range(5)] == names
+    try:
+        sdhash_amd.DigestSet._names(holder, arr, 6)
+    except ValueError:
+        pass
+    else:
+        raise AssertionError("a short name array was accepted")
